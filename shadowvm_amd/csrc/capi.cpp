@@ -1,0 +1,245 @@
+// capi.cpp — extern "C" entry points of libsvm.so (declared in include/svm.h).
+#include <cstring>
+
+#include "exec.h"
+#include "svm_internal.h"
+
+namespace svm {
+int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs);
+int server_start(int max_inflight, uint64_t budget);
+int server_submit(const uint8_t* pb, size_t len, uint64_t* ticket);
+int server_wait(uint64_t ticket, int timeout_ms, svm_result** out);
+int server_stop();
+}  // namespace svm
+
+using namespace svm;
+
+namespace {
+thread_local std::string t_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    t_err.clear();
+    return SVM_OK;
+  } catch (const Error& e) {
+    t_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    t_err = "host allocation failed";
+    return SVM_ERR_NO_MEMORY;
+  } catch (const std::exception& e) {
+    t_err = e.what();
+    return SVM_ERR_INVALID;
+  }
+}
+
+void need_id(const char* id) {
+  if (!id || !*id) fail(SVM_ERR_INVALID, "table id must be a non-empty string");
+}
+}  // namespace
+
+extern "C" {
+
+const char* svm_last_error(void) { return t_err.c_str(); }
+const char* svm_version(void) { return "shadowvm_amd 0.1 (gfx950)"; }
+
+int svm_init(const svm_config* cfg) {
+  return guard([&] {
+    Config c;
+    if (cfg) {
+      c.device = cfg->device;
+      if (cfg->tuple_per_bulk) c.tuple_per_bulk = cfg->tuple_per_bulk;
+      if (cfg->hbm_budget_bytes) c.hbm_budget = cfg->hbm_budget_bytes;
+      if (cfg->num_streams > 0) c.num_streams = cfg->num_streams;
+      c.verbose = cfg->verbose;
+    }
+    engine_init(c);
+  });
+}
+
+int svm_shutdown(void) {
+  return guard([&] {
+    server_stop();
+    engine_shutdown();
+  });
+}
+
+int svm_table_create(const char* id, int ncols, const int32_t* types, const int64_t* val_sizes,
+                     uint64_t rows) {
+  return guard([&] {
+    need_id(id);
+    if (ncols <= 0 || ncols > 24 || !types || !val_sizes)
+      fail(SVM_ERR_INVALID, "a table has 1..24 columns (MaxColumnNum, com/cpp/define.hpp:25)");
+    std::vector<std::pair<int, int64_t>> s;
+    for (int c = 0; c < ncols; ++c) {
+      if (types[c] < 0 || types[c] > 8) fail(SVM_ERR_INVALID, "unknown column type");
+      s.push_back({types[c], val_sizes[c]});
+    }
+    Engine& E = engine();
+    TablePtr t = E.store.create(id, s, rows);
+    t->max_tuple_count = E.cfg.tuple_per_bulk;
+    E.store.put(id, t);
+  });
+}
+
+int svm_table_put_column(const char* id, int col, const void* host_src, uint64_t bytes) {
+  return guard([&] {
+    need_id(id);
+    TablePtr t = engine().store.get(id);
+    if (col < 0 || col >= (int)t->cols.size()) fail(SVM_ERR_INVALID, "column index out of range");
+    Column& c = t->cols[col];
+    uint64_t need = t->rows * (uint64_t)c.size;
+    if (bytes != need) fail(SVM_ERR_INVALID, "column needs " + std::to_string(need) + " bytes, got " + std::to_string(bytes));
+    if (!c.dev) fail(SVM_ERR_INVALID, "column is not materialised");
+    if (bytes) SVM_HIP(hipMemcpy(c.dev, host_src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int svm_table_get_column(const char* id, int col, void* host_dst, uint64_t bytes) {
+  return guard([&] {
+    need_id(id);
+    TablePtr t = engine().store.get(id);
+    if (col < 0 || col >= (int)t->cols.size()) fail(SVM_ERR_INVALID, "column index out of range");
+    Column& c = t->cols[col];
+    uint64_t have = t->rows * (uint64_t)c.size;
+    if (bytes > have) fail(SVM_ERR_INVALID, "column holds only " + std::to_string(have) + " bytes");
+    if (!c.dev) fail(SVM_ERR_INVALID, "column is not materialised");
+    if (bytes) SVM_HIP(hipMemcpy(host_dst, c.dev, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int svm_table_column_device(const char* id, int col, void** dev_ptr) {
+  return guard([&] {
+    need_id(id);
+    TablePtr t = engine().store.get(id);
+    if (col < 0 || col >= (int)t->cols.size() || !dev_ptr) fail(SVM_ERR_INVALID, "bad column");
+    *dev_ptr = t->cols[col].dev;
+  });
+}
+
+int svm_table_info(const char* id, uint64_t* rows, int* ncols) {
+  return guard([&] {
+    need_id(id);
+    TablePtr t = engine().store.get(id);
+    if (rows) *rows = t->rows;
+    if (ncols) *ncols = (int)t->cols.size();
+  });
+}
+
+int svm_table_column_type(const char* id, int col, int32_t* type, int64_t* val_size) {
+  return guard([&] {
+    need_id(id);
+    TablePtr t = engine().store.get(id);
+    if (col < 0 || col >= (int)t->cols.size()) fail(SVM_ERR_INVALID, "column index out of range");
+    if (type) *type = t->cols[col].type;
+    if (val_size) *val_size = t->cols[col].size;
+  });
+}
+
+int svm_table_drop(const char* id) {
+  return guard([&] {
+    need_id(id);
+    if (!engine().store.drop(id)) fail(SVM_ERR_NOT_FOUND, std::string("no table '") + id + "'");
+  });
+}
+
+int svm_table_exists(const char* id) {
+  if (!id || !engine_ready()) return 0;
+  return engine().store.find(id) ? 1 : 0;
+}
+
+int svm_gen_table(const char* id, int kind, uint64_t rows, uint64_t seed, const svm_gen_params* p) {
+  return guard([&] {
+    need_id(id);
+    gen_device_table(id, kind, rows, seed, p);
+  });
+}
+
+int svm_gen_host(int kind, uint64_t seed, const svm_gen_params* p, uint64_t row0, uint64_t n,
+                 void* const* cols, int threads) {
+  return guard([&] {
+    if (!cols) fail(SVM_ERR_INVALID, "null column array");
+    gen_host(kind, seed, p, row0, n, cols, threads);
+  });
+}
+
+int svm_gen_schema(int kind, int32_t* types, int64_t* val_sizes) {
+  int n = 0;
+  int rc = guard([&] {
+    auto s = gen_schema(kind);
+    n = (int)s.size();
+    for (int c = 0; c < n; ++c) {
+      if (types) types[c] = s[c].first;
+      if (val_sizes) val_sizes[c] = s[c].second;
+    }
+  });
+  return rc == SVM_OK ? n : rc;
+}
+
+int svm_pipeline_run(const uint8_t* pb, size_t len, svm_result** out) {
+  return guard([&] {
+    if (out) *out = nullptr;
+    PipelineInfo info = decode_pipeline_info(pb, len);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> g(E.run_mu);
+    RunOptions opt;
+    opt.want_host = out != nullptr;
+    auto r = run_pipeline(info, opt);
+    if (out) *out = r.release();
+  });
+}
+
+int svm_result_kind(const svm_result* r) { return r ? r->kind : -1; }
+int64_t svm_result_rows(const svm_result* r) { return r ? r->rows : -1; }
+int svm_result_cols(const svm_result* r) { return r ? (int)r->types.size() : -1; }
+
+int svm_result_col_type(const svm_result* r, int col, int32_t* type, int64_t* val_size) {
+  return guard([&] {
+    if (!r || col < 0 || col >= (int)r->types.size()) fail(SVM_ERR_INVALID, "bad result column");
+    if (type) *type = r->types[col].first;
+    if (val_size) *val_size = r->types[col].second;
+  });
+}
+
+const void* svm_result_col(const svm_result* r, int col) {
+  if (!r || col < 0 || col >= (int)r->cols.size()) return nullptr;
+  return r->cols[col].data();
+}
+
+int svm_result_timing(const svm_result* r, double* build_ms, double* scan_ms, double* finish_ms) {
+  return guard([&] {
+    if (!r) fail(SVM_ERR_INVALID, "null result");
+    if (build_ms) *build_ms = r->build_ms;
+    if (scan_ms) *scan_ms = r->scan_ms;
+    if (finish_ms) *finish_ms = r->finish_ms;
+  });
+}
+
+void svm_result_free(svm_result* r) { delete r; }
+
+int svm_batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs) {
+  int rc = SVM_OK;
+  int g = guard([&] { rc = batch_run(n, plans, lens, outs); });
+  return g != SVM_OK ? g : rc;
+}
+
+int svm_server_start(int max_inflight, uint64_t budget_bytes) {
+  return guard([&] { server_start(max_inflight, budget_bytes); });
+}
+int svm_server_submit(const uint8_t* pb, size_t len, uint64_t* ticket) {
+  int rc = SVM_OK;
+  int g = guard([&] { rc = server_submit(pb, len, ticket); });
+  return g != SVM_OK ? g : rc;
+}
+int svm_server_wait(uint64_t ticket, int timeout_ms, svm_result** out) {
+  int rc = SVM_OK;
+  int g = guard([&] { rc = server_wait(ticket, timeout_ms, out); });
+  return g != SVM_OK ? g : rc;
+}
+int svm_server_stop(void) {
+  return guard([&] { server_stop(); });
+}
+
+}  // extern "C"

@@ -1,0 +1,341 @@
+// codegen.cpp — emits the HIP source of the fused per-pipeline kernels.
+//
+// Where the reference emits one thread per row that probes a host-built Hash31
+// table and writes unreduced key/metric rows for a host sort (codegen/template.cpp:77-89,
+// codegen/codegen.cpp:100-229), the kernels here are persistent grid-stride scans
+// over HBM-resident int32 columns:
+//   * 128-bit non-temporal loads of 4 rows per column per thread;
+//   * filter predicates inlined as C int expressions (the reference language);
+//   * each join a lookup in a bit-packed dense map (optionally staged in LDS)
+//     whose slot holds existence, the group code of the dimension row, or its row id;
+//   * group-by into a per-workgroup LDS table (dense array, or open-addressing hash
+//     for large group spaces) merged into HBM with 64-bit atomics at the end;
+//   * select pipelines compact passing rows in row order with wave64 ballots.
+#include "codegen.h"
+
+#include <sstream>
+
+#include "kargs.h"
+
+namespace svm {
+
+static const char* kDeviceRT = R"SVMRT(
+typedef unsigned long long svm_u64;
+typedef long long svm_i64;
+typedef int svm_i32x4 __attribute__((ext_vector_type(4)));
+
+/* Hash31 over n bytes read as signed char, uint32 wrap, top bit cleared
+   (reference com/cpp/cuda-runtime.cu:64-71). */
+__device__ __forceinline__ unsigned Hash(const char* p, int n) {
+  unsigned h = 0;
+  for (int k = 0; k < n; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
+  return h & 0x7FFFFFFFu;
+}
+__device__ __forceinline__ unsigned StrHash(const char* p, int n) {
+  unsigned h = 0;
+  for (int k = 0; k < n && p[k] != '\0'; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
+  return h & 0x7FFFFFFFu;
+}
+/* Prefix equality: 0 once either side reaches NUL (com/cpp/cuda-runtime.cu:34-54). */
+__device__ __forceinline__ int StrCmp(const char* a, const char* b, int n) {
+  for (int k = 0; k < n; ++k) {
+    char x = a[k], y = b[k];
+    if (x == '\0' || y == '\0') return 0;
+    if (x != y) return (int)(signed char)x - (int)(signed char)y;
+  }
+  return 0;
+}
+__device__ __forceinline__ void StrCopy(char* d, const char* s, int n) {
+  for (int k = 0; k < n; ++k) d[k] = s[k];
+}
+
+template <int W>
+__device__ __forceinline__ unsigned svm_map_get(const unsigned* m, unsigned off) {
+  if (W == 32) return m[off];
+  constexpr int E = 32 / W;
+  constexpr int S = W == 1 ? 5 : W == 2 ? 4 : W == 4 ? 3 : W == 8 ? 2 : 1;
+  return (m[off >> S] >> ((off & (unsigned)(E - 1)) * (unsigned)W)) & ((1u << W) - 1u);
+}
+
+#define PI(k) (a.pi[k])
+#define PS(off) ((const char*)(a.sblob + (off)))
+)SVMRT";
+
+const char* device_runtime_text() { return kDeviceRT; }
+
+namespace {
+
+std::string I(int v) { return std::to_string(v); }
+
+void emit_prologue(std::ostringstream& o, const KernelShape& s) {
+  o << SVM_KARGS_TEXT << "\n" << kDeviceRT << "\n";
+  o << "#define SVM_BLOCK " << s.block << "\n";
+}
+
+// Column pointer macros used by lowered expressions.
+void emit_colptrs(std::ostringstream& o, const KernelShape& s) {
+  for (int c : s.fact_other)
+    o << "  const char* FCOL" << c << " = (const char*)a.ptr[" << c << "];\n";
+  for (auto& kv : s.dcol_slot)
+    o << "  const char* DCOL" << kv.first.first << "_" << kv.first.second
+      << " = (const char*)a.ptr[" << kv.second << "];\n";
+}
+
+void emit_map_ptrs(std::ostringstream& o, const KernelShape& s) {
+  for (size_t j = 0; j < s.joins.size(); ++j) {
+    const JoinShape& J = s.joins[j];
+    if (J.lds)
+      o << "  const unsigned* m" << j << " = svm_lds + a.iv[" << 4 * j + 3 << "];\n";
+    else
+      o << "  const unsigned* m" << j << " = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "];\n";
+  }
+}
+
+void emit_stage_maps(std::ostringstream& o, const KernelShape& s) {
+  bool any = false;
+  for (size_t j = 0; j < s.joins.size(); ++j) {
+    if (!s.joins[j].lds) continue;
+    any = true;
+    o << "  { const unsigned* g = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "];\n"
+      << "    unsigned* d = svm_lds + a.iv[" << 4 * j + 3 << "];\n"
+      << "    for (int w = threadIdx.x; w < a.iv[" << 4 * j + 2 << "]; w += SVM_BLOCK) d[w] = g[w]; }\n";
+  }
+  (void)any;
+}
+
+// Filter, then every join in plan order (each guarded by the running condition,
+// as GenJoinBlock guards Probe with if(cond), codegen/codegen.cpp:141-145).
+void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std::string& ind) {
+  o << ind << "if (cond) cond = (" << s.filter << ");\n";
+  for (size_t j = 0; j < s.joins.size(); ++j) {
+    const JoinShape& J = s.joins[j];
+    o << ind << "unsigned p" << j << " = 0u;\n"
+      << ind << "if (cond) {\n"
+      << ind << "  const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+      << ind << "  if (off < (unsigned)a.iv[" << 4 * j + 1 << "]) p" << j << " = svm_map_get<"
+      << J.wbits << ">(m" << j << ", off);\n"
+      << ind << "  cond = p" << j << " != 0u;\n"
+      << ind << "}\n";
+    if (J.payload == PAY_ROW)
+      o << ind << "const long long JI" << J.robj << " = (long long)p" << j << " - 1;\n";
+  }
+}
+
+}  // namespace
+
+std::string gen_scan_source(const KernelShape& s) {
+  std::ostringstream o;
+  emit_prologue(o, s);
+  const int M = (int)s.metrics.size();
+  const int U = s.unroll < 1 ? 1 : s.unroll;
+  o << "#define SVM_M " << M << "\n";
+  if (s.agg == AGG_HASH) {
+    o << "#define SVM_HSLOTS " << (1 << s.hash_bits) << "\n";
+    o << "#define SVM_HBITS " << s.hash_bits << "\n";
+  }
+  o << "extern \"C\" __global__ void __launch_bounds__(SVM_BLOCK) svm_scan(SvmArgs a) {\n";
+  o << "  extern __shared__ svm_u64 svm_lds64[];\n  unsigned* svm_lds = (unsigned*)svm_lds64;\n";
+  emit_colptrs(o, s);
+  emit_stage_maps(o, s);
+  emit_map_ptrs(o, s);
+  o << "  const unsigned G = (unsigned)a.iv[44];\n  (void)G;\n";
+  if (s.agg == AGG_DENSE) {
+    o << "  svm_u64* lsum = (svm_u64*)(svm_lds + a.iv[45]);\n"
+      << "  unsigned* lflag = (unsigned*)(lsum + (size_t)G * SVM_M);\n"
+      << "  for (unsigned g = threadIdx.x; g < G * SVM_M; g += SVM_BLOCK) lsum[g] = 0ull;\n"
+      << "  for (unsigned w = threadIdx.x; w < (G + 31u) / 32u; w += SVM_BLOCK) lflag[w] = 0u;\n";
+  } else if (s.agg == AGG_HASH) {
+    o << "  unsigned* hkey = svm_lds + a.iv[45];\n"
+      << "  svm_u64* hsum = (svm_u64*)(hkey + SVM_HSLOTS);\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS; q += SVM_BLOCK) hkey[q] = 0u;\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS * SVM_M; q += SVM_BLOCK) hsum[q] = 0ull;\n";
+  }
+  o << "  svm_u64* gsum = (svm_u64*)a.out[0];\n  unsigned* gflag = (unsigned*)a.out[1];\n"
+    << "  (void)gsum; (void)gflag;\n";
+  o << "  __syncthreads();\n";
+  if (s.agg == AGG_SCALAR)
+    for (int m = 0; m < M; ++m) o << "  svm_i64 acc" << m << " = 0;\n";
+  o << "  const long long r_lo = a.row0, r_hi = a.row0 + a.rows;\n"
+    << "  const long long v_lo = r_lo >> 2, v_hi = (r_hi + 3) >> 2;\n"
+    << "  const long long GT = (long long)gridDim.x * SVM_BLOCK;\n"
+    << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
+    << U << " * GT) {\n";
+  for (int u = 0; u < U; ++u) {
+    o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
+      << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+    for (int c : s.fact_i32)
+      o << "    const svm_i32x4 x" << c << "_" << u
+        << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+  }
+  for (int u = 0; u < U; ++u) {
+    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+      << "      const long long i = v" << u << "_ * 4 + e;\n";
+    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
+    emit_row_body_joins(o, s, "      ");
+    o << "      if (cond) {\n";
+    // group id
+    if (s.agg != AGG_SCALAR) {
+      o << "        unsigned gid = 0u;\n";
+      for (size_t q = 0; q < s.keys.size(); ++q) {
+        const KeyShape& K = s.keys[q];
+        std::string stride = "(unsigned)a.iv[" + I(32 + 4 * (int)q + 1) + "]";
+        if (K.src == KEY_DIMCODE)
+          o << "        gid += (p" << K.join << " - 1u) * " << stride << ";\n";
+        else if (K.src == KEY_DIMROW)
+          o << "        gid += (unsigned)((const int*)a.ptr[" << SVM_SLOT_CODE + q << "])[JI" << K.obj
+            << "] * " << stride << ";\n";
+        else
+          o << "        gid += (unsigned)(f" << K.col << " - a.iv[" << 32 + 4 * q << "]) * " << stride
+            << ";\n";
+      }
+    }
+    for (int m = 0; m < M; ++m)
+      o << "        const svm_i64 mv" << m << " = (svm_i64)(" << s.metric_ctype[m] << ")(" << s.metrics[m]
+        << ");\n";
+    if (s.agg == AGG_SCALAR) {
+      for (int m = 0; m < M; ++m) o << "        acc" << m << " += mv" << m << ";\n";
+    } else if (s.agg == AGG_DENSE) {
+      for (int m = 0; m < M; ++m)
+        o << "        atomicAdd(&lsum[gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+      o << "        atomicOr(&lflag[gid >> 5], 1u << (gid & 31u));\n";
+    } else {
+      o << "        unsigned h = (gid * 2654435761u) >> (32 - SVM_HBITS);\n"
+        << "        int slot = -1;\n"
+        << "        for (int t = 0; t < " << s.hash_probes << "; ++t) {\n"
+        << "          unsigned k = hkey[h];\n"
+        << "          if (k == gid + 1u) { slot = (int)h; break; }\n"
+        << "          if (k == 0u) {\n"
+        << "            unsigned old = atomicCAS(&hkey[h], 0u, gid + 1u);\n"
+        << "            if (old == 0u || old == gid + 1u) { slot = (int)h; break; }\n"
+        << "          }\n"
+        << "          h = (h + 1u) & (SVM_HSLOTS - 1u);\n"
+        << "        }\n"
+        << "        if (slot >= 0) {\n";
+      for (int m = 0; m < M; ++m)
+        o << "          atomicAdd(&hsum[slot * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+      o << "        } else {\n";
+      for (int m = 0; m < M; ++m)
+        o << "          atomicAdd(&gsum[(size_t)gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+      o << "          atomicOr(&gflag[gid >> 5], 1u << (gid & 31u));\n"
+        << "        }\n";
+    }
+    o << "      }\n    }\n";
+  }
+  o << "  }\n";
+  // flush
+  if (s.agg == AGG_SCALAR) {
+    o << "  __shared__ svm_i64 red[SVM_BLOCK / 64][" << (M > 0 ? M : 1) << "];\n";
+    for (int m = 0; m < M; ++m) {
+      o << "  for (int d = 32; d > 0; d >>= 1) acc" << m << " += __shfl_xor(acc" << m << ", d);\n"
+        << "  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][" << m << "] = acc" << m << ";\n";
+    }
+    o << "  __syncthreads();\n"
+      << "  if (threadIdx.x < " << M << ") {\n"
+      << "    svm_i64 t = 0;\n"
+      << "    for (int w = 0; w < SVM_BLOCK / 64; ++w) t += red[w][threadIdx.x];\n"
+      << "    atomicAdd((svm_u64*)a.out[2] + threadIdx.x, (svm_u64)t);\n"
+      << "  }\n";
+  } else if (s.agg == AGG_DENSE) {
+    o << "  __syncthreads();\n"
+      << "  for (unsigned g = threadIdx.x; g < G; g += SVM_BLOCK)\n"
+      << "    if ((lflag[g >> 5] >> (g & 31u)) & 1u) {\n";
+    for (int m = 0; m < M; ++m)
+      o << "      atomicAdd(&gsum[(size_t)g * SVM_M + " << m << "], lsum[g * SVM_M + " << m << "]);\n";
+    o << "    }\n"
+      << "  for (unsigned w = threadIdx.x; w < (G + 31u) / 32u; w += SVM_BLOCK)\n"
+      << "    if (lflag[w]) atomicOr(&gflag[w], lflag[w]);\n";
+  } else {
+    o << "  __syncthreads();\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS; q += SVM_BLOCK) {\n"
+      << "    const unsigned k = hkey[q];\n"
+      << "    if (k) {\n"
+      << "      const unsigned g = k - 1u;\n";
+    for (int m = 0; m < M; ++m)
+      o << "      atomicAdd(&gsum[(size_t)g * SVM_M + " << m << "], hsum[q * SVM_M + " << m << "]);\n";
+    o << "      atomicOr(&gflag[g >> 5], 1u << (g & 31u));\n"
+      << "    }\n  }\n";
+  }
+  o << "}\n";
+  return o.str();
+}
+
+std::string gen_select_source(const KernelShape& s) {
+  std::ostringstream o;
+  emit_prologue(o, s);
+  o << "#define SVM_RPT 4\n";
+  const char* names[2] = {"svm_sel_count", "svm_sel_write"};
+  for (int pass = 0; pass < 2; ++pass) {
+    o << "extern \"C\" __global__ void __launch_bounds__(SVM_BLOCK) " << names[pass] << "(SvmArgs a) {\n";
+    o << "  extern __shared__ svm_u64 svm_lds64[];\n  unsigned* svm_lds = (unsigned*)svm_lds64;\n";
+    emit_colptrs(o, s);
+    emit_stage_maps(o, s);
+    emit_map_ptrs(o, s);
+    o << "  __shared__ unsigned wcnt[SVM_RPT][SVM_BLOCK / 64];\n";
+    o << "  __syncthreads();\n";
+    o << "  const long long r_hi = a.row0 + a.rows;\n"
+      << "  const long long t0 = a.row0 + (long long)blockIdx.x * (SVM_BLOCK * SVM_RPT);\n"
+      << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
+      << "  unsigned passbits = 0u;\n";
+    // first sweep: conditions and per-wave counts
+    o << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
+      << "    const long long i = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
+      << "    bool cond = i < r_hi;\n";
+    for (int c : s.fact_i32)
+      o << "    const int f" << c << " = cond ? ((const int*)a.ptr[" << c << "])[i] : 0;\n";
+    emit_row_body_joins(o, s, "    ");
+    o << "    const unsigned long long mask = __ballot(cond);\n"
+      << "    if (lane == 0) wcnt[k][wave] = (unsigned)__popcll(mask);\n"
+      << "    passbits |= (cond ? 1u : 0u) << k;\n"
+      << "  }\n"
+      << "  __syncthreads();\n";
+    if (pass == 0) {
+      o << "  if (threadIdx.x == 0) {\n"
+        << "    unsigned t = 0;\n"
+        << "    for (int k = 0; k < SVM_RPT; ++k) for (int w = 0; w < SVM_BLOCK / 64; ++w) t += wcnt[k][w];\n"
+        << "    ((unsigned*)a.out[7])[blockIdx.x] = t;\n"
+        << "  }\n";
+    } else {
+      o << "  unsigned long long pos = ((const unsigned long long*)a.out[6])[blockIdx.x];\n"
+        << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
+        << "    unsigned before = 0;\n"
+        << "    for (int w = 0; w < wave; ++w) before += wcnt[k][w];\n"
+        << "    unsigned round = 0;\n"
+        << "    for (int w = 0; w < SVM_BLOCK / 64; ++w) round += wcnt[k][w];\n"
+        << "    const bool cond = (passbits >> k) & 1u;\n"
+        << "    const unsigned long long mask = __ballot(cond);\n"
+        << "    if (cond) {\n"
+        << "      const long long i = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
+        << "      const unsigned long long at = pos + before + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));\n";
+      for (int c : s.fact_i32) o << "      const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
+      // recompute join payloads for the projection (row ids)
+      for (size_t j = 0; j < s.joins.size(); ++j) {
+        const JoinShape& J = s.joins[j];
+        if (J.payload != PAY_ROW) continue;
+        o << "      const unsigned offj" << j << " = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+          << "      const long long JI" << J.robj << " = (long long)svm_map_get<" << J.wbits << ">(m" << j
+          << ", offj" << j << ") - 1;\n";
+      }
+      for (size_t q = 0; q < s.proj.size(); ++q) {
+        int t = s.proj_type[q].first;
+        long long size = s.proj_type[q].second;
+        std::string dst = "((char*)a.ptr[" + I(SVM_SLOT_OUT + (int)q) + "])";
+        if (t == 7) {
+          o << "      StrCopy(" << dst << " + at * " << size << "ull, (const char*)(" << s.proj[q] << "), "
+            << size << ");\n";
+        } else {
+          const char* ct = t == 0 ? "char" : t == 1 ? "short" : t == 3 ? "long long" : "int";
+          o << "      *(" << ct << "*)(" << dst << " + at * " << size << "ull) = (" << ct << ")(" << s.proj[q]
+            << ");\n";
+        }
+      }
+      o << "    }\n"
+        << "    pos += round;\n"
+        << "  }\n";
+    }
+    o << "}\n";
+  }
+  return o.str();
+}
+
+}  // namespace svm

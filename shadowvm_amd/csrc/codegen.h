@@ -1,0 +1,59 @@
+// codegen.h — HIP source generation for one pipeline shape (compiled by hipRTC).
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace svm {
+
+// What a join's packed dense map stores per key slot (0 always means "no row").
+enum PayloadKind {
+  PAY_EXIST = 0,  // 1: a dimension row with this key passed the dimension pipeline
+  PAY_CODE = 1,   // code+1 of the single group-key component this dimension supplies
+  PAY_ROW = 2     // row+1 of the first dimension row with this key
+};
+
+struct JoinShape {
+  int robj = 1;      // ref object (D4: join_index is addressed by this object)
+  int lcol = 0;      // source (fact) column holding the foreign key (Int32)
+  int payload = PAY_EXIST;
+  int wbits = 1;     // 1,2,4,8,16 or 32 bits per key slot
+  bool lds = false;  // map staged in LDS at kernel start
+};
+
+enum KeySource { KEY_DIMCODE = 0, KEY_DIMROW = 1, KEY_FACT = 2 };
+struct KeyShape {
+  int src = KEY_DIMCODE;
+  int join = 0;  // KEY_DIMCODE / KEY_DIMROW: join position supplying the row
+  int obj = 0;
+  int col = 0;   // KEY_FACT: source column
+};
+
+enum AggMode { AGG_SCALAR = 0, AGG_DENSE = 1, AGG_HASH = 2 };
+
+struct KernelShape {
+  int block = 256;
+  std::vector<int> fact_i32;          // Int32 source columns held in registers
+  std::string filter = "true";        // lowered conjunction of Filter predicates
+  std::vector<JoinShape> joins;
+  std::map<std::pair<int, int>, int> dcol_slot;  // (obj,col) -> ptr slot (ROW payload)
+  std::vector<int> fact_other;        // non-Int32 source columns read by expressions
+  // reduce
+  std::vector<KeyShape> keys;
+  std::vector<std::string> metrics;      // lowered metric expressions
+  std::vector<std::string> metric_ctype; // C type of the declared metric column
+  int agg = AGG_SCALAR;
+  int hash_bits = 11;                    // LDS hash table of 2^hash_bits slots
+  int hash_probes = 16;
+  int unroll = 1;                        // int4 groups per thread per iteration
+  // select
+  std::vector<std::string> proj;                      // lowered projection expressions
+  std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs
+};
+
+std::string gen_scan_source(const KernelShape& s);
+std::string gen_select_source(const KernelShape& s);
+const char* device_runtime_text();
+
+}  // namespace svm

@@ -1,0 +1,616 @@
+// exec.cpp — one pipeline on the GPU: resolve inputs, build dimension maps,
+// generate + launch the fused kernel, emit the output table.
+//
+// Reference flow being replaced (per pipeline):
+//   Pipeline::Open   host BuildHashTable per join            core/pipeline.cu:82-137
+//   Pipeline::Run    one thread + kernel per 2M-row bulk     core/pipeline.cu:139-254
+//   TransformTask    Query::Next, ShrinkTask, ReduceTask     core/pipeline.cu:262-472
+// Here: Open becomes a GPU dense-map build (plus per-key dictionaries), Run is a
+// single persistent scan over the HBM-resident source, and the reduce happens in
+// the scan (LDS tables + 64-bit atomics) followed by an ordered group emission.
+#include "exec.h"
+
+#include <climits>
+#include <cstring>
+#include <set>
+
+#include "codegen.h"
+#include "jit.h"
+#include "kargs.h"
+#include "kernels.h"
+
+namespace svm {
+
+namespace {
+
+constexpr uint64_t kMaxDenseRange = 1ull << 28;  // keys per dense join map
+constexpr uint64_t kMaxGroups = 1ull << 27;
+constexpr uint64_t kMaxFactKeyRange = 1ull << 24;
+constexpr uint64_t kDenseAggBytes = 16 * 1024;  // LDS for a dense per-workgroup table
+constexpr int kHashBits = 10;                    // LDS hash table slots = 2^kHashBits
+constexpr uint64_t kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup
+
+const char* ctype(int t) {
+  switch (t) {
+    case Byte: return "char";
+    case Int16: return "short";
+    case Int32: return "int";
+    case Int64: return "long long";
+    default: return nullptr;
+  }
+}
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { SVM_HIP(hipEventCreate(&e)); }
+  ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+float elapsed(const Event& a, const Event& b) {
+  float ms = 0;
+  SVM_HIP(hipEventElapsedTime(&ms, a.e, b.e));
+  return ms;
+}
+
+struct JoinBuild {
+  JoinShape shape;
+  int rcol = 0;
+  int32_t kmin = 0;
+  uint32_t range = 0;
+  uint64_t words = 0;
+  DevBuf packed;
+  uint32_t lds_off = 0;
+};
+
+struct KeyBuild {
+  KeyShape shape;
+  uint32_t n = 0, stride = 1;
+  int32_t fmin = 0;
+  DevBuf dict, codes;
+};
+
+int width_for(uint64_t maxval) {
+  const int ws[] = {1, 2, 4, 8, 16, 32};
+  for (int w : ws)
+    if (w == 32 || maxval <= ((1ull << w) - 1)) return w;
+  return 32;
+}
+
+std::pair<int32_t, int32_t> col_minmax(const void* col, uint64_t n, hipStream_t s) {
+  DevBuf mm(8);
+  k::minmax_i32((const int32_t*)col, n, mm.as<int32_t>(), s);
+  int32_t h[2];
+  SVM_HIP(hipMemcpyAsync(h, mm.p, 8, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return {h[0], h[1]};
+}
+
+// Dictionary of one group-key component of a dimension and each dimension row's code.
+void build_key_dict(const Table& D, int col, KeyBuild& kb, bool want_codes, DevBuf* codes_out,
+                    hipStream_t s) {
+  const Column& c = D.cols[col];
+  if (!c.dev) fail(SVM_ERR_INVALID, "group key column is not materialised");
+  if (c.type != Int32 && c.type != String && c.type != Int16 && c.type != Byte)
+    fail(SVM_ERR_UNSUPPORTED, "group keys must be integer or string columns");
+  uint64_t n = D.rows;
+  DevBuf vals(n * 4 + 4);
+  k::values_of(c.dev, c.type, c.size, n, vals.as<int32_t>(), s);
+  kb.dict = DevBuf(n * 4 + 4);
+  kb.n = (uint32_t)k::dict_build(vals.as<int32_t>(), n, kb.dict.as<int32_t>(), s);
+  if (want_codes) {
+    *codes_out = DevBuf(n * 4 + 4);
+    k::codes_of(vals.as<int32_t>(), n, kb.dict.as<int32_t>(), kb.n, codes_out->as<int32_t>(), s);
+  }
+}
+
+struct Ctx {
+  Engine& E;
+  const Plan& plan;
+  const PipelineInfo& info;
+  hipStream_t s;
+  TablePtr src;
+  std::vector<TablePtr> refs;
+  Schemas schemas;
+};
+
+void resolve(Ctx& c) {
+  if (c.plan.has_loader)
+    fail(SVM_ERR_UNSUPPORTED, "Loader pipelines (CSV ingest) are not part of the device path");
+  if (c.info.pre_pipeline_id.empty())
+    fail(SVM_ERR_NOT_FOUND, "pipeline '" + c.info.id + "' names no source object (pre_pipeline_id[0])");
+  c.src = c.E.store.get(c.info.pre_pipeline_id[0]);
+  for (size_t k = 1; k < c.info.pre_pipeline_id.size(); ++k)
+    c.refs.push_back(c.E.store.get(c.info.pre_pipeline_id[k]));
+  if (c.refs.size() + 1 > SVM_MAX_JOIN)
+    fail(SVM_ERR_UNSUPPORTED, "at most 7 ref objects (join_index[8], codegen/template.cpp:65)");
+  std::vector<std::pair<int, int64_t>> s0;
+  for (auto& col : c.src->cols) s0.push_back({col.type, col.size});
+  const auto& decl = c.plan.source_info.column_list;
+  if (!decl.empty()) {
+    if (decl.size() != s0.size())
+      fail(SVM_ERR_INVALID, "Source schema has " + std::to_string(decl.size()) + " columns, object '" +
+                                c.src->id + "' has " + std::to_string(s0.size()));
+    for (size_t q = 0; q < decl.size(); ++q)
+      if (decl[q].type != s0[q].first || (decl[q].val_size && decl[q].val_size != s0[q].second))
+        fail(SVM_ERR_INVALID, "Source column " + std::to_string(q) + " does not match object '" + c.src->id + "'");
+  }
+  c.schemas.push_back(s0);
+  for (auto& r : c.refs) {
+    std::vector<std::pair<int, int64_t>> sr;
+    for (auto& col : r->cols) sr.push_back({col.type, col.size});
+    c.schemas.push_back(sr);
+  }
+  if (!c.plan.has_sink) fail(SVM_ERR_INVALID, "pipeline '" + c.info.id + "' has no Sink");
+}
+
+// Joins: validation and dense-map builds.  `exprs_refs` are (obj,col) operands read
+// by metric / projection expressions (those dimensions need row-id payloads).
+std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& expr_refs,
+                                   std::vector<KeyBuild>& keys) {
+  std::vector<JoinBuild> joins;
+  std::set<int> seen;
+  for (size_t j = 0; j < c.plan.join_rkey.size(); ++j) {
+    KeyIndex L = c.plan.join_lkey[j], R = c.plan.join_rkey[j];
+    if (L.obj != 0) fail(SVM_ERR_UNSUPPORTED, "JoinWith lkey must be a source column");
+    if (R.obj < 1 || R.obj > (int)c.refs.size())
+      fail(SVM_ERR_INVALID, "JoinWith rkey names ref object " + std::to_string(R.obj) + " which is not an input");
+    if (!seen.insert(R.obj).second) fail(SVM_ERR_UNSUPPORTED, "two JoinWith on the same ref object");
+    if (L.col >= (int)c.schemas[0].size() || R.col >= (int)c.schemas[R.obj].size())
+      fail(SVM_ERR_INVALID, "JoinWith key column out of range");
+    if (c.schemas[0][L.col].first != Int32 || c.schemas[R.obj][R.col].first != Int32)
+      fail(SVM_ERR_UNSUPPORTED, "join keys must be Int32 columns");
+    const Table& D = *c.refs[R.obj - 1];
+    if (D.max_tuple_count && D.rows > D.max_tuple_count)
+      fail(SVM_ERR_UNSUPPORTED, "dimension '" + D.id + "' exceeds one bulk (only bulk 0 is joined, core/pipeline.cu:91)");
+    JoinBuild jb;
+    jb.rcol = R.col;
+    jb.shape.robj = R.obj;
+    jb.shape.lcol = L.col;
+    // payload selection
+    std::vector<int> comps;
+    for (size_t q = 0; q < keys.size(); ++q)
+      if (keys[q].shape.src != KEY_FACT && keys[q].shape.obj == R.obj) comps.push_back((int)q);
+    bool refs = false;
+    for (auto& o : expr_refs) refs = refs || o.first == R.obj;
+    int payload = refs || comps.size() > 1 ? PAY_ROW : comps.size() == 1 ? PAY_CODE : PAY_EXIST;
+    // key dictionaries of this dimension
+    DevBuf codes_for_pack;
+    for (int q : comps) {
+      KeyBuild& kb = keys[q];
+      kb.shape.join = (int)j;
+      kb.shape.src = payload == PAY_CODE ? KEY_DIMCODE : KEY_DIMROW;
+      if (payload == PAY_CODE) build_key_dict(D, kb.shape.col, kb, true, &codes_for_pack, c.s);
+      else build_key_dict(D, kb.shape.col, kb, true, &kb.codes, c.s);
+    }
+    jb.shape.payload = payload;
+    uint64_t maxval = payload == PAY_EXIST ? 1 : payload == PAY_CODE ? keys[comps[0]].n : D.rows;
+    jb.shape.wbits = payload == PAY_ROW ? 32 : width_for(maxval);
+    // dense map over [kmin, kmax]
+    const void* kcol = D.cols[R.col].dev;
+    if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
+    if (D.rows) {
+      auto mm = col_minmax(kcol, D.rows, c.s);
+      uint64_t range = (uint64_t)((int64_t)mm.second - (int64_t)mm.first + 1);
+      if (range > kMaxDenseRange)
+        fail(SVM_ERR_UNSUPPORTED, "join key range " + std::to_string(range) + " of '" + D.id + "' exceeds the dense map limit");
+      jb.kmin = mm.first;
+      jb.range = (uint32_t)range;
+    }
+    uint64_t per = 32 / (uint64_t)jb.shape.wbits;
+    jb.words = ((uint64_t)jb.range + per - 1) / per;
+    jb.packed = DevBuf(jb.words * 4 + 16);
+    if (jb.range) {
+      DevBuf first((uint64_t)jb.range * 4);
+      k::first_rows((const int32_t*)kcol, D.rows, jb.kmin, jb.range, first.as<int32_t>(), c.s);
+      k::pack_map(first.as<int32_t>(), jb.range, payload, codes_for_pack.as<int32_t>(), jb.shape.wbits,
+                  jb.packed.as<uint32_t>(), c.s);
+      SVM_HIP(hipStreamSynchronize(c.s));  // `first` and the codes are freed on return
+    }
+    joins.push_back(std::move(jb));
+  }
+  return joins;
+}
+
+void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t& lds_words) {
+  uint64_t used = (agg_bytes + 7) / 8 * 2;  // agg table first, 8-byte aligned, in words
+  for (auto& jb : joins) {
+    if (jb.words && (used + jb.words) * 4 <= kLdsBudget) {
+      jb.shape.lds = true;
+      jb.lds_off = (uint32_t)used;
+      used += jb.words;
+    }
+  }
+  lds_words = used;
+}
+
+void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
+                      const std::map<std::pair<int, int>, int>& dcol_slot, const Literals& lits,
+                      DevBuf& blob) {
+  memset(&a, 0, sizeof(a));
+  for (size_t q = 0; q < c.src->cols.size() && q < 24; ++q) a.ptr[q] = c.src->cols[q].dev;
+  for (size_t j = 0; j < joins.size(); ++j) {
+    a.ptr[SVM_SLOT_MAP + j] = joins[j].packed.p;
+    a.iv[4 * j + 0] = joins[j].kmin;
+    a.iv[4 * j + 1] = (int)joins[j].range;
+    a.iv[4 * j + 2] = (int)joins[j].words;
+    a.iv[4 * j + 3] = (int)joins[j].lds_off;
+  }
+  for (auto& kv : dcol_slot) {
+    const Table& D = *c.refs[kv.first.first - 1];
+    if (!D.cols[kv.first.second].dev) fail(SVM_ERR_INVALID, "dimension column is not materialised");
+    a.ptr[kv.second] = D.cols[kv.first.second].dev;
+  }
+  for (size_t q = 0; q < lits.ints.size() && q < SVM_MAX_LIT; ++q) a.pi[q] = lits.ints[q];
+  if (!lits.blob.empty()) {
+    // zero tail: a string literal may be read up to a projection's width (<= 4096)
+    blob = DevBuf(lits.blob.size() + 4096);
+    k::fill_u32(blob.p, blob.bytes / 4, 0, c.s);
+    SVM_HIP(hipMemcpyAsync(blob.p, lits.blob.data(), lits.blob.size(), hipMemcpyHostToDevice, c.s));
+    a.sblob = blob.as<const char>();
+  }
+  a.row0 = 0;
+  a.rows = (long long)c.src->rows;
+}
+
+std::string lower_all(Ctx& c, const std::vector<std::string>& exps, Literals& lits,
+                      std::set<std::pair<int, int>>& operands, std::vector<std::string>& out) {
+  for (auto& e : exps) {
+    Lowered l = lower_expr(e, c.schemas, false, &lits);
+    operands.insert(l.operands.begin(), l.operands.end());
+    out.push_back(l.code);
+  }
+  return {};
+}
+
+void classify_fact_columns(Ctx& c, const std::set<std::pair<int, int>>& ops, KernelShape& ks) {
+  std::set<int> i32, other;
+  for (auto& o : ops) {
+    if (o.first != 0) continue;
+    if (c.schemas[0][o.second].first == Int32) i32.insert(o.second);
+    else other.insert(o.second);
+  }
+  for (auto& jb : ks.joins) i32.insert(jb.lcol);
+  for (auto& k : ks.keys)
+    if (k.src == KEY_FACT) i32.insert(k.col);
+  ks.fact_i32.assign(i32.begin(), i32.end());
+  ks.fact_other.assign(other.begin(), other.end());
+  for (int col : ks.fact_i32)
+    if (col >= 24) fail(SVM_ERR_UNSUPPORTED, "source column index >= 24");
+}
+
+std::map<std::pair<int, int>, int> assign_dcols(const std::set<std::pair<int, int>>& ops) {
+  std::map<std::pair<int, int>, int> slots;
+  int next = SVM_SLOT_DCOL;
+  for (auto& o : ops) {
+    if (o.first == 0) continue;
+    if (next >= SVM_SLOT_CODE) fail(SVM_ERR_UNSUPPORTED, "too many dimension columns in expressions");
+    slots[o] = next++;
+  }
+  return slots;
+}
+
+unsigned scan_grid(Engine& E, uint64_t vec, int block, int unroll, uint64_t lds_bytes, int bpc) {
+  if (bpc <= 0) {
+    uint64_t lds_cu = 160 * 1024;
+    bpc = lds_bytes ? (int)(lds_cu / (lds_bytes + 512)) : 8;
+    if (bpc > 8) bpc = 8;
+    if (bpc < 1) bpc = 1;
+  }
+  uint64_t need = (vec + (uint64_t)block * unroll - 1) / ((uint64_t)block * unroll);
+  uint64_t cap = (uint64_t)E.num_cus * bpc;
+  uint64_t g = need < cap ? need : cap;
+  return (unsigned)(g ? g : 1);
+}
+
+// ------------------------------------------------------------------ reduce path
+std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
+  const Plan& p = c.plan;
+  auto res = std::make_unique<svm_result>();
+  const int K = (int)p.reduce_key.size(), M = (int)p.metric_exp.size();
+  if (M == 0) fail(SVM_ERR_INVALID, "ReduceBy without metrics");
+  if (K > 3 || (K > 0 && M > 3) || M > 8)
+    fail(SVM_ERR_UNSUPPORTED, "ReduceBy supports 1-3 keys x 1-3 metrics (core/pipeline.cu:380-471) or a scalar");
+  const auto& outcols = p.output_info.column_list;
+  if ((int)outcols.size() < K + M) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than keys + metrics");
+  Event ev0, ev1, ev2, ev3;
+  SVM_HIP(hipEventRecord(ev0.e, c.s));
+
+  KernelShape ks;
+  ks.block = opt.block;
+  ks.unroll = opt.unroll;
+  Literals lits;
+  std::set<std::pair<int, int>> ops, metric_ops;
+  // filters: the reference evaluates them before any join (codegen/codegen.cpp:100-110),
+  // so only source columns are defined there.
+  std::vector<std::string> fl;
+  lower_all(c, p.filter_exp, lits, ops, fl);
+  for (auto& o : ops)
+    if (o.first != 0) fail(SVM_ERR_UNSUPPORTED, "Filter predicates may only read source columns");
+  std::string filt;
+  for (size_t q = 0; q < fl.size(); ++q) filt += (q ? "&&" : "") + fl[q];
+  ks.filter = fl.empty() ? "true" : filt;
+  lower_all(c, p.metric_exp, lits, metric_ops, ks.metrics);
+  for (int m = 0; m < M; ++m) {
+    const char* t = ctype(outcols[K + m].type);
+    if (!t) fail(SVM_ERR_UNSUPPORTED, "metric columns must be integer");
+    ks.metric_ctype.push_back(t);
+  }
+  ops.insert(metric_ops.begin(), metric_ops.end());
+  // keys
+  std::vector<KeyBuild> keys(K);
+  for (int q = 0; q < K; ++q) {
+    KeyIndex ki = p.reduce_key[q];
+    if (ki.obj < 0 || ki.obj >= (int)c.schemas.size() || ki.col < 0 || ki.col >= (int)c.schemas[ki.obj].size())
+      fail(SVM_ERR_INVALID, "reduce key $(" + std::to_string(ki.obj) + "." + std::to_string(ki.col) + ") does not exist");
+    if (outcols[q].type != Int32) fail(SVM_ERR_UNSUPPORTED, "reduce key columns must be declared Int32");
+    keys[q].shape.obj = ki.obj;
+    keys[q].shape.col = ki.col;
+    keys[q].shape.src = ki.obj == 0 ? KEY_FACT : KEY_DIMCODE;
+    if (ki.obj == 0 && c.schemas[0][ki.col].first != Int32)
+      fail(SVM_ERR_UNSUPPORTED, "source-column group keys must be Int32");
+  }
+  for (int q = 0; q < K; ++q) {
+    if (keys[q].shape.obj == 0) continue;
+    bool joined = false;
+    for (auto& r : p.join_rkey) joined = joined || r.obj == keys[q].shape.obj;
+    if (!joined) fail(SVM_ERR_INVALID, "reduce key reads ref object " + std::to_string(keys[q].shape.obj) + " which no JoinWith probes");
+  }
+  for (auto& o : metric_ops) {
+    if (o.first == 0) continue;
+    bool joined = false;
+    for (auto& r : p.join_rkey) joined = joined || r.obj == o.first;
+    if (!joined) fail(SVM_ERR_INVALID, "metric reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
+  }
+  std::vector<JoinBuild> joins = build_joins(c, metric_ops, keys);
+  for (auto& jb : joins) ks.joins.push_back(jb.shape);
+  // source-column keys: code = value - min over the column
+  for (int q = 0; q < K; ++q) {
+    if (keys[q].shape.src != KEY_FACT) continue;
+    auto mm = c.src->rows ? col_minmax(c.src->cols[keys[q].shape.col].dev, c.src->rows, c.s)
+                          : std::make_pair(0, -1);
+    uint64_t range = (uint64_t)((int64_t)mm.second - (int64_t)mm.first + 1);
+    if (range > kMaxFactKeyRange) fail(SVM_ERR_UNSUPPORTED, "source-column group key range exceeds 2^24");
+    keys[q].fmin = mm.first;
+    keys[q].n = (uint32_t)range;
+  }
+  uint64_t G = 1;
+  for (int q = K - 1; q >= 0; --q) {
+    keys[q].stride = (uint32_t)G;
+    G *= keys[q].n;
+    if (G > kMaxGroups) fail(SVM_ERR_UNSUPPORTED, "group space exceeds 2^27");
+  }
+  for (int q = 0; q < K; ++q) ks.keys.push_back(keys[q].shape);
+  ks.dcol_slot = assign_dcols(metric_ops);
+  classify_fact_columns(c, ops, ks);
+  // aggregation mode and LDS layout
+  uint64_t agg_bytes = 0;
+  if (K == 0) ks.agg = AGG_SCALAR;
+  else if (G * M * 8 + (G + 31) / 32 * 4 <= kDenseAggBytes) {
+    ks.agg = AGG_DENSE;
+    agg_bytes = G * M * 8 + (G + 31) / 32 * 4;
+  } else {
+    ks.agg = AGG_HASH;
+    ks.hash_bits = kHashBits;
+    agg_bytes = (uint64_t)(1 << kHashBits) * (4 + 8 * M);
+  }
+  uint64_t lds_words = 0;
+  assign_lds(joins, agg_bytes, lds_words);
+  for (size_t j = 0; j < joins.size(); ++j) ks.joins[j].lds = joins[j].shape.lds;
+
+  // outputs
+  const uint64_t nflag = (G + 31) / 32;
+  DevBuf gsum(K ? G * M * 8 + 8 : 8), gflag(nflag * 4 + 4), ssum(8 * 8);
+  if (K) {
+    k::fill_u32(gsum.p, G * M * 2, 0, c.s);
+    k::fill_u32(gflag.p, nflag, 0, c.s);
+  }
+  k::fill_u32(ssum.p, 16, 0, c.s);
+  SvmArgs a;
+  DevBuf blob;
+  fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
+  for (int q = 0; q < K; ++q) {
+    a.iv[32 + 4 * q + 0] = keys[q].fmin;
+    a.iv[32 + 4 * q + 1] = (int)keys[q].stride;
+    a.iv[32 + 4 * q + 2] = (int)keys[q].n;
+    if (keys[q].shape.src == KEY_DIMROW) a.ptr[SVM_SLOT_CODE + q] = keys[q].codes.p;
+  }
+  a.iv[44] = (int)G;
+  a.iv[45] = 0;
+  a.out[0] = gsum.p;
+  a.out[1] = gflag.p;
+  a.out[2] = ssum.p;
+  const uint64_t rows = c.src->rows;
+  const bool run_scan = rows > 0 && (K == 0 || G > 0);
+  hipFunction_t fn = nullptr;
+  if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
+  SVM_HIP(hipEventRecord(ev1.e, c.s));
+  if (run_scan) {
+    uint64_t vec = (rows + 3) / 4;
+    unsigned grid = scan_grid(c.E, vec, ks.block, ks.unroll, lds_words * 4, opt.blocks_per_cu);
+    launch(fn, grid, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+  }
+  SVM_HIP(hipEventRecord(ev2.e, c.s));
+
+  // finish: ordered group emission into the output table
+  std::vector<std::pair<int, int64_t>> oschema;
+  for (int q = 0; q < K; ++q) oschema.push_back({Int32, 4});
+  for (int m = 0; m < M; ++m) oschema.push_back({Int64, 8});
+  uint64_t out_rows = 1;
+  TablePtr out;
+  if (K == 0) {
+    out = c.E.store.create(c.info.id, oschema, 1);
+    for (int m = 0; m < M; ++m)
+      SVM_HIP(hipMemcpyAsync(out->cols[m].dev, (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));
+  } else {
+    uint64_t nb = (nflag + 255) / 256;
+    DevBuf blocks(nb * 8 + 8), offs(nb * 8 + 16);
+    out_rows = G ? k::group_count(gflag.as<uint32_t>(), G, blocks.as<unsigned long long>(),
+                                  offs.as<unsigned long long>(), c.s)
+                 : 0;
+    out = c.E.store.create(c.info.id, oschema, out_rows);
+    if (out_rows) {
+      k::EmitArgs ea;
+      memset(&ea, 0, sizeof(ea));
+      ea.flags = gflag.as<uint32_t>();
+      ea.sums = gsum.as<uint64_t>();
+      ea.G = G;
+      ea.K = K;
+      ea.M = M;
+      for (int q = 0; q < K; ++q) {
+        ea.n[q] = keys[q].n;
+        ea.stride[q] = keys[q].stride;
+        ea.fmin[q] = keys[q].fmin;
+        ea.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict.as<int32_t>();
+        ea.keys_out[q] = (int32_t*)out->cols[q].dev;
+      }
+      for (int m = 0; m < M; ++m) ea.sums_out[m] = (int64_t*)out->cols[K + m].dev;
+      k::group_emit(ea, offs.as<unsigned long long>(), c.s);
+    }
+  }
+  out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+  SVM_HIP(hipEventRecord(ev3.e, c.s));
+  SVM_HIP(hipStreamSynchronize(c.s));
+  res->kind = K == 0 ? 2 : 1;
+  res->rows = (int64_t)out_rows;
+  res->types = oschema;
+  if (opt.want_host) {
+    for (size_t q = 0; q < oschema.size(); ++q) {
+      std::vector<uint8_t> h(out_rows * (uint64_t)oschema[q].second);
+      if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out->cols[q].dev, h.size(), hipMemcpyDeviceToHost));
+      res->cols.push_back(std::move(h));
+    }
+  }
+  res->build_ms = elapsed(ev0, ev1);
+  res->scan_ms = elapsed(ev1, ev2);
+  res->finish_ms = elapsed(ev2, ev3);
+  c.E.store.put(c.info.id, out);
+  return res;
+}
+
+// ------------------------------------------------------------------ select path
+std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
+  const Plan& p = c.plan;
+  auto res = std::make_unique<svm_result>();
+  const auto& outcols = p.output_info.column_list;
+  const int P = (int)p.select_exp.size();
+  if (P > SVM_MAX_SELECT) fail(SVM_ERR_UNSUPPORTED, "at most 10 projected columns");
+  if ((int)outcols.size() < P) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than the Map select list");
+  Event ev0, ev1, ev2, ev3;
+  SVM_HIP(hipEventRecord(ev0.e, c.s));
+  KernelShape ks;
+  ks.block = 256;
+  Literals lits;
+  std::set<std::pair<int, int>> ops, proj_ops;
+  std::vector<std::string> fl;
+  lower_all(c, p.filter_exp, lits, ops, fl);
+  for (auto& o : ops)
+    if (o.first != 0) fail(SVM_ERR_UNSUPPORTED, "Filter predicates may only read source columns");
+  std::string filt;
+  for (size_t q = 0; q < fl.size(); ++q) filt += (q ? "&&" : "") + fl[q];
+  ks.filter = fl.empty() ? "true" : filt;
+  lower_all(c, p.select_exp, lits, proj_ops, ks.proj);
+  for (int q = 0; q < P; ++q) {
+    int t = outcols[q].type;
+    if (t != Byte && t != Int16 && t != Int32 && t != Int64 && t != String)
+      fail(SVM_ERR_UNSUPPORTED, "projection type unsupported on the integer path");
+    if (outcols[q].val_size <= 0) fail(SVM_ERR_INVALID, "projected column without val_size");
+    if (t == String) {
+      // StrCopy copies val_size bytes from the operand's row (codegen/codegen.cpp:168);
+      // a wider projection reads into the following rows, bounded by the column padding.
+      if (outcols[q].val_size > 4096) fail(SVM_ERR_UNSUPPORTED, "string projections wider than 4096 bytes");
+      Lowered l = lower_expr(p.select_exp[q], c.schemas, false, nullptr);
+      for (auto& o : l.operands)
+        if (outcols[q].val_size > c.schemas[o.first][o.second].second + (int64_t)kColumnPad)
+          fail(SVM_ERR_UNSUPPORTED, "string projection wider than its source column + padding");
+    }
+    ks.proj_type.push_back({t, (long long)outcols[q].val_size});
+  }
+  ops.insert(proj_ops.begin(), proj_ops.end());
+  for (auto& o : proj_ops) {
+    if (o.first == 0) continue;
+    bool joined = false;
+    for (auto& r : p.join_rkey) joined = joined || r.obj == o.first;
+    if (!joined) fail(SVM_ERR_INVALID, "projection reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
+  }
+  std::vector<KeyBuild> nokeys;
+  std::vector<JoinBuild> joins = build_joins(c, proj_ops, nokeys);
+  for (auto& jb : joins) ks.joins.push_back(jb.shape);
+  ks.dcol_slot = assign_dcols(proj_ops);
+  classify_fact_columns(c, ops, ks);
+  uint64_t lds_words = 0;
+  assign_lds(joins, 0, lds_words);
+  for (size_t j = 0; j < joins.size(); ++j) ks.joins[j].lds = joins[j].shape.lds;
+
+  const uint64_t rows = c.src->rows;
+  const uint64_t tile = (uint64_t)ks.block * 4;
+  const uint64_t tiles = (rows + tile - 1) / tile;
+  DevBuf counts(tiles * 4 + 4), offs(tiles * 8 + 16);
+  SvmArgs a;
+  DevBuf blob;
+  fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
+  a.out[6] = offs.p;
+  a.out[7] = counts.p;
+  SVM_HIP(hipEventRecord(ev1.e, c.s));
+  uint64_t total = 0;
+  std::string src;
+  if (tiles) {
+    src = gen_select_source(ks);
+    hipFunction_t fcount = jit_function(src, "svm_sel_count");
+    if (tiles > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "select source too large");
+    launch(fcount, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
+                          offs.as<unsigned long long>() + tiles, c.s);
+    SVM_HIP(hipMemcpyAsync(&total, offs.as<unsigned long long>() + tiles, 8, hipMemcpyDeviceToHost, c.s));
+    SVM_HIP(hipStreamSynchronize(c.s));
+  }
+  // output object: the sink schema; only projected columns are materialised (the
+  // reference reserves the trailing column for the join hash table, core/pipeline.cu:94)
+  std::vector<std::pair<int, int64_t>> oschema;
+  std::vector<bool> mat;
+  for (size_t q = 0; q < outcols.size(); ++q) {
+    int64_t vs = outcols[q].val_size > 0 ? outcols[q].val_size : 4;
+    oschema.push_back({outcols[q].type, vs});
+    mat.push_back((int)q < P);
+  }
+  TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat);
+  out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+  for (int q = 0; q < P; ++q) a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
+  if (tiles && total) {
+    hipFunction_t fwrite = jit_function(src, "svm_sel_write");
+    launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+  }
+  SVM_HIP(hipEventRecord(ev2.e, c.s));
+  SVM_HIP(hipEventRecord(ev3.e, c.s));
+  SVM_HIP(hipStreamSynchronize(c.s));
+  res->kind = 0;
+  res->rows = (int64_t)total;
+  for (int q = 0; q < P; ++q) res->types.push_back(oschema[q]);
+  if (opt.want_host) {
+    for (int q = 0; q < P; ++q) {
+      std::vector<uint8_t> h(total * (uint64_t)oschema[q].second);
+      if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out->cols[q].dev, h.size(), hipMemcpyDeviceToHost));
+      res->cols.push_back(std::move(h));
+    }
+  }
+  res->build_ms = elapsed(ev0, ev1);
+  res->scan_ms = elapsed(ev1, ev2);
+  res->finish_ms = elapsed(ev2, ev3);
+  c.E.store.put(c.info.id, out);
+  return res;
+}
+
+}  // namespace
+
+std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOptions& opt) {
+  Engine& E = engine();
+  Plan plan = decode_plan(info);
+  Ctx c{E, plan, info, opt.stream ? opt.stream : E.stream, nullptr, {}, {}};
+  resolve(c);
+  if (plan.is_reduce && !plan.select_exp.empty())
+    fail(SVM_ERR_UNSUPPORTED, "a pipeline with both Map and ReduceBy is not well defined in the reference");
+  if (plan.is_reduce) return run_reduce(c, opt);
+  if (!plan.select_exp.empty()) return run_select(c, opt);
+  fail(SVM_ERR_UNSUPPORTED, "pipeline '" + info.id + "' has neither Map nor ReduceBy");
+}
+
+}  // namespace svm

@@ -1,0 +1,57 @@
+// kernels.h — precompiled gfx950 kernels of libsvm (build, finish and data generation
+// steps that do not depend on the plan's expressions).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "datagen.h"
+
+namespace svm {
+namespace k {
+
+// Every kernel below runs on `s` and returns without synchronising unless noted.
+void fill_u32(void* p, uint64_t nwords, uint32_t v, hipStream_t s);
+// out[0] = min, out[1] = max of col[0..n) (out[0..1] are initialised here).
+void minmax_i32(const int32_t* col, uint64_t n, int32_t* out, hipStream_t s);
+// Group-key value of each dimension row: the int value, or Hash(bytes, size) for strings.
+void values_of(const void* col, int type, int64_t size, uint64_t n, int32_t* vals, hipStream_t s);
+// Sorted distinct values of vals[0..n) into dict; returns the count (synchronises).
+uint64_t dict_build(const int32_t* vals, uint64_t n, int32_t* dict, hipStream_t s);
+// codes[r] = position of vals[r] in dict (binary search).
+void codes_of(const int32_t* vals, uint64_t n, const int32_t* dict, uint64_t nd, int32_t* codes,
+              hipStream_t s);
+// first[key-kmin] = smallest row r with keys[r] == key (first initialised to INT32_MAX).
+void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int32_t* first,
+                hipStream_t s);
+// Packed map: slot = 0 if first == INT32_MAX else 1 / codes[row]+1 / row+1 by payload.
+void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
+              uint32_t* packed, hipStream_t s);
+// out[i] = sum(in[0..i)), *total = sum(in) (single workgroup).
+void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,
+                        unsigned long long* total, hipStream_t s);
+
+struct EmitArgs {
+  const uint32_t* flags;      // bit per group
+  const uint64_t* sums;       // G x M
+  uint64_t G;
+  int K, M;
+  uint32_t n[3], stride[3];   // cardinality and stride of each key component
+  int32_t fmin[3];            // value of code 0 when dict is null
+  const int32_t* dict[3];     // sorted dictionary of the component, or null
+  int32_t* keys_out[3];       // K int32 columns
+  int64_t* sums_out[8];       // M int64 columns
+};
+// Writes the groups whose flag is set, in group-id order, and returns their count
+// (synchronises once to size the output).
+uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scratch_blocks,
+                     unsigned long long* scratch_offsets, hipStream_t s);
+void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
+
+// Synthetic table rows [row0, row0+n) of `kind` into device columns.
+void gen_table(const svm_gen_ctx& ctx, int kind, uint64_t row0, uint64_t n, void* const* cols,
+               hipStream_t s);
+
+}  // namespace k
+}  // namespace svm

@@ -1,0 +1,298 @@
+// kernels.hip — precompiled gfx950 kernels: dense-map build, dictionaries, group
+// emission, scans and the synthetic table generator.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include "kernels.h"
+#include "svm_internal.h"
+
+namespace svm {
+namespace k {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(uint64_t n, int per_block = kBlock, unsigned cap = 65536) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  return (unsigned)(g > cap ? cap : g);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_minmax_init(int32_t* out) {
+  out[0] = INT_MAX;
+  out[1] = INT_MIN;
+}
+
+__global__ void k_minmax(const int32_t* __restrict__ col, uint64_t n, int32_t* out) {
+  int lo = INT_MAX, hi = INT_MIN;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    int v = col[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    int a = __shfl_xor(lo, d), b = __shfl_xor(hi, d);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&out[0], lo);
+    atomicMax(&out[1], hi);
+  }
+}
+
+__device__ inline unsigned hash31(const char* p, int n) {
+  unsigned h = 0;
+  for (int q = 0; q < n; ++q) h = h * 31u + (unsigned)(int)(signed char)p[q];
+  return h & 0x7FFFFFFFu;
+}
+
+__global__ void k_values(const char* __restrict__ col, int type, int64_t size, uint64_t n,
+                         int32_t* vals) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const char* p = col + r * (uint64_t)size;
+    int32_t v;
+    switch (type) {
+      case Byte: v = (int32_t)*(const signed char*)p; break;
+      case Int16: v = (int32_t)*(const short*)p; break;
+      case String: v = (int32_t)hash31(p, (int)size); break;
+      default: v = *(const int32_t*)p; break;
+    }
+    vals[r] = v;
+  }
+}
+
+__global__ void k_codes(const int32_t* __restrict__ vals, uint64_t n, const int32_t* __restrict__ dict,
+                        uint64_t nd, int32_t* codes) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t v = vals[r];
+    uint64_t lo = 0, hi = nd;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (dict[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    codes[r] = (int32_t)lo;
+  }
+}
+
+__global__ void k_first_rows(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
+                             int32_t* first) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t off = (uint32_t)(keys[r] - kmin);
+    if (off < range) atomicMin(&first[off], (int32_t)r);
+  }
+}
+
+__global__ void k_pack(const int32_t* __restrict__ first, uint32_t range, int payload,
+                       const int32_t* __restrict__ codes, int wbits, uint32_t* packed, uint64_t nwords) {
+  const int per = 32 / wbits;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t word = 0;
+    for (int e = 0; e < per; ++e) {
+      uint64_t slot = w * (uint64_t)per + (uint64_t)e;
+      if (slot >= range) break;
+      int32_t f = first[slot];
+      uint32_t val = 0;
+      if (f != INT_MAX) val = payload == 0 ? 1u : payload == 1 ? (uint32_t)codes[f] + 1u : (uint32_t)f + 1u;
+      word |= wbits == 32 ? val : (val << (e * wbits));
+    }
+    packed[w] = word;
+  }
+}
+
+// Single-workgroup exclusive scan with a running carry (inputs up to a few million).
+template <typename T>
+__global__ void k_exscan(const T* __restrict__ in, uint64_t n, unsigned long long* out,
+                         unsigned long long* total) {
+  __shared__ unsigned long long buf[1024];
+  unsigned long long carry = 0;
+  for (uint64_t base = 0; base < n; base += 1024) {
+    uint64_t i = base + threadIdx.x;
+    unsigned long long v = i < n ? (unsigned long long)in[i] : 0ull;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      unsigned long long t = threadIdx.x >= (unsigned)d ? buf[threadIdx.x - d] : 0ull;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) out[i] = carry + buf[threadIdx.x] - v;
+    unsigned long long blk = buf[1023];
+    __syncthreads();
+    carry += blk;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_group_popc(const uint32_t* __restrict__ flags, uint64_t nwords,
+                             unsigned long long* counts) {
+  __shared__ unsigned part[kBlock / 64];
+  uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  unsigned c = w < nwords ? (unsigned)__popc(flags[w]) : 0u;
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int q = 0; q < kBlock / 64; ++q) t += part[q];
+    counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ offsets) {
+  __shared__ unsigned pre[kBlock];
+  const uint64_t nwords = (a.G + 31) / 32;
+  uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t bits = w < nwords ? a.flags[w] : 0u;
+  unsigned c = (unsigned)__popc(bits);
+  pre[threadIdx.x] = c;
+  __syncthreads();
+  for (int d = 1; d < kBlock; d <<= 1) {
+    unsigned t = threadIdx.x >= (unsigned)d ? pre[threadIdx.x - d] : 0u;
+    __syncthreads();
+    pre[threadIdx.x] += t;
+    __syncthreads();
+  }
+  unsigned long long row = offsets[blockIdx.x] + pre[threadIdx.x] - c;
+  while (bits) {
+    int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    uint64_t g = w * 32 + (uint64_t)b;
+    for (int q = 0; q < a.K; ++q) {
+      uint32_t code = (uint32_t)((g / a.stride[q]) % a.n[q]);
+      a.keys_out[q][row] = a.dict[q] ? a.dict[q][code] : a.fmin[q] + (int32_t)code;
+    }
+    for (int m = 0; m < a.M; ++m) a.sums_out[m][row] = (int64_t)a.sums[g * (uint64_t)a.M + m];
+    row++;
+  }
+}
+
+struct GenCols {
+  void* c[SVM_GEN_MAX_COLS];
+};
+
+__global__ void k_gen(svm_gen_ctx ctx, int kind, uint64_t row0, uint64_t n, GenCols cols) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    svm_gen_row(&ctx, kind, row0 + r, r, cols.c);
+}
+
+}  // namespace
+
+#define LAUNCH_CHECK(what) ::svm::hip_check(hipGetLastError(), what)
+
+void fill_u32(void* p, uint64_t nwords, uint32_t v, hipStream_t s) {
+  if (!nwords) return;
+  k_fill_u32<<<grid_for(nwords), kBlock, 0, s>>>((uint32_t*)p, nwords, v);
+  LAUNCH_CHECK("fill_u32");
+}
+
+void minmax_i32(const int32_t* col, uint64_t n, int32_t* out, hipStream_t s) {
+  k_minmax_init<<<1, 1, 0, s>>>(out);
+  if (n) k_minmax<<<grid_for(n, kBlock, 2048), kBlock, 0, s>>>(col, n, out);
+  LAUNCH_CHECK("minmax_i32");
+}
+
+void values_of(const void* col, int type, int64_t size, uint64_t n, int32_t* vals, hipStream_t s) {
+  if (!n) return;
+  k_values<<<grid_for(n), kBlock, 0, s>>>((const char*)col, type, size, n, vals);
+  LAUNCH_CHECK("values_of");
+}
+
+uint64_t dict_build(const int32_t* vals, uint64_t n, int32_t* dict, hipStream_t s) {
+  if (!n) return 0;
+  DevBuf sorted(n * sizeof(int32_t));
+  size_t t1 = 0, t2 = 0;
+  SVM_HIP(rocprim::radix_sort_keys(nullptr, t1, vals, sorted.as<int32_t>(), n, 0, 32, s));
+  SVM_HIP(rocprim::unique(nullptr, t2, sorted.as<int32_t>(), dict, (unsigned long long*)nullptr, n,
+                          rocprim::equal_to<int32_t>(), s));
+  DevBuf tmp((t1 > t2 ? t1 : t2) + 256);
+  DevBuf cnt(sizeof(unsigned long long));
+  SVM_HIP(rocprim::radix_sort_keys(tmp.p, t1, vals, sorted.as<int32_t>(), n, 0, 32, s));
+  SVM_HIP(rocprim::unique(tmp.p, t2, sorted.as<int32_t>(), dict, cnt.as<unsigned long long>(), n,
+                          rocprim::equal_to<int32_t>(), s));
+  unsigned long long h = 0;
+  SVM_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void codes_of(const int32_t* vals, uint64_t n, const int32_t* dict, uint64_t nd, int32_t* codes,
+              hipStream_t s) {
+  if (!n) return;
+  k_codes<<<grid_for(n), kBlock, 0, s>>>(vals, n, dict, nd, codes);
+  LAUNCH_CHECK("codes_of");
+}
+
+void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int32_t* first,
+                hipStream_t s) {
+  fill_u32(first, range, (uint32_t)INT_MAX, s);
+  if (n) k_first_rows<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first);
+  LAUNCH_CHECK("first_rows");
+}
+
+void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
+              uint32_t* packed, hipStream_t s) {
+  uint64_t per = 32 / (uint64_t)wbits;
+  uint64_t nwords = ((uint64_t)range + per - 1) / per;
+  if (!nwords) return;
+  k_pack<<<grid_for(nwords), kBlock, 0, s>>>(first, range, payload, codes, wbits, packed, nwords);
+  LAUNCH_CHECK("pack_map");
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,
+                        unsigned long long* total, hipStream_t s) {
+  k_exscan<uint32_t><<<1, 1024, 0, s>>>(in, n, out, total);
+  LAUNCH_CHECK("exclusive_scan_u32");
+}
+
+uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* blocks,
+                     unsigned long long* offsets, hipStream_t s) {
+  uint64_t nwords = (G + 31) / 32;
+  uint64_t nb = (nwords + kBlock - 1) / kBlock;
+  if (!nb) return 0;
+  k_group_popc<<<(unsigned)nb, kBlock, 0, s>>>(flags, nwords, blocks);
+  k_exscan<unsigned long long><<<1, 1024, 0, s>>>(blocks, nb, offsets, offsets + nb);
+  LAUNCH_CHECK("group_count");
+  unsigned long long total = 0;
+  SVM_HIP(hipMemcpyAsync(&total, offsets + nb, sizeof(total), hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return total;
+}
+
+void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s) {
+  uint64_t nwords = (a.G + 31) / 32;
+  uint64_t nb = (nwords + kBlock - 1) / kBlock;
+  if (!nb) return;
+  k_group_emit<<<(unsigned)nb, kBlock, 0, s>>>(a, offsets);
+  LAUNCH_CHECK("group_emit");
+}
+
+void gen_table(const svm_gen_ctx& ctx, int kind, uint64_t row0, uint64_t n, void* const* cols,
+               hipStream_t s) {
+  if (!n) return;
+  GenCols gc;
+  for (int c = 0; c < SVM_GEN_MAX_COLS; ++c) gc.c[c] = cols[c];
+  k_gen<<<grid_for(n, kBlock, 16384), kBlock, 0, s>>>(ctx, kind, row0, n, gc);
+  LAUNCH_CHECK("gen_table");
+}
+
+}  // namespace k
+}  // namespace svm

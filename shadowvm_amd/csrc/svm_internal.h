@@ -1,0 +1,139 @@
+// svm_internal.h — shared declarations of libsvm's host side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/svm.h"
+
+namespace svm {
+
+// ----------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define SVM_HIP(x) ::svm::hip_check((x), #x)
+
+// ----------------------------------------------------------------- plan format
+// Field layout of com/svm.proto:73-144 (ColumnInfo, ObjectInfo, OperatorInfo, PipelineInfo).
+enum ValType { Byte = 0, Int16 = 1, Int32 = 2, Int64 = 3, Float16 = 4, Float32 = 5,
+               Float64 = 6, String = 7, Tensor = 8 };
+enum OpType { OpSource = 0, OpLoader = 1, OpFilter = 2, OpMap = 3, OpReduceBy = 4,
+              OpSortBy = 5, OpJoinWith = 6, OpSink = 7 };
+
+struct ColumnInfo {
+  std::string id;
+  int type = Int32;
+  bool isnull = false;
+  std::vector<int32_t> dim;
+  int64_t val_size = 4;
+};
+struct ObjectInfo {
+  std::string id, host, path;
+  std::vector<ColumnInfo> column_list;
+  int storage = 0;
+};
+struct OperatorInfo {
+  std::string id;
+  int type = 0;
+  std::map<std::string, std::string> args;
+  ObjectInfo output;
+  bool has_output = false;
+  int perf = 0;
+};
+struct PipelineInfo {
+  std::string id;
+  std::vector<OperatorInfo> workflow;
+  std::vector<std::string> pre_pipeline_id;
+  bool codegen = false, debug = false, isGPU = false;
+};
+PipelineInfo decode_pipeline_info(const uint8_t* data, size_t len);
+std::string encode_pipeline_info(const PipelineInfo& p);
+
+// ----------------------------------------------------------------- store
+struct Column {
+  int type = Int32;
+  int64_t size = 4;
+  void* dev = nullptr;      // nullptr: declared but not materialised (e.g. a hash-table column)
+  uint64_t bytes = 0;       // allocation size (data + padding)
+};
+struct Table {
+  std::string id;
+  uint64_t rows = 0;
+  uint64_t max_tuple_count = 0;
+  std::vector<Column> cols;
+  ObjectInfo info;          // schema as ObjectInfo (ref object info_)
+  ~Table();
+};
+using TablePtr = std::shared_ptr<Table>;
+
+// Column allocations carry this many zero bytes after the last row, so fixed-width
+// copies that read past a row (StrCopy of a wider projection) and 128-bit vector
+// loads of a partial last group stay inside the allocation.
+constexpr uint64_t kColumnPad = 256;
+
+class Store {
+ public:
+  TablePtr create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,
+                  uint64_t rows, const std::vector<bool>* materialise = nullptr);
+  void put(const std::string& id, TablePtr t);
+  TablePtr get(const std::string& id);  // throws SVM_ERR_NOT_FOUND
+  TablePtr find(const std::string& id);
+  bool drop(const std::string& id);
+  void clear();
+ private:
+  std::mutex mu_;
+  std::map<std::string, TablePtr> tables_;
+};
+
+// ----------------------------------------------------------------- device memory
+void* dev_alloc(uint64_t bytes);  // hipMalloc with error mapping
+void dev_free(void* p);
+struct DevBuf {
+  void* p = nullptr;
+  uint64_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(uint64_t b) : p(b ? dev_alloc(b) : nullptr), bytes(b) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { dev_free(p); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { dev_free(p); }
+  template <typename T> T* as() const { return (T*)p; }
+};
+
+// ----------------------------------------------------------------- engine
+struct Config {
+  int device = 0;
+  uint64_t tuple_per_bulk = 3000000;
+  uint64_t hbm_budget = 32ull << 30;
+  int num_streams = 8;
+  int verbose = 0;
+};
+
+struct Engine {
+  Config cfg;
+  Store store;
+  int num_cus = 256;
+  int max_lds = 65536;
+  hipStream_t stream = nullptr;  // default execution stream of svm_pipeline_run
+  std::mutex run_mu;             // serialises plain pipeline runs on `stream`
+};
+Engine& engine();  // throws SVM_ERR_STATE if not initialised
+bool engine_ready();
+
+}  // namespace svm

@@ -1,0 +1,192 @@
+"""Host-side operator surface of the data plane, mirroring the reference's
+``PipelineTest`` harness (test/test.hpp:78-149) and ``CreateSchema``
+(test/test.hpp:17-27): build a ``PipelineInfo`` from Source / Filter / Map /
+JoinWith / ReduceBy / Sink operators, then run it against objects held in the
+device store.  Execution happens in libsvm.so on the GPU; this module only
+assembles plans and moves columns.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .proto import serialize
+
+# ValType (com/svm.proto:62-72)
+BYTE, INT16, INT32, INT64, FLOAT16, FLOAT32, FLOAT64, STRING, TENSOR = range(9)
+TInt32, TString = INT32, STRING
+# OperatorInfo.Type (com/svm.proto:102-111)
+SOURCE, LOADER, FILTER, MAP, REDUCEBY, SORTBY, JOINWITH, SINK = range(8)
+
+NP_DTYPE = {BYTE: np.int8, INT16: np.int16, INT32: np.int32, INT64: np.int64}
+HashRate = 5  # com/cpp/define.hpp:29
+
+
+def create_schema(types, sizes):
+    """CreateSchema (test/test.hpp:17-27): an ObjectInfo with the given columns."""
+    if len(types) != len(sizes):
+        raise ValueError("types and sizes differ in length")
+    return {"column_list": [{"type": int(t), "val_size": int(s)} for t, s in zip(types, sizes)],
+            "storage": 1}
+
+
+# ---------------------------------------------------------------- engine
+def init(device=0, tuple_per_bulk=3_000_000, hbm_budget=0, num_streams=8, verbose=0):
+    cfg = _lib.SvmConfig(device, tuple_per_bulk, hbm_budget, num_streams, verbose)
+    _lib.check(_lib.lib().svm_init(ctypes.byref(cfg)))
+
+
+def shutdown():
+    _lib.check(_lib.lib().svm_shutdown())
+
+
+def table_create(id, types, sizes, rows):
+    n = len(types)
+    t = (ctypes.c_int32 * n)(*types)
+    s = (ctypes.c_int64 * n)(*sizes)
+    _lib.check(_lib.lib().svm_table_create(id.encode(), n, t, s, rows))
+
+
+def table_put(id, col, array):
+    a = np.ascontiguousarray(array)
+    _lib.check(_lib.lib().svm_table_put_column(id.encode(), col, a.ctypes.data, a.nbytes))
+
+
+def table_upload(id, types, sizes, columns):
+    """Create table `id` and copy host columns (int arrays, or (rows, size) uint8 for strings)."""
+    rows = int(columns[0].shape[0]) if columns else 0
+    table_create(id, types, sizes, rows)
+    for c, a in enumerate(columns):
+        if a is not None:
+            table_put(id, c, a)
+
+
+def table_info(id):
+    rows = ctypes.c_uint64()
+    ncols = ctypes.c_int()
+    _lib.check(_lib.lib().svm_table_info(id.encode(), ctypes.byref(rows), ctypes.byref(ncols)))
+    return int(rows.value), int(ncols.value)
+
+
+def table_column(id, col):
+    rows, _ = table_info(id)
+    t, s = ctypes.c_int32(), ctypes.c_int64()
+    _lib.check(_lib.lib().svm_table_column_type(id.encode(), col, ctypes.byref(t), ctypes.byref(s)))
+    buf = np.zeros(rows * s.value, np.uint8)
+    _lib.check(_lib.lib().svm_table_get_column(id.encode(), col, buf.ctypes.data, buf.nbytes))
+    return _view(buf, t.value, s.value)
+
+
+def table_drop(id):
+    _lib.check(_lib.lib().svm_table_drop(id.encode()))
+
+
+def table_exists(id):
+    return bool(_lib.lib().svm_table_exists(id.encode()))
+
+
+def gen_params(n_cust=0, n_supp=0, n_part=0, zipf_s=1.1):
+    return _lib.SvmGenParams(n_cust, n_supp, n_part, zipf_s)
+
+
+def gen_schema(kind):
+    t = (ctypes.c_int32 * 8)()
+    s = (ctypes.c_int64 * 8)()
+    n = _lib.check(_lib.lib().svm_gen_schema(kind, t, s))
+    return list(t[:n]), list(s[:n])
+
+
+def gen_table(id, kind, rows, seed, params=None):
+    p = params or gen_params()
+    _lib.check(_lib.lib().svm_gen_table(id.encode(), kind, rows, seed, ctypes.byref(p)))
+
+
+def gen_host(kind, seed, row0, n, params=None, threads=8):
+    """The same rows a device table of `kind` holds, generated on the host."""
+    types, sizes = gen_schema(kind)
+    cols = [np.zeros(n * s + 1, np.uint8) for s in sizes]
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    p = params or gen_params()
+    _lib.check(_lib.lib().svm_gen_host(kind, seed, ctypes.byref(p), row0, n, ptrs, threads))
+    return types, sizes, [_view(c[: n * s], t, s) for c, t, s in zip(cols, types, sizes)]
+
+
+def _view(buf, t, s):
+    if t == STRING or t == TENSOR:
+        return buf.reshape(-1, s)
+    return buf.view(NP_DTYPE[t])
+
+
+# ---------------------------------------------------------------- results
+@dataclass
+class Result:
+    kind: str          # "select" | "group" | "scalar"
+    rows: int
+    columns: list      # numpy arrays (group-by: K int32 keys then M int64 sums)
+    build_ms: float
+    scan_ms: float
+    finish_ms: float
+
+
+def _take_result(ptr) -> Result:
+    L = _lib.lib()
+    try:
+        rows = int(L.svm_result_rows(ptr))
+        ncols = int(L.svm_result_cols(ptr))
+        cols, types = [], []
+        for c in range(ncols):
+            t, s = ctypes.c_int32(), ctypes.c_int64()
+            _lib.check(L.svm_result_col_type(ptr, c, ctypes.byref(t), ctypes.byref(s)))
+            nbytes = rows * s.value
+            addr = L.svm_result_col(ptr, c)
+            buf = np.zeros(nbytes, np.uint8)
+            if nbytes and addr:
+                ctypes.memmove(buf.ctypes.data, addr, nbytes)
+            cols.append(_view(buf, t.value, s.value))
+            types.append(t.value)
+        b, sc, f = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(L.svm_result_timing(ptr, ctypes.byref(b), ctypes.byref(sc), ctypes.byref(f)))
+        kind = {0: "select", 1: "group", 2: "scalar"}[int(L.svm_result_kind(ptr))]
+        return Result(kind, rows, cols, b.value, sc.value, f.value)
+    finally:
+        L.svm_result_free(ptr)
+
+
+def run_plan(plan: dict, want_result=True) -> Result | None:
+    """Serialize a plan dict to PipelineInfo bytes and run it (svm_pipeline_run)."""
+    data = serialize(plan)
+    buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().svm_pipeline_run(buf, len(data), ctypes.byref(out) if want_result else None))
+    return _take_result(out) if want_result else None
+
+
+class PipelineTest:
+    """test/test.hpp:78-149: AddOperator / SetMode / Run against cached objects."""
+
+    def __init__(self, id):
+        self.pipe_info = {"id": id, "workflow": [], "pre_pipeline_id": [],
+                          "codegen": True, "debug": False, "isGPU": True}
+
+    def AddOperator(self, type, args, output):
+        self.pipe_info["workflow"].append(
+            {"type": int(type), "id": self.pipe_info["id"], "args": dict(args), "output": output})
+        return self
+
+    def SetMode(self, codegen, debug, isGPU=True):
+        self.pipe_info.update(codegen=codegen, debug=debug, isGPU=isGPU)
+        return self
+
+    def plan(self, input_obj, ref_obj=()):
+        p = dict(self.pipe_info)
+        p["pre_pipeline_id"] = [input_obj] + list(ref_obj)
+        return p
+
+    def Run(self, input_obj, ref_obj=(), codegen=True, print_=False, isGPU=True, want_result=True):
+        """Run against the objects named `input_obj` (source) and `ref_obj` (refs);
+        the output is cached in the store under this pipeline's id."""
+        self.pipe_info.update(codegen=codegen, debug=print_, isGPU=isGPU)
+        return run_plan(self.plan(input_obj, ref_obj), want_result)
