@@ -67,6 +67,8 @@ int svm_table_get_column(const char* id, int col, void* host_dst, uint64_t bytes
 int svm_table_column_device(const char* id, int col, void** dev_ptr);
 int svm_table_info(const char* id, uint64_t* rows, int* ncols);
 int svm_table_column_type(const char* id, int col, int32_t* type, int64_t* val_size);
+/* Zero-copy view `id` of rows [row0, row0+rows) of table `base_id` (a fact slice). */
+int svm_table_view(const char* id, const char* base_id, uint64_t row0, uint64_t rows);
 int svm_table_drop(const char* id);
 int svm_table_exists(const char* id);
 
@@ -101,14 +103,28 @@ const void* svm_result_col(const svm_result* r, int col);
 int svm_result_timing(const svm_result* r, double* build_ms, double* scan_ms, double* finish_ms);
 void svm_result_free(svm_result* r);
 
-/* ---- batched execution: many fact pipelines fused into shared-scan kernels ---- */
+/* ---- batched execution: many star-join queries fused into one shared scan ----
+ * plans: dimension and fact pipelines of all queries, in submission order.  Fact
+ * pipelines whose operators fit the shared scan (see DESIGN) run fused; their
+ * dimension pipelines are absorbed (outs[i] is NULL for an absorbed one and
+ * its object is not cached); every other plan runs as svm_pipeline_run would.
+ * Returns the number of fused fact pipelines (>= 0) or a negative status. */
 int svm_batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs);
 
-/* ---- serving: continuous batching against device-resident tables ---- */
-int svm_server_start(int max_inflight, uint64_t budget_bytes);
-int svm_server_submit(const uint8_t* pipelineinfo_pb, size_t len, uint64_t* ticket);
-/* Waits up to timeout_ms (<0: forever); returns SVM_ERR_BUSY on timeout. */
+/* ---- serving: continuous batching against device-resident tables ----
+ * A query is submitted as its plans (dimension pipelines, then the fact pipeline
+ * last).  Queries waiting for admission are taken as HBM budget (maps, group
+ * tables, descriptors) and worker slots free up, fused into one shared scan per
+ * batch, and run on `workers` streams so batches overlap on the device.
+ * mode 0: continuous batching; mode 1: one query per launch through the plain
+ * kernels (the comparison baseline).  max_batch caps the queries of one batch. */
+int svm_server_start(int workers, uint64_t budget_bytes, int max_batch, int mode);
+int svm_server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64_t* ticket);
+/* Waits up to timeout_ms (<0: forever) for the fact pipeline's result; returns
+ * SVM_ERR_BUSY on timeout.  out may be NULL.  The ticket is released on success. */
 int svm_server_wait(uint64_t ticket, int timeout_ms, svm_result** out);
+/* Queue length, admitted queries, completed queries and HBM bytes in use. */
+int svm_server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes_in_use);
 int svm_server_stop(void);
 
 #ifdef __cplusplus

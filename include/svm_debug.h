@@ -1,0 +1,33 @@
+/*
+ * svm_debug.h — inspection entry points of libsvm.so that need no GPU.
+ *
+ * These have no counterpart in the reference; they expose what the reference
+ * only prints (the decoded plan, core/pipeline.cu:11-80, and the generated code,
+ * test/test.hpp:106-114) so the boundary and the code generator can be checked
+ * on a machine without a GPU.
+ */
+#ifndef SVM_DEBUG_H_
+#define SVM_DEBUG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Decode PipelineInfo bytes and write the operator chain as JSON into buf:
+ * id, pre ids, filter predicates with their lowered HIP text, select / metric
+ * lists, reduce and join keys, literal parameters, output schema. */
+int svm_debug_describe_plan(const uint8_t* pipelineinfo_pb, size_t len, char* buf, size_t cap);
+
+/* Generate representative kernels (scalar / dense / hash / row-id scans and a
+ * select) and compile them with hipRTC for gfx950; writes a report, returns 0
+ * when all compile. */
+int svm_debug_jit_selftest(char* buf, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVM_DEBUG_H_ */

@@ -45,6 +45,7 @@ SIGNATURES = {
     "svm_table_column_device": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(c_vp)]),
     "svm_table_info": (ctypes.c_int, [ctypes.c_char_p, c_u64p, ctypes.POINTER(ctypes.c_int)]),
     "svm_table_column_type": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_i32p, c_i64p]),
+    "svm_table_view": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64]),
     "svm_table_drop": (ctypes.c_int, [ctypes.c_char_p]),
     "svm_table_exists": (ctypes.c_int, [ctypes.c_char_p]),
     "svm_gen_table": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
@@ -62,9 +63,11 @@ SIGNATURES = {
     "svm_result_free": (None, [c_vp]),
     "svm_batch_run": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_u8p), ctypes.POINTER(ctypes.c_size_t),
                                      ctypes.POINTER(c_vp)]),
-    "svm_server_start": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64]),
-    "svm_server_submit": (ctypes.c_int, [c_u8p, ctypes.c_size_t, c_u64p]),
+    "svm_server_start": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]),
+    "svm_server_submit": (ctypes.c_int, [ctypes.POINTER(c_u8p), ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
+                                         c_u64p]),
     "svm_server_wait": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "svm_server_stats": (ctypes.c_int, [c_u64p, c_u64p, c_u64p, c_u64p]),
     "svm_server_stop": (ctypes.c_int, []),
 }
 

@@ -6,9 +6,10 @@
 
 namespace svm {
 int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs);
-int server_start(int max_inflight, uint64_t budget);
-int server_submit(const uint8_t* pb, size_t len, uint64_t* ticket);
+int server_start(int workers, uint64_t budget, int max_batch, int mode);
+int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64_t* ticket);
 int server_wait(uint64_t ticket, int timeout_ms, svm_result** out);
+int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes);
 int server_stop();
 }  // namespace svm
 
@@ -93,6 +94,7 @@ int svm_table_put_column(const char* id, int col, const void* host_src, uint64_t
     uint64_t need = t->rows * (uint64_t)c.size;
     if (bytes != need) fail(SVM_ERR_INVALID, "column needs " + std::to_string(need) + " bytes, got " + std::to_string(bytes));
     if (!c.dev) fail(SVM_ERR_INVALID, "column is not materialised");
+    if (!c.owns) fail(SVM_ERR_INVALID, "'" + std::string(id) + "' is a view and read-only");
     if (bytes) SVM_HIP(hipMemcpy(c.dev, host_src, bytes, hipMemcpyHostToDevice));
   });
 }
@@ -106,7 +108,7 @@ int svm_table_get_column(const char* id, int col, void* host_dst, uint64_t bytes
     uint64_t have = t->rows * (uint64_t)c.size;
     if (bytes > have) fail(SVM_ERR_INVALID, "column holds only " + std::to_string(have) + " bytes");
     if (!c.dev) fail(SVM_ERR_INVALID, "column is not materialised");
-    if (bytes) SVM_HIP(hipMemcpy(host_dst, c.dev, bytes, hipMemcpyDeviceToHost));
+    if (bytes) SVM_HIP(hipMemcpy(host_dst, t->col_ptr(col), bytes, hipMemcpyDeviceToHost));
   });
 }
 
@@ -115,7 +117,7 @@ int svm_table_column_device(const char* id, int col, void** dev_ptr) {
     need_id(id);
     TablePtr t = engine().store.get(id);
     if (col < 0 || col >= (int)t->cols.size() || !dev_ptr) fail(SVM_ERR_INVALID, "bad column");
-    *dev_ptr = t->cols[col].dev;
+    *dev_ptr = (void*)t->col_ptr(col);
   });
 }
 
@@ -135,6 +137,15 @@ int svm_table_column_type(const char* id, int col, int32_t* type, int64_t* val_s
     if (col < 0 || col >= (int)t->cols.size()) fail(SVM_ERR_INVALID, "column index out of range");
     if (type) *type = t->cols[col].type;
     if (val_size) *val_size = t->cols[col].size;
+  });
+}
+
+int svm_table_view(const char* id, const char* base_id, uint64_t row0, uint64_t rows) {
+  return guard([&] {
+    need_id(id);
+    need_id(base_id);
+    Engine& E = engine();
+    E.store.put(id, make_view(id, E.store.get(base_id), row0, rows));
   });
 }
 
@@ -225,21 +236,74 @@ int svm_batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_re
   return g != SVM_OK ? g : rc;
 }
 
-int svm_server_start(int max_inflight, uint64_t budget_bytes) {
-  return guard([&] { server_start(max_inflight, budget_bytes); });
+int svm_server_start(int workers, uint64_t budget_bytes, int max_batch, int mode) {
+  return guard([&] { server_start(workers, budget_bytes, max_batch, mode); });
 }
-int svm_server_submit(const uint8_t* pb, size_t len, uint64_t* ticket) {
-  int rc = SVM_OK;
-  int g = guard([&] { rc = server_submit(pb, len, ticket); });
-  return g != SVM_OK ? g : rc;
+int svm_server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64_t* ticket) {
+  return guard([&] { server_submit(plans, lens, n, ticket); });
 }
 int svm_server_wait(uint64_t ticket, int timeout_ms, svm_result** out) {
+  if (out) *out = nullptr;
   int rc = SVM_OK;
   int g = guard([&] { rc = server_wait(ticket, timeout_ms, out); });
+  if (g == SVM_OK && rc == SVM_ERR_BUSY) t_err = "timed out waiting for ticket";
   return g != SVM_OK ? g : rc;
+}
+int svm_server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes_in_use) {
+  return guard([&] { server_stats(queued, running, done, bytes_in_use); });
 }
 int svm_server_stop(void) {
   return guard([&] { server_stop(); });
+}
+
+// Debug: the decoded operator chain of a PipelineInfo as JSON, with every
+// expression lowered against the Source operator's declared schema (object 0).
+int svm_debug_describe_plan(const uint8_t* pb, size_t len, char* buf, size_t cap) {
+  return guard([&] {
+    PipelineInfo info = decode_pipeline_info(pb, len);
+    Plan p = decode_plan(info);
+    auto q = [](const std::string& s) {
+      std::string o = "\"";
+      for (char ch : s) {
+        if (ch == '"' || ch == '\\') o += '\\';
+        if ((unsigned char)ch < 0x20) { o += ' '; continue; }
+        o += ch;
+      }
+      return o + "\"";
+    };
+    Schemas sch(1);
+    for (auto& c : p.source_info.column_list) sch[0].push_back({c.type, c.val_size});
+    std::string j = "{\"id\":" + q(p.id) + ",\"source_type\":" + q(p.source_type) + ",\"pre\":[";
+    for (size_t k = 0; k < info.pre_pipeline_id.size(); ++k) j += (k ? "," : "") + q(info.pre_pipeline_id[k]);
+    j += "],\"isGPU\":" + std::string(info.isGPU ? "true" : "false") + ",\"filter\":[";
+    Literals lits;
+    for (size_t k = 0; k < p.filter_exp.size(); ++k) {
+      std::string low;
+      try { low = lower_expr(p.filter_exp[k], sch, false, &lits).code; } catch (const Error& e) { low = std::string("!") + e.what(); }
+      j += (k ? "," : "") + std::string("[") + q(p.filter_exp[k]) + "," + q(low) + "]";
+    }
+    j += "],\"select\":[";
+    for (size_t k = 0; k < p.select_exp.size(); ++k) j += (k ? "," : "") + q(p.select_exp[k]);
+    j += "],\"metric\":[";
+    for (size_t k = 0; k < p.metric_exp.size(); ++k) j += (k ? "," : "") + q(p.metric_exp[k]);
+    auto ki = [](const std::vector<KeyIndex>& v) {
+      std::string o = "[";
+      for (size_t k = 0; k < v.size(); ++k) o += (k ? "," : "") + std::string("[") + std::to_string(v[k].obj) + "," + std::to_string(v[k].col) + "]";
+      return o + "]";
+    };
+    j += "],\"reduce_key\":" + ki(p.reduce_key) + ",\"join_lkey\":" + ki(p.join_lkey) +
+         ",\"join_rkey\":" + ki(p.join_rkey) + ",\"is_reduce\":" + (p.is_reduce ? "true" : "false") +
+         ",\"has_loader\":" + (p.has_loader ? "true" : "false") + ",\"int_literals\":[";
+    for (size_t k = 0; k < lits.ints.size(); ++k) j += (k ? "," : "") + std::to_string(lits.ints[k]);
+    j += "],\"string_literals\":" + q(lits.blob) + ",\"output\":[";
+    for (size_t k = 0; k < p.output_info.column_list.size(); ++k)
+      j += (k ? "," : "") + std::string("[") + std::to_string(p.output_info.column_list[k].type) + "," +
+           std::to_string(p.output_info.column_list[k].val_size) + "]";
+    j += "],\"reencoded_len\":" + std::to_string(encode_pipeline_info(info).size()) + "}";
+    if (!buf || cap == 0) fail(SVM_ERR_INVALID, "null output buffer");
+    if (j.size() + 1 > cap) fail(SVM_ERR_INVALID, "output buffer too small");
+    memcpy(buf, j.c_str(), j.size() + 1);
+  });
 }
 
 }  // extern "C"

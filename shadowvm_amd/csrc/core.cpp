@@ -32,7 +32,23 @@ void dev_free(void* p) {
 }
 
 Table::~Table() {
-  for (auto& c : cols) dev_free(c.dev);
+  for (auto& c : cols)
+    if (c.owns) dev_free(c.dev);
+}
+
+TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t rows) {
+  if (row0 + rows > base->rows) fail(SVM_ERR_INVALID, "view rows exceed table '" + base->id + "'");
+  auto v = std::make_shared<Table>();
+  v->id = id;
+  v->rows = rows;
+  v->row0 = base->row0 + row0;
+  v->max_tuple_count = base->max_tuple_count;
+  v->cols = base->cols;
+  for (auto& c : v->cols) c.owns = false;
+  v->info = base->info;
+  v->info.id = id;
+  v->base = base->base ? base->base : base;
+  return v;
 }
 
 TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,

@@ -94,7 +94,7 @@ void build_key_dict(const Table& D, int col, KeyBuild& kb, bool want_codes, DevB
     fail(SVM_ERR_UNSUPPORTED, "group keys must be integer or string columns");
   uint64_t n = D.rows;
   DevBuf vals(n * 4 + 4);
-  k::values_of(c.dev, c.type, c.size, n, vals.as<int32_t>(), s);
+  k::values_of(D.col_ptr(col), c.type, c.size, n, vals.as<int32_t>(), s);
   kb.dict = DevBuf(n * 4 + 4);
   kb.n = (uint32_t)k::dict_build(vals.as<int32_t>(), n, kb.dict.as<int32_t>(), s);
   if (want_codes) {
@@ -186,7 +186,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     uint64_t maxval = payload == PAY_EXIST ? 1 : payload == PAY_CODE ? keys[comps[0]].n : D.rows;
     jb.shape.wbits = payload == PAY_ROW ? 32 : width_for(maxval);
     // dense map over [kmin, kmax]
-    const void* kcol = D.cols[R.col].dev;
+    const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
     if (D.rows) {
       auto mm = col_minmax(kcol, D.rows, c.s);
@@ -227,6 +227,7 @@ void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
                       const std::map<std::pair<int, int>, int>& dcol_slot, const Literals& lits,
                       DevBuf& blob) {
   memset(&a, 0, sizeof(a));
+  // source columns by their base address; the kernels scan rows [row0, row0 + rows)
   for (size_t q = 0; q < c.src->cols.size() && q < 24; ++q) a.ptr[q] = c.src->cols[q].dev;
   for (size_t j = 0; j < joins.size(); ++j) {
     a.ptr[SVM_SLOT_MAP + j] = joins[j].packed.p;
@@ -238,7 +239,7 @@ void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
   for (auto& kv : dcol_slot) {
     const Table& D = *c.refs[kv.first.first - 1];
     if (!D.cols[kv.first.second].dev) fail(SVM_ERR_INVALID, "dimension column is not materialised");
-    a.ptr[kv.second] = D.cols[kv.first.second].dev;
+    a.ptr[kv.second] = D.col_ptr(kv.first.second);
   }
   for (size_t q = 0; q < lits.ints.size() && q < SVM_MAX_LIT; ++q) a.pi[q] = lits.ints[q];
   if (!lits.blob.empty()) {
@@ -248,7 +249,7 @@ void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
     SVM_HIP(hipMemcpyAsync(blob.p, lits.blob.data(), lits.blob.size(), hipMemcpyHostToDevice, c.s));
     a.sblob = blob.as<const char>();
   }
-  a.row0 = 0;
+  a.row0 = (long long)c.src->row0;
   a.rows = (long long)c.src->rows;
 }
 
@@ -366,7 +367,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   // source-column keys: code = value - min over the column
   for (int q = 0; q < K; ++q) {
     if (keys[q].shape.src != KEY_FACT) continue;
-    auto mm = c.src->rows ? col_minmax(c.src->cols[keys[q].shape.col].dev, c.src->rows, c.s)
+    auto mm = c.src->rows ? col_minmax(c.src->col_ptr(keys[q].shape.col), c.src->rows, c.s)
                           : std::make_pair(0, -1);
     uint64_t range = (uint64_t)((int64_t)mm.second - (int64_t)mm.first + 1);
     if (range > kMaxFactKeyRange) fail(SVM_ERR_UNSUPPORTED, "source-column group key range exceeds 2^24");

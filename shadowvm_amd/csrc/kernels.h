@@ -28,6 +28,8 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
 // Packed map: slot = 0 if first == INT32_MAX else 1 / codes[row]+1 / row+1 by payload.
 void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
               uint32_t* packed, hipStream_t s);
+// True when two rows of keys[0..n) share a key (keys within [kmin, kmin+range)); synchronises.
+bool has_duplicates(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, hipStream_t s);
 // out[i] = sum(in[0..i)), *total = sum(in) (single workgroup).
 void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* total, hipStream_t s);

@@ -98,6 +98,15 @@ __global__ void k_first_rows(const int32_t* __restrict__ keys, uint64_t n, int32
   }
 }
 
+__global__ void k_dup_flag(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
+                           const int32_t* __restrict__ first, uint32_t* flag) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t off = (uint32_t)(keys[r] - kmin);
+    if (off < range && first[off] != (int32_t)r) *flag = 1u;
+  }
+}
+
 __global__ void k_pack(const int32_t* __restrict__ first, uint32_t range, int payload,
                        const int32_t* __restrict__ codes, int wbits, uint32_t* packed, uint64_t nwords) {
   const int per = 32 / wbits;
@@ -255,6 +264,19 @@ void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* 
   if (!nwords) return;
   k_pack<<<grid_for(nwords), kBlock, 0, s>>>(first, range, payload, codes, wbits, packed, nwords);
   LAUNCH_CHECK("pack_map");
+}
+
+bool has_duplicates(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, hipStream_t s) {
+  if (!n || !range) return false;
+  DevBuf first((uint64_t)range * 4), flag(4);
+  first_rows(keys, n, kmin, range, first.as<int32_t>(), s);
+  fill_u32(flag.p, 1, 0, s);
+  k_dup_flag<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first.as<int32_t>(), flag.as<uint32_t>());
+  LAUNCH_CHECK("has_duplicates");
+  uint32_t h = 0;
+  SVM_HIP(hipMemcpyAsync(&h, flag.p, 4, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return h != 0;
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,

@@ -67,16 +67,27 @@ struct Column {
   int64_t size = 4;
   void* dev = nullptr;      // nullptr: declared but not materialised (e.g. a hash-table column)
   uint64_t bytes = 0;       // allocation size (data + padding)
+  bool owns = true;         // false for the columns of a view
 };
+struct Table;
+using TablePtr = std::shared_ptr<Table>;
 struct Table {
   std::string id;
   uint64_t rows = 0;
+  uint64_t row0 = 0;        // views: first row inside the base allocation (column dev = base)
   uint64_t max_tuple_count = 0;
   std::vector<Column> cols;
   ObjectInfo info;          // schema as ObjectInfo (ref object info_)
+  TablePtr base;            // views keep their base table alive
+  // address of row 0 of column c (views offset into the base allocation)
+  const char* col_ptr(int c) const {
+    return cols[c].dev ? (const char*)cols[c].dev + row0 * (uint64_t)cols[c].size : nullptr;
+  }
   ~Table();
 };
-using TablePtr = std::shared_ptr<Table>;
+
+// Zero-copy view of rows [row0, row0+rows) of `base` (row0 relative to base).
+TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t rows);
 
 // Column allocations carry this many zero bytes after the last row, so fixed-width
 // copies that read past a row (StrCopy of a wider projection) and 128-bit vector

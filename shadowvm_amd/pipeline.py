@@ -80,6 +80,63 @@ def table_column(id, col):
     return _view(buf, t.value, s.value)
 
 
+def table_view(id, base_id, row0, rows):
+    """Zero-copy view of rows [row0, row0+rows) of `base_id`."""
+    _lib.check(_lib.lib().svm_table_view(id.encode(), base_id.encode(), row0, rows))
+
+
+def batch_run(plans, want_result=True):
+    """svm_batch_run over plan dicts; returns (results list, number fused).
+    Absorbed dimension pipelines give None."""
+    datas = [serialize(p) for p in plans]
+    n = len(datas)
+    bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
+    arr = (ctypes.POINTER(ctypes.c_uint8) * n)(*[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
+    lens = (ctypes.c_size_t * n)(*[len(d) for d in datas])
+    outs = (ctypes.c_void_p * n)()
+    fused = _lib.check(_lib.lib().svm_batch_run(n, arr, lens, outs if want_result else None))
+    res = [(_take_result(outs[i]) if outs[i] else None) for i in range(n)] if want_result else None
+    return res, fused
+
+
+def _plan_array(plans):
+    datas = [serialize(p) for p in plans]
+    bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
+    arr = (ctypes.POINTER(ctypes.c_uint8) * len(datas))(
+        *[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
+    lens = (ctypes.c_size_t * len(datas))(*[len(d) for d in datas])
+    return arr, lens, bufs
+
+
+def server_start(workers=2, budget=0, max_batch=64, mode=0):
+    """Serving loop (mode 0 continuous batching, mode 1 one query per plain launch)."""
+    _lib.check(_lib.lib().svm_server_start(workers, budget, max_batch, mode))
+
+
+def server_submit(plans):
+    """Submit one query (its dimension pipelines, then the fact pipeline); returns a ticket."""
+    arr, lens, keep = _plan_array(plans)
+    t = ctypes.c_uint64()
+    _lib.check(_lib.lib().svm_server_submit(arr, lens, len(plans), ctypes.byref(t)))
+    return int(t.value)
+
+
+def server_wait(ticket, timeout_ms=-1):
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().svm_server_wait(ticket, timeout_ms, ctypes.byref(out)))
+    return _take_result(out) if out.value else None
+
+
+def server_stats():
+    v = [ctypes.c_uint64() for _ in range(4)]
+    _lib.check(_lib.lib().svm_server_stats(*[ctypes.byref(x) for x in v]))
+    return dict(zip(("queued", "running", "done", "bytes_in_use"), (x.value for x in v)))
+
+
+def server_stop():
+    _lib.check(_lib.lib().svm_server_stop())
+
+
 def table_drop(id):
     _lib.check(_lib.lib().svm_table_drop(id.encode()))
 

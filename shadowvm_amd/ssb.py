@@ -182,6 +182,179 @@ def four_way(seed=1, fact_rows=600_000_000, n_cust=3_000_000, n_supp=200_000, n_
                     {"year": year, "c_nation": cn, "s_nation": sn})
 
 
+@dataclass
+class Query:
+    """One star-join query of a batch: its dimension pipelines, the fact pipeline
+    over a zero-copy slice (view) of the fact table, and the slice itself."""
+    qid: str
+    view: tuple           # (view id, base fact id, row0, rows)
+    dims: list            # (PipelineTest, source id, ref ids)
+    fact: tuple           # (PipelineTest, source id, ref ids)
+    shape: dict
+
+    def plans(self):
+        return [pt.plan(src, refs) for pt, src, refs in self.dims] + [self.fact[0].plan(self.fact[1], self.fact[2])]
+
+
+# Dimension choices of the template generator over the four-way tables:
+# (filter template, approximate selectivity, projected key column, key cardinality)
+_GEO = [("nation", 2, 25), ("city", 1, 250), ("region", 3, 5)]
+
+
+def batch_queries(wl: Workload, n=64, seed=1, window=60_000_000, min_rows=1_000_000, prefix="bq"):
+    """Seeded template generator of batched star-join queries over the four-way
+    tables: 1-4 joins, selectivities about 1e-4..0.5, group cardinality 1..100K,
+    fact slices of U[min_rows, window] rows inside the first `window` rows."""
+    import random
+    rng = random.Random(_mix(seed, 77))
+    ids = {t.kind: t.id for t in wl.tables}
+    fact_id = ids[GEN_FACT3]
+    fact_rows = wl.fact_rows
+    window = min(window, fact_rows)
+    min_rows = min(min_rows, window)
+    out = []
+    for qn in range(n):
+        qid = f"{prefix}{qn}"
+        rows = rng.randint(min_rows, window)
+        row0 = rng.randint(0, window - rows)
+        view = (f"{qid}_slice", fact_id, row0, rows)
+        nj = rng.randint(1, 4)
+        dims = rng.sample(["date", "customer", "supplier", "part"], nj)
+        order = ["date", "customer", "supplier", "part"]
+        dims.sort(key=order.index)
+        max_groups = rng.choice([1, 10, 100, 1000, 10_000, 100_000])
+        groups = 1
+        dim_specs, keys = [], []
+        for d in dims:
+            if d == "date":
+                span = rng.choice([0, 1, 2, 6])
+                y = rng.randint(1992, 1998 - span)
+                filt = f"$(0.1)>={y} && $(0.1)<={y + span}" if span else f"$(0.1)=={y}"
+                card, keycol = span + 1, 1
+                dim_specs.append((d, ids[GEN_DATE], [I, I, I], [4, 4, 4], filt, "$(0.0), $(0.1)",
+                                  [I, I, S], [4, 4, 40]))
+            elif d in ("customer", "supplier"):
+                attr, col, card = rng.choice(_GEO[:2])
+                pick = rng.randint(0, card - 1)
+                lit = f"NATION_{pick:02d}" if attr == "nation" else f"CITY_{(pick // 50) * 50 + pick % 50:03d}"
+                filt = f'StrCmp($(0.{col}),"{lit}",10)==0'
+                key_attr = rng.choice([1, 2, 3])  # city / nation / region of the passing rows
+                card = {1: 250, 2: 25, 3: 5}[key_attr]
+                keycol = 1
+                src_id = ids[GEN_CUST3] if d == "customer" else ids[GEN_SUPP3]
+                dim_specs.append((d, src_id, [I, S, S, S], [4, 10, 10, 10], filt, f"$(0.0), $(0.{key_attr})",
+                                  [I, S, S], [4, 10, 40]))
+            else:
+                cat = rng.randint(1, 25)
+                filt = f"$(0.1)=={cat}" if rng.random() < 0.7 else f"$(0.1)<={cat}"
+                card, keycol = 1000, 1
+                dim_specs.append((d, ids[GEN_PART3], [I, I, I], [4, 4, 4], filt, "$(0.0), $(0.2)",
+                                  [I, I, S], [4, 4, 40]))
+            if groups * card <= max_groups and len(keys) < 3:
+                keys.append((len(dim_specs), keycol))
+                groups *= card
+        # optional fact predicate on revenue / supplycost to spread selectivity
+        fact_filters = []
+        r = rng.random()
+        if r < 0.3:
+            hi = rng.randint(10_000, 10_000_000)
+            fact_filters.append(f"$(0.4)<={hi}")
+        elif r < 0.45:
+            lo = rng.randint(1_000, 119_000)
+            fact_filters.append(f"$(0.5)>={lo} && $(0.5)<{lo + rng.randint(100, 60_000)}")
+        metric = rng.choice(["$(0.4)", "$(0.4)-$(0.5)", "$(0.5)"])
+        dpts, refs = [], []
+        for k, (d, src_id, st, ss, filt, sel, ot, os_) in enumerate(dim_specs):
+            pt = P.PipelineTest(f"{qid}_{d}")
+            osch = P.create_schema(ot, os_)
+            pt.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema(st, ss))
+            pt.AddOperator(P.FILTER, {"predication": filt}, osch)
+            pt.AddOperator(P.MAP, {"select": sel}, osch)
+            pt.AddOperator(P.SINK, {}, osch)
+            dpts.append((pt, src_id, []))
+            refs.append(pt.pipe_info["id"])
+        lk = {"date": 0, "customer": 1, "supplier": 2, "part": 3}
+        res = P.create_schema([I] * 12, [4] * 12)
+        fq = P.PipelineTest(qid)
+        fq.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I] * 6, [4] * 6))
+        for f in fact_filters:
+            fq.AddOperator(P.FILTER, {"predication": f}, res)
+        for k, d in enumerate(dims):
+            fq.AddOperator(P.JOINWITH, {"lkey": f"$(0.{lk[d]})", "rkey": f"$({k + 1}.0)"}, res)
+        dim_s = ", ".join(f"$({o}.{c})" for o, c in keys)
+        fq.AddOperator(P.REDUCEBY, {"dim": dim_s, "metric": metric} if keys else {"dim": "", "metric": metric}, res)
+        fq.AddOperator(P.SINK, {}, res)
+        out.append(Query(qid, view, dpts, (fq, view[0], refs),
+                         {"joins": nj, "keys": len(keys), "groups_max": groups, "rows": rows}))
+    return out
+
+
+def serve_queries(wl: Workload, n, seed=1, light_frac=0.9, light_rows=60_000_000, prefix="sq"):
+    """Serving mix over the four-way tables: light queries (one join, <= 0.1% of
+    rows pass, <= 100 groups, a slice of at most `light_rows` rows anywhere in the
+    fact table) and heavy queries (the full four-way star join over the whole
+    fact table, thousands of groups)."""
+    import random
+    rng = random.Random(_mix(seed, 99))
+    ids = {t.kind: t.id for t in wl.tables}
+    fact_id, fact_rows = ids[GEN_FACT3], wl.fact_rows
+    out = []
+    for qn in range(n):
+        qid = f"{prefix}{qn}"
+        if rng.random() >= light_frac:
+            # heavy: the four-way star join with seeded constants, whole table
+            h = four_way(seed=rng.randint(1, 1 << 30), fact_rows=fact_rows, prefix=qid)
+            year, cn, sn = h.constants["year"], h.constants["c_nation"], h.constants["s_nation"]
+            fq, _, refs = h.fact
+            dims = []
+            for (pt, _, _), src in zip(h.dims, [ids[GEN_DATE], ids[GEN_CUST3], ids[GEN_SUPP3], ids[GEN_PART3]]):
+                dims.append((pt, src, []))
+            view = (f"{qid}_slice", fact_id, 0, fact_rows)
+            out.append(Query(qid, view, dims, (fq, view[0], refs), {"kind": "heavy", "rows": fact_rows}))
+            continue
+        rows = rng.randint(min(1_000_000, light_rows), min(light_rows, fact_rows))
+        row0 = rng.randint(0, fact_rows - rows)
+        view = (f"{qid}_slice", fact_id, row0, rows)
+        which = rng.choice(["date", "customer", "supplier"])
+        res = P.create_schema([I] * 12, [4] * 12)
+        pt = P.PipelineTest(f"{qid}_{which}")
+        if which == "date":
+            y = rng.randint(1992, 1998)
+            dsch = P.create_schema([I, I, S], [4, 4, 40])
+            pt.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I] * 3, [4] * 3))
+            pt.AddOperator(P.FILTER, {"predication": f"$(0.1)=={y}"}, dsch)
+            pt.AddOperator(P.MAP, {"select": "$(0.0), $(0.2)"}, dsch)  # key + month (12 groups)
+            lkey, src = 0, ids[GEN_DATE]
+            fact_filter = f"$(0.4)<={rng.randint(1_000, 60_000)}"   # ~0.1..0.6% of rows, times 1/7
+        else:
+            c = rng.randint(0, 249)
+            dsch = P.create_schema([I, S, S], [4, 10, 40])
+            pt.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I, S, S, S], [4, 10, 10, 10]))
+            pt.AddOperator(P.FILTER, {"predication": f'StrCmp($(0.1),"CITY_{c:03d}",10)==0'}, dsch)
+            pt.AddOperator(P.MAP, {"select": "$(0.0), $(0.2)"}, dsch)  # key + nation
+            lkey = 1 if which == "customer" else 2
+            src = ids[GEN_CUST3] if which == "customer" else ids[GEN_SUPP3]
+            fact_filter = f"$(0.4)<={rng.randint(100_000, 2_000_000)}"  # 1..20% of rows, times ~1/250
+        pt.AddOperator(P.SINK, {}, dsch)
+        fq = P.PipelineTest(qid)
+        fq.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I] * 6, [4] * 6))
+        fq.AddOperator(P.FILTER, {"predication": fact_filter}, res)
+        fq.AddOperator(P.JOINWITH, {"lkey": f"$(0.{lkey})", "rkey": "$(1.0)"}, res)
+        fq.AddOperator(P.REDUCEBY, {"dim": "$(1.1)", "metric": rng.choice(["$(0.4)", "$(0.4)-$(0.5)"])}, res)
+        fq.AddOperator(P.SINK, {}, res)
+        out.append(Query(qid, view, [(pt, src, [])], (fq, view[0], [pt.pipe_info["id"]]),
+                         {"kind": "light", "rows": rows}))
+    return out
+
+
+def make_views(queries):
+    for q in queries:
+        vid, base, row0, rows = q.view
+        if P.table_exists(vid):
+            P.table_drop(vid)
+        P.table_view(vid, base, row0, rows)
+
+
 def load_device(wl: Workload):
     """Generate every table of the workload directly in HBM."""
     for t in wl.tables:

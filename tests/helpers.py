@@ -15,13 +15,13 @@ def host_tables(wl, threads=8, tuple_per_bulk=3_000_000):
     return tabs
 
 
-def oracle_query(wl, tabs, tuple_per_bulk=3_000_000, threads=8):
+def oracle_query(wl, tabs, tuple_per_bulk=3_000_000, threads=8, dim_tuple_per_bulk=None):
     """Run the workload's dimension pipelines then its fact pipeline on the oracle.
     Returns (fact result, {dim pipeline id: select result})."""
     dims = {}
     for pt, src, refs in wl.dims:
         r = O.run_pipeline(pt.plan(src, refs), tabs[src], [dims[x].table for x in refs],
-                           tuple_per_bulk=tuple_per_bulk, threads=threads)
+                           tuple_per_bulk=dim_tuple_per_bulk or tuple_per_bulk, threads=threads)
         dims[pt.pipe_info["id"]] = r
     fq, src, refs = wl.fact
     res = O.run_pipeline(fq.plan(src, refs), tabs[src], [dims[x].table for x in refs],
