@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-rows", type=int, default=60_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -103,12 +104,30 @@ def main():
         f"scan {scan_avg:.3f} ms ({achieved:.0f} GB/s)  finish {sum(finish_ms)/len(finish_ms):.3f} ms  "
         f"groups {r.rows}")
 
+    if args.calib:
+        # a streaming copy of one fact column (read n, write n bytes): the reference
+        # dispatch for scaling the FETCH_SIZE / WRITE_SIZE counters of svm_scan
+        import ctypes
+        from shadowvm_amd import _lib
+        P.table_create("calib_dst", [P.INT32], [4], rows)
+        L = ctypes.CDLL(_lib.LIB_PATH)
+        L.svm_debug_copy_column.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        _lib.check(L.svm_debug_copy_column(b"calib_dst", 0, wl.fact[1].encode(), 4))
+        log(f"[calib] svm_calib_copy moved {rows * 4} bytes each way")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(wl, min(args.cpu_sample_rows, rows), threads)
         log(f"[cpu] {cpu['value']:.3e} fact rows/s on {threads} threads")
     P.shutdown()
+
+    # HBM bytes per svm_scan launch from the committed FETCH_SIZE / WRITE_SIZE counter
+    # runs of this same command (scaled by a copy kernel of known size)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "svm_scan_four_way_600M_traffic.json")
+    if rows == 600_000_000 and os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes"]
 
     out = {
         "metric": "fact rows/s + % HBM roofline per star-join query; batched p50/p99 latency",
@@ -127,7 +146,7 @@ def main():
                                "(d_year, c_city, s_city), SUM(revenue-supplycost)",
                    "fact_rows": rows, "groups": r.rows, "constants": wl.constants},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "svm_scan", "kernel_ms": scan_avg},
         "cpu_baseline": cpu,
     }

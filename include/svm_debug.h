@@ -26,6 +26,11 @@ int svm_debug_describe_plan(const uint8_t* pipelineinfo_pb, size_t len, char* bu
  * when all compile. */
 int svm_debug_jit_selftest(char* buf, size_t cap);
 
+/* Copy column `src_col` of table `src_id` into column `dst_col` of `dst_id` (same
+ * rows and width) with a plain 128-bit streaming copy kernel (svm_calib_copy).
+ * A kernel of known traffic (read n, write n bytes) to calibrate HBM counters. */
+int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, int src_col);
+
 #ifdef __cplusplus
 }
 #endif

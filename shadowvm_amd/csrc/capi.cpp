@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "exec.h"
+#include "kernels.h"
 #include "svm_internal.h"
 
 namespace svm {
@@ -254,6 +255,22 @@ int svm_server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64
 }
 int svm_server_stop(void) {
   return guard([&] { server_stop(); });
+}
+
+int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, int src_col) {
+  return guard([&] {
+    need_id(dst_id);
+    need_id(src_id);
+    Engine& E = engine();
+    TablePtr d = E.store.get(dst_id), s = E.store.get(src_id);
+    if (dst_col < 0 || dst_col >= (int)d->cols.size() || src_col < 0 || src_col >= (int)s->cols.size())
+      fail(SVM_ERR_INVALID, "column index out of range");
+    uint64_t bytes = s->rows * (uint64_t)s->cols[src_col].size;
+    if (d->rows * (uint64_t)d->cols[dst_col].size != bytes || !d->cols[dst_col].owns)
+      fail(SVM_ERR_INVALID, "destination column must be an owned column of the same byte size");
+    k::calib_copy(s->col_ptr(src_col), d->cols[dst_col].dev, bytes & ~15ull, E.stream);
+    SVM_HIP(hipStreamSynchronize(E.stream));
+  });
 }
 
 // Debug: the decoded operator chain of a PipelineInfo as JSON, with every

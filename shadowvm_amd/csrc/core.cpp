@@ -31,6 +31,20 @@ void dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
 
+void* dev_alloc_async(uint64_t bytes, hipStream_t s) {
+  void* p = nullptr;
+  hipError_t e = hipMallocAsync(&p, bytes ? bytes : 1, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(SVM_ERR_NO_MEMORY, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void dev_free_async(void* p, hipStream_t s) {
+  if (p) (void)hipFreeAsync(p, s);
+}
+
 Table::~Table() {
   for (auto& c : cols)
     if (c.owns) dev_free(c.dev);
@@ -130,6 +144,14 @@ void engine_init(const Config& cfg) {
   e->num_cus = prop.multiProcessorCount;
   e->max_lds = (int)prop.sharedMemPerBlock;
   SVM_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  // keep freed stream-ordered memory in the pool instead of returning it to the
+  // driver at every synchronisation
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, cfg.device) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  (void)hipGetLastError();
   g_engine = std::move(e);
 }
 

@@ -66,7 +66,8 @@ struct KeyBuild {
   KeyShape shape;
   uint32_t n = 0, stride = 1;
   int32_t fmin = 0;
-  DevBuf dict, codes;
+  std::shared_ptr<DictEntry> dict;  // sorted values of the key column's origin
+  DevBuf codes;                     // code of every dimension row (row-id payloads)
 };
 
 int width_for(uint64_t maxval) {
@@ -76,31 +77,22 @@ int width_for(uint64_t maxval) {
   return 32;
 }
 
-std::pair<int32_t, int32_t> col_minmax(const void* col, uint64_t n, hipStream_t s) {
-  DevBuf mm(8);
-  k::minmax_i32((const int32_t*)col, n, mm.as<int32_t>(), s);
-  int32_t h[2];
-  SVM_HIP(hipMemcpyAsync(h, mm.p, 8, hipMemcpyDeviceToHost, s));
-  SVM_HIP(hipStreamSynchronize(s));
-  return {h[0], h[1]};
-}
-
-// Dictionary of one group-key component of a dimension and each dimension row's code.
-void build_key_dict(const Table& D, int col, KeyBuild& kb, bool want_codes, DevBuf* codes_out,
-                    hipStream_t s) {
-  const Column& c = D.cols[col];
+// Group-key component of a dimension: the dictionary of the column's origin (the
+// base table it was projected from, cached across queries, so no host round trip)
+// and, by binary search, the code of every row of this dimension object.
+void build_key_dict(const TablePtr& D, int col, KeyBuild& kb, DevBuf* codes_out, hipStream_t s) {
+  const Column& c = D->cols[col];
   if (!c.dev) fail(SVM_ERR_INVALID, "group key column is not materialised");
   if (c.type != Int32 && c.type != String && c.type != Int16 && c.type != Byte)
     fail(SVM_ERR_UNSUPPORTED, "group keys must be integer or string columns");
-  uint64_t n = D.rows;
-  DevBuf vals(n * 4 + 4);
-  k::values_of(D.col_ptr(col), c.type, c.size, n, vals.as<int32_t>(), s);
-  kb.dict = DevBuf(n * 4 + 4);
-  kb.n = (uint32_t)k::dict_build(vals.as<int32_t>(), n, kb.dict.as<int32_t>(), s);
-  if (want_codes) {
-    *codes_out = DevBuf(n * 4 + 4);
-    k::codes_of(vals.as<int32_t>(), n, kb.dict.as<int32_t>(), kb.n, codes_out->as<int32_t>(), s);
-  }
+  auto root = column_root(D, col);
+  kb.dict = table_dict(root.first, root.second, s);
+  kb.n = kb.dict->n;
+  uint64_t n = D->rows;
+  DevBuf vals(n * 4 + 4, s);
+  k::values_of(D->col_ptr(col), c.type, c.size, n, vals.as<int32_t>(), s);
+  *codes_out = DevBuf(n * 4 + 4, s);
+  k::codes_of(vals.as<int32_t>(), n, kb.dict->dict.as<int32_t>(), kb.n, codes_out->as<int32_t>(), s);
 }
 
 struct Ctx {
@@ -174,37 +166,39 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     for (auto& o : expr_refs) refs = refs || o.first == R.obj;
     int payload = refs || comps.size() > 1 ? PAY_ROW : comps.size() == 1 ? PAY_CODE : PAY_EXIST;
     // key dictionaries of this dimension
+    const TablePtr& Dp = c.refs[R.obj - 1];
     DevBuf codes_for_pack;
     for (int q : comps) {
       KeyBuild& kb = keys[q];
       kb.shape.join = (int)j;
       kb.shape.src = payload == PAY_CODE ? KEY_DIMCODE : KEY_DIMROW;
-      if (payload == PAY_CODE) build_key_dict(D, kb.shape.col, kb, true, &codes_for_pack, c.s);
-      else build_key_dict(D, kb.shape.col, kb, true, &kb.codes, c.s);
+      build_key_dict(Dp, kb.shape.col, kb, payload == PAY_CODE ? &codes_for_pack : &kb.codes, c.s);
     }
     jb.shape.payload = payload;
     uint64_t maxval = payload == PAY_EXIST ? 1 : payload == PAY_CODE ? keys[comps[0]].n : D.rows;
     jb.shape.wbits = payload == PAY_ROW ? 32 : width_for(maxval);
-    // dense map over [kmin, kmax]
+    // dense map over the key range of the column's origin (a superset of this
+    // dimension object's keys, cached per resident table: no host round trip)
     const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
     if (D.rows) {
-      auto mm = col_minmax(kcol, D.rows, c.s);
-      uint64_t range = (uint64_t)((int64_t)mm.second - (int64_t)mm.first + 1);
+      auto root = column_root(Dp, R.col);
+      KeyEntry ke = table_key(root.first, root.second, c.s);
+      uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
       if (range > kMaxDenseRange)
         fail(SVM_ERR_UNSUPPORTED, "join key range " + std::to_string(range) + " of '" + D.id + "' exceeds the dense map limit");
-      jb.kmin = mm.first;
+      jb.kmin = ke.kmin;
       jb.range = (uint32_t)range;
     }
     uint64_t per = 32 / (uint64_t)jb.shape.wbits;
     jb.words = ((uint64_t)jb.range + per - 1) / per;
-    jb.packed = DevBuf(jb.words * 4 + 16);
+    jb.packed = DevBuf(jb.words * 4 + 16, c.s);
     if (jb.range) {
-      DevBuf first((uint64_t)jb.range * 4);
+      DevBuf first((uint64_t)jb.range * 4, c.s);
       k::first_rows((const int32_t*)kcol, D.rows, jb.kmin, jb.range, first.as<int32_t>(), c.s);
       k::pack_map(first.as<int32_t>(), jb.range, payload, codes_for_pack.as<int32_t>(), jb.shape.wbits,
                   jb.packed.as<uint32_t>(), c.s);
-      SVM_HIP(hipStreamSynchronize(c.s));  // `first` and the codes are freed on return
+      // `first` and the codes are released in stream order after pack_map
     }
     joins.push_back(std::move(jb));
   }
@@ -244,7 +238,7 @@ void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
   for (size_t q = 0; q < lits.ints.size() && q < SVM_MAX_LIT; ++q) a.pi[q] = lits.ints[q];
   if (!lits.blob.empty()) {
     // zero tail: a string literal may be read up to a projection's width (<= 4096)
-    blob = DevBuf(lits.blob.size() + 4096);
+    blob = DevBuf(lits.blob.size() + 4096, c.s);
     k::fill_u32(blob.p, blob.bytes / 4, 0, c.s);
     SVM_HIP(hipMemcpyAsync(blob.p, lits.blob.data(), lits.blob.size(), hipMemcpyHostToDevice, c.s));
     a.sblob = blob.as<const char>();
@@ -367,11 +361,12 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   // source-column keys: code = value - min over the column
   for (int q = 0; q < K; ++q) {
     if (keys[q].shape.src != KEY_FACT) continue;
-    auto mm = c.src->rows ? col_minmax(c.src->col_ptr(keys[q].shape.col), c.src->rows, c.s)
-                          : std::make_pair(0, -1);
-    uint64_t range = (uint64_t)((int64_t)mm.second - (int64_t)mm.first + 1);
+    // range of the source column's origin (cached per resident table)
+    auto root = column_root(c.src, keys[q].shape.col);
+    KeyEntry ke = c.src->rows ? table_key(root.first, root.second, c.s) : KeyEntry();
+    uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
     if (range > kMaxFactKeyRange) fail(SVM_ERR_UNSUPPORTED, "source-column group key range exceeds 2^24");
-    keys[q].fmin = mm.first;
+    keys[q].fmin = ke.kmin;
     keys[q].n = (uint32_t)range;
   }
   uint64_t G = 1;
@@ -400,7 +395,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
 
   // outputs
   const uint64_t nflag = (G + 31) / 32;
-  DevBuf gsum(K ? G * M * 8 + 8 : 8), gflag(nflag * 4 + 4), ssum(8 * 8);
+  DevBuf gsum(K ? G * M * 8 + 8 : 8, c.s), gflag(nflag * 4 + 4, c.s), ssum(8 * 8, c.s);
   if (K) {
     k::fill_u32(gsum.p, G * M * 2, 0, c.s);
     k::fill_u32(gflag.p, nflag, 0, c.s);
@@ -439,16 +434,16 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   uint64_t out_rows = 1;
   TablePtr out;
   if (K == 0) {
-    out = c.E.store.create(c.info.id, oschema, 1);
+    out = c.E.store.create(c.info.id, oschema, 1, nullptr, c.s);
     for (int m = 0; m < M; ++m)
       SVM_HIP(hipMemcpyAsync(out->cols[m].dev, (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));
   } else {
     uint64_t nb = (nflag + 255) / 256;
-    DevBuf blocks(nb * 8 + 8), offs(nb * 8 + 16);
+    DevBuf blocks(nb * 8 + 8, c.s), offs(nb * 8 + 16, c.s);
     out_rows = G ? k::group_count(gflag.as<uint32_t>(), G, blocks.as<unsigned long long>(),
                                   offs.as<unsigned long long>(), c.s)
                  : 0;
-    out = c.E.store.create(c.info.id, oschema, out_rows);
+    out = c.E.store.create(c.info.id, oschema, out_rows, nullptr, c.s);
     if (out_rows) {
       k::EmitArgs ea;
       memset(&ea, 0, sizeof(ea));
@@ -461,7 +456,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         ea.n[q] = keys[q].n;
         ea.stride[q] = keys[q].stride;
         ea.fmin[q] = keys[q].fmin;
-        ea.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict.as<int32_t>();
+        ea.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict->dict.as<int32_t>();
         ea.keys_out[q] = (int32_t*)out->cols[q].dev;
       }
       for (int m = 0; m < M; ++m) ea.sums_out[m] = (int64_t*)out->cols[K + m].dev;

@@ -51,6 +51,8 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scra
                      unsigned long long* scratch_offsets, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
 
+// Streaming 16-byte copy of `bytes` (a multiple of 16) — counter calibration.
+void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Synthetic table rows [row0, row0+n) of `kind` into device columns.
 void gen_table(const svm_gen_ctx& ctx, int kind, uint64_t row0, uint64_t n, void* const* cols,
                hipStream_t s);

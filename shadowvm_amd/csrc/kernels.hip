@@ -193,6 +193,12 @@ __global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ 
   }
 }
 
+__global__ void svm_calib_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 struct GenCols {
   void* c[SVM_GEN_MAX_COLS];
 };
@@ -305,6 +311,13 @@ void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_
   if (!nb) return;
   k_group_emit<<<(unsigned)nb, kBlock, 0, s>>>(a, offsets);
   LAUNCH_CHECK("group_emit");
+}
+
+void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  uint64_t n16 = bytes / 16;
+  if (!n16) return;
+  svm_calib_copy<<<grid_for(n16, kBlock, 8192), kBlock, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
+  LAUNCH_CHECK("calib_copy");
 }
 
 void gen_table(const svm_gen_ctx& ctx, int kind, uint64_t row0, uint64_t n, void* const* cols,

@@ -62,15 +62,20 @@ PipelineInfo decode_pipeline_info(const uint8_t* data, size_t len);
 std::string encode_pipeline_info(const PipelineInfo& p);
 
 // ----------------------------------------------------------------- store
+struct Table;
+using TablePtr = std::shared_ptr<Table>;
 struct Column {
   int type = Int32;
   int64_t size = 4;
   void* dev = nullptr;      // nullptr: declared but not materialised (e.g. a hash-table column)
   uint64_t bytes = 0;       // allocation size (data + padding)
   bool owns = true;         // false for the columns of a view
+  // A bare-column projection of a select pipeline holds a subset of its source
+  // column's values: origin names that column (key ranges / dictionaries of the
+  // origin bound this column's).
+  std::weak_ptr<Table> origin;
+  int origin_col = -1;
 };
-struct Table;
-using TablePtr = std::shared_ptr<Table>;
 struct Table {
   std::string id;
   uint64_t rows = 0;
@@ -109,21 +114,40 @@ class Store {
 };
 
 // ----------------------------------------------------------------- device memory
-void* dev_alloc(uint64_t bytes);  // hipMalloc with error mapping
+void* dev_alloc(uint64_t bytes);  // hipMalloc with error mapping (long-lived buffers)
 void dev_free(void* p);
+// Stream-ordered allocation from the device memory pool (no implicit device
+// synchronisation on free; memory is reused by later work on any stream once the
+// freeing stream has passed the free).
+void* dev_alloc_async(uint64_t bytes, hipStream_t s);
+void dev_free_async(void* p, hipStream_t s);
+// Temporary device buffer.  With a stream it is stream-ordered: allocated and freed
+// in that stream's order; without one it is a plain hipMalloc allocation.
 struct DevBuf {
   void* p = nullptr;
   uint64_t bytes = 0;
+  hipStream_t s = nullptr;
+  bool async = false;
   DevBuf() = default;
   explicit DevBuf(uint64_t b) : p(b ? dev_alloc(b) : nullptr), bytes(b) {}
+  DevBuf(uint64_t b, hipStream_t st) : p(b ? dev_alloc_async(b, st) : nullptr), bytes(b), s(st), async(true) {}
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), s(o.s), async(o.async) { o.p = nullptr; o.bytes = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { dev_free(p); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes; s = o.s; async = o.async;
+      o.p = nullptr; o.bytes = 0;
+    }
     return *this;
   }
-  ~DevBuf() { dev_free(p); }
+  void release() {
+    if (!p) return;
+    if (async) dev_free_async(p, s); else dev_free(p);
+    p = nullptr;
+  }
+  ~DevBuf() { release(); }
   template <typename T> T* as() const { return (T*)p; }
 };
 
