@@ -60,6 +60,116 @@ def cpu_baseline(wl, sample_rows, threads):
                       f"per-bulk reduce + merge (best of 2)"}
 
 
+def _pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(p / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def _same(a, b):
+    import numpy as np
+    return a.rows == b.rows and all(np.array_equal(x, y) for x, y in zip(a.columns, b.columns))
+
+
+def bench_batched(wl, nq, seed, reps=3):
+    """64 seeded template queries over the four-way tables (fact slices of 1M..60M
+    rows inside the first 60M rows): one svm_batch_run (shared scan) against the
+    same queries run one after another through the plain per-query kernels."""
+    from shadowvm_amd import pipeline as P
+    from shadowvm_amd import ssb
+    qs = ssb.batch_queries(wl, n=nq, seed=seed, prefix="bb")
+    ssb.make_views(qs)
+    plans, pos = [], []
+    for q in qs:
+        plans += q.plans()
+        pos.append(len(plans) - 1)
+
+    def run_plain():
+        lat, res = [], []
+        for q in qs:
+            t0 = time.perf_counter()
+            for pt, src, refs in q.dims:
+                pt.Run(src, refs, want_result=False)
+            pt, src, refs = q.fact
+            res.append(pt.Run(src, refs))
+            lat.append(time.perf_counter() - t0)
+        return lat, res
+
+    run_plain()
+    P.batch_run(plans)  # warm: caches and code objects
+    plain_t, batch_t = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lat, pres = run_plain()
+        plain_t.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        bres, fused = P.batch_run(plans)
+        batch_t.append(time.perf_counter() - t0)
+    exact = all(_same(bres[p], r) for p, r in zip(pos, pres))
+    tp, tb = sorted(plain_t)[len(plain_t) // 2], sorted(batch_t)[len(batch_t) // 2]
+    log(f"[batch] {nq} queries: plain {tp*1e3:.1f} ms, batched {tb*1e3:.1f} ms ({fused} fused), bit-exact={exact}")
+    return {"queries": nq, "fused": fused, "bit_exact_vs_plain": exact,
+            "batched_qps": nq / tb, "plain_qps": nq / tp, "speedup": tp / tb,
+            "batched_ms": tb * 1e3, "plain_ms": tp * 1e3,
+            "batched_p50_ms": tb * 1e3, "batched_p99_ms": tb * 1e3,
+            "plain_p50_ms": _pct(lat, 50) * 1e3, "plain_p99_ms": _pct(lat, 99) * 1e3,
+            "workload": "seeded template generator: 1-4 joins over the four-way dimensions, selectivity "
+                        "~1e-4..0.5, 1..100K groups, fact slices U[1M,60M] rows in the first 60M rows; "
+                        "all queries submitted at once (latency = batch completion)"}
+
+
+def bench_served(wl, rate, seconds, seed, mode, workers=4):
+    """Open-loop Poisson arrivals of the serving mix (90% light / 10% heavy
+    four-way queries) against the resident tables; returns qps and latency."""
+    import random
+    import threading
+    from shadowvm_amd import pipeline as P
+    from shadowvm_amd import ssb
+    n = max(1, int(rate * seconds))
+    qs = ssb.serve_queries(wl, n, seed=seed, prefix=f"s{mode}_")
+    ssb.make_views(qs)
+    plans = [q.plans() for q in qs]
+    rng = random.Random(seed)
+    arrivals, t = [], 0.0
+    for _ in range(n):
+        t += rng.expovariate(rate)
+        arrivals.append(t)
+    P.server_start(workers=workers, budget=32 << 30, max_batch=64, mode=mode)
+    done = [None] * n
+    tickets = [None] * n
+    submitted = threading.Semaphore(0)
+
+    def waiter():
+        for i in range(n):
+            submitted.acquire()
+            P.server_wait(tickets[i], timeout_ms=-1)
+            done[i] = time.perf_counter()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    sub = [0.0] * n
+    for i in range(n):
+        now = time.perf_counter() - t0
+        if arrivals[i] > now:
+            time.sleep(arrivals[i] - now)
+        sub[i] = t0 + arrivals[i]
+        tickets[i] = P.server_submit(plans[i])
+        submitted.release()
+    th.join()
+    t_end = max(done)
+    P.server_stop()
+    lat = [(d - s) * 1e3 for d, s in zip(done, sub)]
+    out = {"offered_qps": rate, "queries": n, "achieved_qps": n / (t_end - t0),
+           "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "workers": workers,
+           "heavy": sum(1 for q in qs if q.shape["kind"] == "heavy")}
+    log(f"[serve] mode {mode}: offered {rate} q/s, achieved {out['achieved_qps']:.0f} q/s, "
+        f"p50 {out['p50_ms']:.2f} ms p99 {out['p99_ms']:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +180,11 @@ def main():
     ap.add_argument("--cpu-sample-rows", type=int, default=60_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
+    ap.add_argument("--batch-queries", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -115,6 +230,12 @@ def main():
         _lib.check(L.svm_debug_copy_column(b"calib_dst", 0, wl.fact[1].encode(), 4))
         log(f"[calib] svm_calib_copy moved {rows * 4} bytes each way")
 
+    batched = None if args.no_batch else bench_batched(wl, args.batch_queries, args.seed)
+    served = None
+    if not args.no_serve:
+        served = {"batching": bench_served(wl, args.serve_rate, args.serve_seconds, args.seed, mode=0),
+                  "plain": bench_served(wl, args.serve_rate, args.serve_seconds, args.seed, mode=1)}
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -149,6 +270,8 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "svm_scan", "kernel_ms": scan_avg},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

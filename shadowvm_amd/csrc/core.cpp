@@ -66,7 +66,7 @@ TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t
 }
 
 TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,
-                       uint64_t rows, const std::vector<bool>* materialise) {
+                       uint64_t rows, const std::vector<bool>* materialise, hipStream_t fill_stream) {
   auto t = std::make_shared<Table>();
   t->id = id;
   t->rows = rows;
@@ -81,7 +81,10 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
     if (mat) {
       col.bytes = rows * (uint64_t)col.size + kColumnPad;
       col.dev = dev_alloc(col.bytes);
-      SVM_HIP(hipMemset(col.dev, 0, col.bytes));
+      if (fill_stream)
+        SVM_HIP(hipMemsetAsync((char*)col.dev + rows * (uint64_t)col.size, 0, kColumnPad, fill_stream));
+      else
+        SVM_HIP(hipMemset(col.dev, 0, col.bytes));
     }
     t->cols.push_back(col);
     ColumnInfo ci;

@@ -18,6 +18,7 @@
 #include "jit.h"
 #include "kargs.h"
 #include "kernels.h"
+#include "stats.h"
 
 namespace svm {
 
@@ -540,7 +541,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   const uint64_t rows = c.src->rows;
   const uint64_t tile = (uint64_t)ks.block * 4;
   const uint64_t tiles = (rows + tile - 1) / tile;
-  DevBuf counts(tiles * 4 + 4), offs(tiles * 8 + 16);
+  DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
   SvmArgs a;
   DevBuf blob;
   fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
@@ -568,9 +569,22 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     oschema.push_back({outcols[q].type, vs});
     mat.push_back((int)q < P);
   }
-  TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat);
+  TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat, c.s);
   out->max_tuple_count = c.E.cfg.tuple_per_bulk;
-  for (int q = 0; q < P; ++q) a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
+  for (int q = 0; q < P; ++q) {
+    a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
+    // a bare source column projected unchanged keeps a subset of that column's values
+    int obj = -1, col = -1;
+    std::string e = p.select_exp[q];
+    size_t b = e.find_first_not_of(" \t"), t = e.find_last_not_of(" \t");
+    if (b != std::string::npos && sscanf(e.substr(b, t - b + 1).c_str(), "$(%d.%d)", &obj, &col) == 2 &&
+        e.substr(b, t - b + 1) == "$(" + std::to_string(obj) + "." + std::to_string(col) + ")" && obj == 0 &&
+        col < (int)c.schemas[0].size() && c.schemas[0][col].first == oschema[q].first &&
+        c.schemas[0][col].second == oschema[q].second) {
+      out->cols[q].origin = c.src;
+      out->cols[q].origin_col = col;
+    }
+  }
   if (tiles && total) {
     hipFunction_t fwrite = jit_function(src, "svm_sel_write");
     launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);

@@ -101,8 +101,12 @@ constexpr uint64_t kColumnPad = 256;
 
 class Store {
  public:
+  // Allocates the columns.  Without a stream every byte is zeroed synchronously
+  // (tables filled by the caller); with a stream only the padding after the last
+  // row is zeroed, asynchronously, for outputs whose rows a kernel writes in full.
   TablePtr create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,
-                  uint64_t rows, const std::vector<bool>* materialise = nullptr);
+                  uint64_t rows, const std::vector<bool>* materialise = nullptr,
+                  hipStream_t fill_stream = nullptr);
   void put(const std::string& id, TablePtr t);
   TablePtr get(const std::string& id);  // throws SVM_ERR_NOT_FOUND
   TablePtr find(const std::string& id);
