@@ -185,6 +185,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-pipeline timing lines from libsvm")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -194,7 +195,7 @@ def main():
     from shadowvm_amd import pipeline as P
     from shadowvm_amd import ssb
 
-    P.init(device=0, tuple_per_bulk=3_000_000)
+    P.init(device=0, tuple_per_bulk=3_000_000, verbose=1 if args.verbose else 0)
     wl = ssb.four_way(seed=args.seed, fact_rows=args.fact_rows)
     t0 = time.perf_counter()
     ssb.load_device(wl)

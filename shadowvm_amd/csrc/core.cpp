@@ -14,21 +14,110 @@ void hip_check(hipError_t e, const char* what) {
   fail(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-static std::atomic<uint64_t> g_live_bytes{0};
+// Caching allocator: allocations below 1 GiB are rounded up to a power of two and
+// returned to a free list on release instead of to the driver (hipFree
+// synchronises the device, which would stall every stream in flight); larger ones
+// (resident fact columns) go straight to hipMalloc / hipFree.
+namespace {
+std::mutex g_alloc_mu;
+std::map<uint64_t, std::vector<void*>> g_free_lists;  // bucket size -> free blocks
+std::map<void*, uint64_t> g_bucket_of;                // live cached block -> bucket size
+constexpr uint64_t kCacheLimit = 1ull << 30;
 
-void* dev_alloc(uint64_t bytes) {
+uint64_t bucket_for(uint64_t b) {
+  uint64_t s = 256;
+  while (s < b) s <<= 1;
+  return s;
+}
+
+void* raw_alloc(uint64_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     fail(SVM_ERR_NO_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
   }
-  g_live_bytes += bytes;
+  return p;
+}
+}  // namespace
+
+void* dev_alloc(uint64_t bytes) {
+  if (bytes >= kCacheLimit) return raw_alloc(bytes);
+  uint64_t b = bucket_for(bytes);
+  {
+    std::lock_guard<std::mutex> g(g_alloc_mu);
+    auto& fl = g_free_lists[b];
+    if (!fl.empty()) {
+      void* p = fl.back();
+      fl.pop_back();
+      g_bucket_of[p] = b;
+      return p;
+    }
+  }
+  void* p = raw_alloc(b);
+  std::lock_guard<std::mutex> g(g_alloc_mu);
+  g_bucket_of[p] = b;
   return p;
 }
 
 void dev_free(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> g(g_alloc_mu);
+    auto it = g_bucket_of.find(p);
+    if (it != g_bucket_of.end()) {
+      g_free_lists[it->second].push_back(p);
+      g_bucket_of.erase(it);
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+namespace {
+struct Arena {
+  std::vector<std::pair<char*, uint64_t>> chunks;
+  size_t cur = 0;
+  uint64_t off = 0;
+};
+std::mutex g_arena_mu;
+std::map<hipStream_t, Arena> g_arenas;
+constexpr uint64_t kArenaChunk = 256ull << 20;
+}  // namespace
+
+void* arena_alloc(hipStream_t s, uint64_t bytes) {
+  bytes = (bytes + 255) & ~255ull;
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  Arena& a = g_arenas[s];
+  while (a.cur < a.chunks.size()) {
+    if (a.off + bytes <= a.chunks[a.cur].second) {
+      char* p = a.chunks[a.cur].first + a.off;
+      a.off += bytes;
+      return p;
+    }
+    a.cur++;
+    a.off = 0;
+  }
+  uint64_t sz = bytes > kArenaChunk ? bytes : kArenaChunk;
+  char* p = (char*)raw_alloc(sz);
+  a.chunks.push_back({p, sz});
+  a.cur = a.chunks.size() - 1;
+  a.off = bytes;
+  return p;
+}
+
+void arena_reset(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  Arena& a = g_arenas[s];
+  a.cur = 0;
+  a.off = 0;
+}
+
+void arena_release_all() {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  for (auto& kv : g_arenas)
+    for (auto& c : kv.second.chunks) (void)hipFree(c.first);
+  g_arenas.clear();
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {
@@ -163,6 +252,7 @@ void engine_shutdown() {
   if (!g_engine) return;
   (void)hipStreamSynchronize(g_engine->stream);
   g_engine->store.clear();
+  arena_release_all();
   (void)hipStreamDestroy(g_engine->stream);
   g_engine.reset();
 }

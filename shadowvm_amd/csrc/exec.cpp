@@ -47,6 +47,10 @@ struct Event {
   ~Event() { if (e) (void)hipEventDestroy(e); }
 };
 
+double host_ms(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 float elapsed(const Event& a, const Event& b) {
   float ms = 0;
   SVM_HIP(hipEventElapsedTime(&ms, a.e, b.e));
@@ -298,6 +302,38 @@ unsigned scan_grid(Engine& E, uint64_t vec, int block, int unroll, uint64_t lds_
   return (unsigned)(g ? g : 1);
 }
 
+// ------------------------------------------------------------------ autotuner
+// The scan's launch configuration (workgroup size, 128-bit groups per thread per
+// iteration, workgroups per CU) is chosen per kernel shape and source size by
+// timing the candidates on the first large run of that shape; integer results do
+// not depend on it, so the accumulators are simply cleared between trials.
+struct TuneChoice {
+  int block = 256, unroll = 1, bpc = 0;  // bpc 0: as many as LDS allows (<= 8)
+  bool stage = true;                      // small join maps copied into LDS
+};
+struct TuneState {
+  bool done = false;
+  TuneChoice best;
+  std::vector<std::pair<TuneChoice, float>> trials;
+};
+std::mutex g_tune_mu;
+std::map<uint64_t, TuneState> g_tune;
+const TuneChoice kTuneCandidates[] = {{256, 1, 0, true},  {256, 2, 0, true},  {512, 1, 0, true},
+                                      {256, 1, 0, false}, {256, 2, 0, false}, {512, 1, 0, false},
+                                      {512, 2, 0, false}, {256, 2, 8, false}, {1024, 1, 0, false}};
+constexpr uint64_t kTuneMinRows = 1ull << 22;
+
+uint64_t text_hash(const std::string& s) {
+  uint64_t h = 1469598103934665603ull;
+  for (unsigned char ch : s) { h ^= ch; h *= 1099511628211ull; }
+  return h;
+}
+
+bool autotune_enabled() {
+  const char* e = getenv("SVM_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
 // ------------------------------------------------------------------ reduce path
 std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   const Plan& p = c.plan;
@@ -308,6 +344,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     fail(SVM_ERR_UNSUPPORTED, "ReduceBy supports 1-3 keys x 1-3 metrics (core/pipeline.cu:380-471) or a scalar");
   const auto& outcols = p.output_info.column_list;
   if ((int)outcols.size() < K + M) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than keys + metrics");
+  const auto t_host0 = std::chrono::steady_clock::now();
   Event ev0, ev1, ev2, ev3;
   SVM_HIP(hipEventRecord(ev0.e, c.s));
 
@@ -358,6 +395,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     if (!joined) fail(SVM_ERR_INVALID, "metric reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
   }
   std::vector<JoinBuild> joins = build_joins(c, metric_ops, keys);
+  const double t_build_host = host_ms(t_host0);
   for (auto& jb : joins) ks.joins.push_back(jb.shape);
   // source-column keys: code = value - min over the column
   for (int q = 0; q < K; ++q) {
@@ -418,13 +456,76 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   a.out[2] = ssum.p;
   const uint64_t rows = c.src->rows;
   const bool run_scan = rows > 0 && (K == 0 || G > 0);
+  const uint64_t vec = (rows + 3) / 4;
+  auto clear_outputs = [&] {
+    if (K) {
+      k::fill_u32(gsum.p, G * M * 2, 0, c.s);
+      k::fill_u32(gflag.p, nflag, 0, c.s);
+    }
+    k::fill_u32(ssum.p, 16, 0, c.s);
+  };
+  TuneChoice choice;
+  choice.block = ks.block;
+  choice.unroll = ks.unroll;
+  choice.bpc = opt.blocks_per_cu;
+  // the kernel shape and dynamic LDS of a launch configuration (maps staged or not)
+  const uint64_t agg_words = (agg_bytes + 7) / 8 * 2;
+  auto shape_for = [&](const TuneChoice& t, KernelShape& k2, uint64_t& words) {
+    k2 = ks;
+    k2.block = t.block;
+    k2.unroll = t.unroll;
+    for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
+    words = t.stage ? lds_words : agg_words;
+  };
+  if (run_scan && autotune_enabled() && rows >= kTuneMinRows && opt.block == 256 && opt.unroll == 1) {
+    const uint64_t tkey = text_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
+    std::unique_lock<std::mutex> lk(g_tune_mu);
+    TuneState& st = g_tune[tkey];
+    if (!st.done) {
+      // trial runs of every candidate (the results are discarded)
+      float best = 1e30f;
+      for (const TuneChoice& cand : kTuneCandidates) {
+        KernelShape k2;
+        uint64_t w2 = 0;
+        shape_for(cand, k2, w2);
+        hipFunction_t f2 = jit_function(gen_scan_source(k2), "svm_scan");
+        unsigned g2 = scan_grid(c.E, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
+        clear_outputs();
+        Event ta, tb;
+        SVM_HIP(hipEventRecord(ta.e, c.s));
+        launch(f2, g2, (unsigned)k2.block, (unsigned)(w2 * 4), c.s, a);
+        SVM_HIP(hipEventRecord(tb.e, c.s));
+        SVM_HIP(hipEventSynchronize(tb.e));
+        float ms = elapsed(ta, tb);
+        st.trials.push_back({cand, ms});
+        if (ms < best) { best = ms; st.best = cand; }
+      }
+      st.done = true;
+      clear_outputs();
+      if (c.E.cfg.verbose) {
+        fprintf(stderr, "[svm] autotune %s rows=%llu:", c.info.id.c_str(), (unsigned long long)rows);
+        for (auto& t : st.trials)
+          fprintf(stderr, " (%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc, (int)t.first.stage,
+                  t.second);
+        fprintf(stderr, "\n");
+      }
+    }
+    choice = st.best;
+  }
+  uint64_t run_words = lds_words;
+  {
+    KernelShape k2;
+    shape_for(choice, k2, run_words);
+    ks = k2;
+  }
+  res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps");
   hipFunction_t fn = nullptr;
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
+  const double t_launch_host = host_ms(t_host0);
   SVM_HIP(hipEventRecord(ev1.e, c.s));
   if (run_scan) {
-    uint64_t vec = (rows + 3) / 4;
-    unsigned grid = scan_grid(c.E, vec, ks.block, ks.unroll, lds_words * 4, opt.blocks_per_cu);
-    launch(fn, grid, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    unsigned grid = scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc);
+    launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
   }
   SVM_HIP(hipEventRecord(ev2.e, c.s));
 
@@ -481,6 +582,10 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   res->scan_ms = elapsed(ev1, ev2);
   res->finish_ms = elapsed(ev2, ev3);
   c.E.store.put(c.info.id, out);
+  if (c.E.cfg.verbose)
+    fprintf(stderr, "[svm] %s: host %.3f ms (joins built %.3f, launch %.3f); gpu build %.3f scan %.3f finish %.3f; G=%llu agg=%d %s\n",
+            c.info.id.c_str(), host_ms(t_host0), t_build_host, t_launch_host, res->build_ms, res->scan_ms,
+            res->finish_ms, (unsigned long long)G, ks.agg, res->tune.c_str());
   return res;
 }
 
@@ -492,6 +597,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   const int P = (int)p.select_exp.size();
   if (P > SVM_MAX_SELECT) fail(SVM_ERR_UNSUPPORTED, "at most 10 projected columns");
   if ((int)outcols.size() < P) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than the Map select list");
+  const auto t_host0 = std::chrono::steady_clock::now();
   Event ev0, ev1, ev2, ev3;
   SVM_HIP(hipEventRecord(ev0.e, c.s));
   KernelShape ks;
@@ -606,6 +712,9 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   res->scan_ms = elapsed(ev1, ev2);
   res->finish_ms = elapsed(ev2, ev3);
   c.E.store.put(c.info.id, out);
+  if (c.E.cfg.verbose)
+    fprintf(stderr, "[svm] %s: select host %.3f ms; gpu build %.3f scan %.3f; rows %llu\n", c.info.id.c_str(),
+            host_ms(t_host0), res->build_ms, res->scan_ms, (unsigned long long)total);
   return res;
 }
 
@@ -615,6 +724,9 @@ std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOpti
   Engine& E = engine();
   Plan plan = decode_plan(info);
   Ctx c{E, plan, info, opt.stream ? opt.stream : E.stream, nullptr, {}, {}};
+  // runs on one stream are sequential and each ends synchronised, so the previous
+  // run's temporaries on this stream are free
+  arena_reset(c.s);
   resolve(c);
   if (plan.is_reduce && !plan.select_exp.empty())
     fail(SVM_ERR_UNSUPPORTED, "a pipeline with both Map and ReduceBy is not well defined in the reference");

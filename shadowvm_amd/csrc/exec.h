@@ -13,6 +13,7 @@ struct svm_result {
   std::vector<std::pair<int, int64_t>> types;
   std::vector<std::vector<uint8_t>> cols;
   double build_ms = 0, scan_ms = 0, finish_ms = 0;
+  std::string tune;  // launch configuration chosen for the scan
 };
 
 namespace svm {

@@ -125,8 +125,14 @@ void dev_free(void* p);
 // freeing stream has passed the free).
 void* dev_alloc_async(uint64_t bytes, hipStream_t s);
 void dev_free_async(void* p, hipStream_t s);
-// Temporary device buffer.  With a stream it is stream-ordered: allocated and freed
-// in that stream's order; without one it is a plain hipMalloc allocation.
+// Per-stream bump arena for the temporaries of one pipeline run: allocation is a
+// pointer bump; everything is released at once by arena_reset(s), which a run calls
+// when it starts (the previous run on that stream has synchronised).
+void* arena_alloc(hipStream_t s, uint64_t bytes);
+void arena_reset(hipStream_t s);
+void arena_release_all();
+// Temporary device buffer.  With a stream it lives in that stream's arena until
+// the next arena_reset; without one it is a plain (cached) device allocation.
 struct DevBuf {
   void* p = nullptr;
   uint64_t bytes = 0;
@@ -134,7 +140,7 @@ struct DevBuf {
   bool async = false;
   DevBuf() = default;
   explicit DevBuf(uint64_t b) : p(b ? dev_alloc(b) : nullptr), bytes(b) {}
-  DevBuf(uint64_t b, hipStream_t st) : p(b ? dev_alloc_async(b, st) : nullptr), bytes(b), s(st), async(true) {}
+  DevBuf(uint64_t b, hipStream_t st) : p(b ? arena_alloc(st, b) : nullptr), bytes(b), s(st), async(true) {}
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), s(o.s), async(o.async) { o.p = nullptr; o.bytes = 0; }
@@ -148,7 +154,7 @@ struct DevBuf {
   }
   void release() {
     if (!p) return;
-    if (async) dev_free_async(p, s); else dev_free(p);
+    if (!async) dev_free(p);  // arena memory is reclaimed by arena_reset
     p = nullptr;
   }
   ~DevBuf() { release(); }
