@@ -181,8 +181,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
     ap.add_argument("--batch-queries", type=int, default=64)
-    ap.add_argument("--serve-rate", type=float, default=500.0)
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rates", default="200,1000,2000", help="offered query rates (q/s)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-pipeline timing lines from libsvm")
@@ -234,8 +234,14 @@ def main():
     batched = None if args.no_batch else bench_batched(wl, args.batch_queries, args.seed)
     served = None
     if not args.no_serve:
-        served = {"batching": bench_served(wl, args.serve_rate, args.serve_seconds, args.seed, mode=0),
-                  "plain": bench_served(wl, args.serve_rate, args.serve_seconds, args.seed, mode=1)}
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        served = {"mix": "90% light (1 join, <=0.1% selectivity, <=100 groups, slice <=60M rows) / "
+                         "10% heavy (4-way star join over all 600M rows); open-loop Poisson arrivals; "
+                         "HBM admission budget 32 GB",
+                  "batching": [bench_served(wl, r, args.serve_seconds, args.seed + i, mode=0)
+                               for i, r in enumerate(rates)],
+                  "plain": [bench_served(wl, r, args.serve_seconds, args.seed + i, mode=1)
+                            for i, r in enumerate(rates)]}
 
     cpu = None
     if not args.no_cpu:

@@ -157,15 +157,33 @@ std::string gen_scan_source(const KernelShape& s) {
     for (int m = 0; m < M; ++m) o << "  svm_i64 acc" << m << " = 0;\n";
   o << "  const long long r_lo = a.row0, r_hi = a.row0 + a.rows;\n"
     << "  const long long v_lo = r_lo >> 2, v_hi = (r_hi + 3) >> 2;\n"
-    << "  const long long GT = (long long)gridDim.x * SVM_BLOCK;\n"
-    << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
-    << U << " * GT) {\n";
-  for (int u = 0; u < U; ++u) {
-    o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
-      << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+    << "  const long long GT = (long long)gridDim.x * SVM_BLOCK;\n";
+  const bool pf = s.prefetch && U == 1;
+  if (pf) {
+    // software pipelining: the next iteration's 128-bit groups are requested before
+    // this iteration's rows are probed and aggregated (out-of-range indexes are
+    // clamped to v_lo, a valid group whose rows fail the range test)
+    o << "  long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x;\n";
     for (int c : s.fact_i32)
-      o << "    const svm_i32x4 x" << c << "_" << u
-        << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+      o << "  svm_i32x4 n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
+        << "] + (v0 < v_hi ? v0 : v_lo));\n";
+    o << "  for (; v0 < v_hi; v0 += GT) {\n"
+      << "    const long long v0_ = v0;\n";
+    for (int c : s.fact_i32) o << "    const svm_i32x4 x" << c << "_0 = n" << c << ";\n";
+    o << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
+    for (int c : s.fact_i32)
+      o << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
+    o << "    }\n";
+  } else {
+    o << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
+      << U << " * GT) {\n";
+    for (int u = 0; u < U; ++u) {
+      o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
+        << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+      for (int c : s.fact_i32)
+        o << "    const svm_i32x4 x" << c << "_" << u
+          << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+    }
   }
   for (int u = 0; u < U; ++u) {
     o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"

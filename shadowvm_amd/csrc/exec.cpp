@@ -310,6 +310,7 @@ unsigned scan_grid(Engine& E, uint64_t vec, int block, int unroll, uint64_t lds_
 struct TuneChoice {
   int block = 256, unroll = 1, bpc = 0;  // bpc 0: as many as LDS allows (<= 8)
   bool stage = true;                      // small join maps copied into LDS
+  bool prefetch = false;                  // software-pipelined column loads
 };
 struct TuneState {
   bool done = false;
@@ -318,9 +319,11 @@ struct TuneState {
 };
 std::mutex g_tune_mu;
 std::map<uint64_t, TuneState> g_tune;
-const TuneChoice kTuneCandidates[] = {{256, 1, 0, true},  {256, 2, 0, true},  {512, 1, 0, true},
-                                      {256, 1, 0, false}, {256, 2, 0, false}, {512, 1, 0, false},
-                                      {512, 2, 0, false}, {256, 2, 8, false}, {1024, 1, 0, false}};
+const TuneChoice kTuneCandidates[] = {
+    {256, 1, 0, true, false},  {256, 2, 0, true, false},   {512, 1, 0, true, false},
+    {256, 1, 0, false, false}, {512, 1, 0, false, false},  {512, 2, 0, false, false},
+    {1024, 1, 0, false, false}, {256, 1, 0, true, true},   {512, 1, 0, true, true},
+    {256, 1, 0, false, true},  {512, 1, 0, false, true},   {1024, 1, 0, false, true}};
 constexpr uint64_t kTuneMinRows = 1ull << 22;
 
 uint64_t text_hash(const std::string& s) {
@@ -474,6 +477,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2 = ks;
     k2.block = t.block;
     k2.unroll = t.unroll;
+    k2.prefetch = t.prefetch;
     for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
     words = t.stage ? lds_words : agg_words;
   };
@@ -505,8 +509,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
       if (c.E.cfg.verbose) {
         fprintf(stderr, "[svm] autotune %s rows=%llu:", c.info.id.c_str(), (unsigned long long)rows);
         for (auto& t : st.trials)
-          fprintf(stderr, " (%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc, (int)t.first.stage,
-                  t.second);
+          fprintf(stderr, " (%d,%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc,
+                  (int)t.first.stage, (int)t.first.prefetch, t.second);
         fprintf(stderr, "\n");
       }
     }
@@ -518,7 +522,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     shape_for(choice, k2, run_words);
     ks = k2;
   }
-  res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps");
+  res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
+              (choice.prefetch ? " prefetch" : "");
   hipFunction_t fn = nullptr;
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
   const double t_launch_host = host_ms(t_host0);

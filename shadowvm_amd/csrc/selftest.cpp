@@ -54,6 +54,10 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.agg = AGG_HASH;
     s.hash_bits = 10;
     out.push_back({"scan_hash", gen_scan_source(s)});
+    s.unroll = 1;
+    s.prefetch = true;
+    s.block = 512;
+    out.push_back({"scan_hash_prefetch", gen_scan_source(s)});
   }
   {  // row-id payload, dimension attribute in the metric, source-column key
     KernelShape s;
