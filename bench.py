@@ -137,6 +137,11 @@ def bench_served(wl, rate, seconds, seed, mode, workers=4):
         t += rng.expovariate(rate)
         arrivals.append(t)
     P.server_start(workers=workers, budget=32 << 30, max_batch=64, mode=mode)
+    # warm pass (untimed): code objects and per-table caches of every query shape
+    warm = ssb.serve_queries(wl, 40, seed=seed + 1000, prefix=f"w{mode}_")
+    ssb.make_views(warm)
+    for t in [P.server_submit(q.plans()) for q in warm]:
+        P.server_wait(t, timeout_ms=-1)
     done = [None] * n
     tickets = [None] * n
     submitted = threading.Semaphore(0)

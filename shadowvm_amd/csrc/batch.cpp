@@ -514,12 +514,17 @@ static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr
   memcpy(host + oe, em.data(), em.size() * sizeof(bk::Emit));
   char* dd = (char*)sc.get(4, total + 256);
   SVM_HIP(hipMemcpyAsync(dd, host, total, hipMemcpyHostToDevice, s));
+  hipEvent_t eb, es;
+  SVM_HIP(hipEventCreate(&eb));
+  SVM_HIP(hipEventCreate(&es));
   bk::launch_map_build((const bk::MapBuild*)(dd + ob), (int)builds.size(), max_rows, s);
+  SVM_HIP(hipEventRecord(eb, s));
   const void* cols[bk::kMaxCols] = {nullptr};
   for (size_t c = 0; c < base->cols.size() && c < (size_t)bk::kMaxCols; ++c) cols[c] = base->cols[c].dev;
   if (!items.empty())
     bk::launch_shared_scan((const bk::Query*)(dd + oq), (const bk::Item*)(dd + oi), (unsigned)items.size(),
                            counter, cols, tile, E.num_cus * 4, s);
+  SVM_HIP(hipEventRecord(es, s));
   unsigned long long* counts = (unsigned long long*)sc.get(5, blocks * 8 + 8);
   unsigned long long* offsets = (unsigned long long*)sc.get(6, blocks * 8 + 8);
   unsigned long long* totals = (unsigned long long*)sc.get(7, nq * 8 + 8);
@@ -545,10 +550,23 @@ static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr
   }
   SVM_HIP(hipEventRecord(e1, s));
   SVM_HIP(hipStreamSynchronize(s));
-  float ms = 0;
+  float ms = 0, ms_build = 0, ms_scan = 0;
   SVM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  SVM_HIP(hipEventElapsedTime(&ms_build, e0, eb));
+  SVM_HIP(hipEventElapsedTime(&ms_scan, eb, es));
+  if (E.cfg.verbose) {
+    uint64_t slice_rows = 0, dim_rows = 0;
+    for (auto& q : qd) slice_rows += (uint64_t)(q.row1 - q.row0);
+    for (auto& b : builds) dim_rows += b.n;
+    fprintf(stderr, "[svm] batch of %d: gpu %.3f ms (map builds %.3f over %llu dim rows, shared scan %.3f over "
+                    "%llu slice rows / %zu items, emit %.3f)\n",
+            nq, ms, ms_build, (unsigned long long)dim_rows, ms_scan, (unsigned long long)slice_rows,
+            items.size(), ms - ms_build - ms_scan);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(eb);
+  (void)hipEventDestroy(es);
   for (int i = 0; i < nq; ++i) {
     auto r = std::make_unique<svm_result>();
     const int K = qs[i]->K;
@@ -574,6 +592,10 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
   Engine& E = engine();
   std::lock_guard<std::mutex> g(E.run_mu);
   hipStream_t s = E.stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto host_ms = [&] {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
   std::vector<Job> jobs(n);
   for (int i = 0; i < n; ++i) {
     jobs[i].info = decode_pipeline_info(plans[i], lens[i]);
@@ -608,6 +630,7 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
       for (auto& id : facts[i].deferred)
         if (!absorbed.count(id)) fused[i] = false;  // its dimension is materialised instead
   std::vector<std::unique_ptr<svm_result>> res(n);
+  const double t_lowered = host_ms();
   // plain plans in submission order (absorbed dimension pipelines are skipped)
   for (int i = 0; i < n; ++i) {
     if (fused[i] || absorbed.count(jobs[i].info.id)) continue;
@@ -615,6 +638,7 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
     opt.want_host = outs != nullptr;
     res[i] = run_pipeline(jobs[i].info, opt);
   }
+  const double t_plain = host_ms();
   // fused queries, one shared scan per base fact table
   std::map<const Table*, std::vector<int>> groups;
   for (int i = 0; i < n; ++i)
@@ -633,6 +657,9 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
   for (int i = 0; i < n; ++i) nfused += fused[i] ? 1 : 0;
   if (outs)
     for (int i = 0; i < n; ++i) outs[i] = res[i] ? res[i].release() : nullptr;
+  if (E.cfg.verbose)
+    fprintf(stderr, "[svm] batch_run %d plans, %d fused: host lowering %.3f ms, plain plans %.3f ms, total %.3f ms\n",
+            n, nfused, t_lowered, t_plain - t_lowered, host_ms());
   return nfused;
 }
 
@@ -666,6 +693,7 @@ struct Server {
 std::unique_ptr<Server> g_server;
 std::mutex g_server_mu;
 constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a query on the plain path
+constexpr uint64_t kSharedScanMaxRows = 64ull << 20;  // larger slices take the JIT-specialised path
 
 void run_plain_query(Ticket& t, hipStream_t s) {
   std::unique_ptr<svm_result> last;
@@ -781,6 +809,10 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
     t->fact = lower_fact(t->jobs[n - 1], dims, E, S->submit_stream);
     // every dimension pipeline of the query must be absorbed by the fused fact pipeline
     t->fused = t->fact.ok && t->fact.deferred.size() == (size_t)(n - 1);
+    // a query over a large slice runs faster on its own specialised kernel (the
+    // shared scan interprets descriptors); it still overlaps with batches on
+    // another worker
+    if (t->fused && (uint64_t)(t->fact.q.row1 - t->fact.q.row0) > kSharedScanMaxRows) t->fused = false;
   }
   t->need = t->fused ? fused_bytes(t->fact) : kPlainNeed;
   if (t->need > S->budget) fail(SVM_ERR_BUSY, "query needs more HBM than the serving budget");

@@ -320,11 +320,12 @@ struct TuneState {
 std::mutex g_tune_mu;
 std::map<uint64_t, TuneState> g_tune;
 const TuneChoice kTuneCandidates[] = {
-    {256, 1, 0, true, false},  {256, 2, 0, true, false},   {512, 1, 0, true, false},
-    {256, 1, 0, false, false}, {512, 1, 0, false, false},  {512, 2, 0, false, false},
-    {1024, 1, 0, false, false}, {256, 1, 0, true, true},   {512, 1, 0, true, true},
-    {256, 1, 0, false, true},  {512, 1, 0, false, true},   {1024, 1, 0, false, true}};
-constexpr uint64_t kTuneMinRows = 1ull << 22;
+    {256, 1, 0, true, false},   {256, 2, 0, true, false},  {512, 1, 0, true, false},
+    {512, 1, 0, false, false},  {1024, 1, 0, false, false}, {512, 1, 0, true, true},
+    {512, 1, 0, false, true},   {1024, 1, 0, false, true}};
+// Below this many source rows the launch configuration hardly matters and tuning
+// would only add compiles to a short query's latency.
+constexpr uint64_t kTuneMinRows = 1ull << 26;
 
 uint64_t text_hash(const std::string& s) {
   uint64_t h = 1469598103934665603ull;
