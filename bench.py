@@ -79,21 +79,23 @@ def bench_batched(wl, nq, seed, reps=3):
     same queries run one after another through the plain per-query kernels."""
     from shadowvm_amd import pipeline as P
     from shadowvm_amd import ssb
+    from shadowvm_amd.proto import serialize
     qs = ssb.batch_queries(wl, n=nq, seed=seed, prefix="bb")
     ssb.make_views(qs)
+    # both paths take PipelineInfo bytes, as the engine boundary does
+    per_query = [[serialize(p) for p in q.plans()] for q in qs]
     plans, pos = [], []
-    for q in qs:
-        plans += q.plans()
+    for qp in per_query:
+        plans += qp
         pos.append(len(plans) - 1)
 
     def run_plain():
         lat, res = [], []
-        for q in qs:
+        for qp in per_query:
             t0 = time.perf_counter()
-            for pt, src, refs in q.dims:
-                pt.Run(src, refs, want_result=False)
-            pt, src, refs = q.fact
-            res.append(pt.Run(src, refs))
+            for d in qp[:-1]:
+                P.run_plan(d, want_result=False)
+            res.append(P.run_plan(qp[-1]))
             lat.append(time.perf_counter() - t0)
         return lat, res
 
@@ -117,62 +119,107 @@ def bench_batched(wl, nq, seed, reps=3):
             "plain_p50_ms": _pct(lat, 50) * 1e3, "plain_p99_ms": _pct(lat, 99) * 1e3,
             "workload": "seeded template generator: 1-4 joins over the four-way dimensions, selectivity "
                         "~1e-4..0.5, 1..100K groups, fact slices U[1M,60M] rows in the first 60M rows; "
-                        "all queries submitted at once (latency = batch completion)"}
+                        "all queries submitted at once as PipelineInfo bytes (latency = batch completion)"}
 
 
-def bench_served(wl, rate, seconds, seed, mode, workers=4):
+def _token(i, width):
+    """A unique ASCII id of exactly `width` characters for submission i."""
+    digits = "0123456789abcdefghijklmnopqrstuvwxyz"
+    out = ""
+    for _ in range(width - 1):
+        out = digits[i % 36] + out
+        i //= 36
+    return "u" + out
+
+
+def _renamed(plans, qid, token):
+    """The query's PipelineInfo bytes with its pipeline ids renamed to `token`
+    (same length, so every length prefix stays valid); its slice view keeps its id."""
+    q, t, view, hold = qid.encode(), token.encode(), (qid + "_slice").encode(), b"\x01" * (len(qid) + 6)
+    return [p.replace(view, hold).replace(q, t).replace(hold, view) for p in plans]
+
+
+def bench_served(wl, rate, seconds, seed, mode, workers, pool):
     """Open-loop Poisson arrivals of the serving mix (90% light / 10% heavy
-    four-way queries) against the resident tables; returns qps and latency."""
+    four-way queries) against the resident tables at `rate` q/s for `seconds`.
+    Every submission is a distinct query (pipeline ids renamed per submission);
+    latency is measured by the server from submission to result."""
     import random
     import threading
     from shadowvm_amd import pipeline as P
-    from shadowvm_amd import ssb
+    qs, plans = pool
     n = max(1, int(rate * seconds))
-    qs = ssb.serve_queries(wl, n, seed=seed, prefix=f"s{mode}_")
-    ssb.make_views(qs)
-    plans = [q.plans() for q in qs]
+    width = len(qs[0].qid)
     rng = random.Random(seed)
     arrivals, t = [], 0.0
     for _ in range(n):
         t += rng.expovariate(rate)
         arrivals.append(t)
     P.server_start(workers=workers, budget=32 << 30, max_batch=64, mode=mode)
-    # warm pass (untimed): code objects and per-table caches of every query shape
-    warm = ssb.serve_queries(wl, 40, seed=seed + 1000, prefix=f"w{mode}_")
-    ssb.make_views(warm)
-    for t in [P.server_submit(q.plans()) for q in warm]:
-        P.server_wait(t, timeout_ms=-1)
-    done = [None] * n
+    # warm pass (untimed): code objects, dimension profiles and autotuned shapes
+    warm = [P.server_submit(_renamed(plans[i % len(qs)], qs[i % len(qs)].qid, _token(10_000_000 + i, width)))
+            for i in range(min(len(qs), 200))]
+    for tk in warm:
+        P.server_wait_timed(tk, timeout_ms=-1, want_result=False)
+    lat, qlat = [0.0] * n, [0.0] * n
     tickets = [None] * n
     submitted = threading.Semaphore(0)
+    t_last = [0.0]
 
     def waiter():
         for i in range(n):
             submitted.acquire()
-            P.server_wait(tickets[i], timeout_ms=-1)
-            done[i] = time.perf_counter()
+            _, qlat[i], lat[i] = P.server_wait_timed(tickets[i], timeout_ms=-1, want_result=False)
+        t_last[0] = time.perf_counter()
 
     th = threading.Thread(target=waiter)
     th.start()
     t0 = time.perf_counter()
-    sub = [0.0] * n
+    base = 20_000_000 * (mode + 1) + int(rate) * 1000
     for i in range(n):
         now = time.perf_counter() - t0
         if arrivals[i] > now:
             time.sleep(arrivals[i] - now)
-        sub[i] = t0 + arrivals[i]
-        tickets[i] = P.server_submit(plans[i])
+        k = i % len(qs)
+        tickets[i] = P.server_submit(_renamed(plans[k], qs[k].qid, _token(base + i, width)))
         submitted.release()
+    t_sub = time.perf_counter() - t0
     th.join()
-    t_end = max(done)
     P.server_stop()
-    lat = [(d - s) * 1e3 for d, s in zip(done, sub)]
-    out = {"offered_qps": rate, "queries": n, "achieved_qps": n / (t_end - t0),
-           "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "workers": workers,
-           "heavy": sum(1 for q in qs if q.shape["kind"] == "heavy")}
-    log(f"[serve] mode {mode}: offered {rate} q/s, achieved {out['achieved_qps']:.0f} q/s, "
-        f"p50 {out['p50_ms']:.2f} ms p99 {out['p99_ms']:.2f} ms")
+    out = {"offered_qps": rate, "queries": n, "achieved_qps": n / (t_last[0] - t0),
+           "submit_qps": n / t_sub, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+           "queue_p50_ms": _pct(qlat, 50), "workers": workers}
+    log(f"[serve] mode {mode}: offered {rate:.0f} q/s, achieved {out['achieved_qps']:.0f} q/s, "
+        f"p50 {out['p50_ms']:.2f} ms p99 {out['p99_ms']:.2f} ms (queued p50 {out['queue_p50_ms']:.2f} ms)")
     return out
+
+
+def serve_sweep(wl, rates, seconds, seed, workers_batched, workers_plain, p99_bound_ms):
+    """Both serving modes over increasing offered rates (each mode stops once it
+    falls behind its offered rate or its p99 passes 10x the bound); the capacity of
+    a mode is the largest offered rate it sustains with p99 <= p99_bound_ms."""
+    from shadowvm_amd import ssb
+    from shadowvm_amd.proto import serialize
+    res = {}
+    for mode, workers in ((0, workers_batched), (1, workers_plain)):
+        qs = ssb.serve_queries(wl, 400, seed=seed + mode, prefix=f"p{mode}", id_digits=5)
+        ssb.make_views(qs)
+        pool = (qs, [[serialize(p) for p in q.plans()] for q in qs])  # clients send PipelineInfo bytes
+        runs = []
+        for i, r in enumerate(rates):
+            o = bench_served(wl, r, seconds, seed + 100 * mode + i, mode, workers, pool)
+            runs.append(o)
+            if o["achieved_qps"] < 0.9 * r or o["p99_ms"] > 10 * p99_bound_ms:
+                break
+        ok = [o["offered_qps"] for o in runs if o["p99_ms"] <= p99_bound_ms and o["achieved_qps"] >= 0.95 * o["offered_qps"]]
+        res["batching" if mode == 0 else "plain"] = {"runs": runs, "workers": workers,
+                                                     "capacity_qps": max(ok) if ok else 0.0}
+    cb, cp = res["batching"]["capacity_qps"], res["plain"]["capacity_qps"]
+    res["p99_bound_ms"] = p99_bound_ms
+    res["capacity_ratio"] = cb / cp if cp else None
+    log(f"[serve] capacity at p99 <= {p99_bound_ms:.0f} ms: continuous batching {cb:.0f} q/s, "
+        f"plain {cp:.0f} q/s ({res['capacity_ratio'] or 0:.2f}x)")
+    return res
 
 
 def main():
@@ -186,8 +233,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
     ap.add_argument("--batch-queries", type=int, default=64)
-    ap.add_argument("--serve-rates", default="200,1000,2000", help="offered query rates (q/s)")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-rates", default="500,1000,2000,3000,4000,6000,8000,12000,16000",
+                    help="offered query rates (q/s), swept upward per mode")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-p99-ms", type=float, default=50.0, help="latency bound of the capacity comparison")
+    ap.add_argument("--serve-workers", default="2,4", help="worker streams: continuous batching, plain")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-pipeline timing lines from libsvm")
@@ -240,13 +290,12 @@ def main():
     served = None
     if not args.no_serve:
         rates = [float(r) for r in args.serve_rates.split(",")]
-        served = {"mix": "90% light (1 join, <=0.1% selectivity, <=100 groups, slice <=60M rows) / "
-                         "10% heavy (4-way star join over all 600M rows); open-loop Poisson arrivals; "
-                         "HBM admission budget 32 GB",
-                  "batching": [bench_served(wl, r, args.serve_seconds, args.seed + i, mode=0)
-                               for i, r in enumerate(rates)],
-                  "plain": [bench_served(wl, r, args.serve_seconds, args.seed + i, mode=1)
-                            for i, r in enumerate(rates)]}
+        wb, wp = (int(w) for w in args.serve_workers.split(","))
+        served = serve_sweep(wl, rates, args.serve_seconds, args.seed, wb, wp, args.serve_p99_ms)
+        served["mix"] = ("90% light (1 join, <=0.1% selectivity, <=100 groups, slice <=60M rows) / "
+                         "10% heavy (4-way star join over all 600M rows); open-loop Poisson arrivals; every "
+                         "submission a distinct query; latency measured by the server from submission to "
+                         "result; HBM admission budget 32 GB")
 
     cpu = None
     if not args.no_cpu:

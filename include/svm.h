@@ -105,7 +105,9 @@ void svm_result_free(svm_result* r);
 
 /* ---- batched execution: many star-join queries fused into one shared scan ----
  * plans: dimension and fact pipelines of all queries, in submission order.  Fact
- * pipelines whose operators fit the shared scan (see DESIGN) run fused; their
+ * pipelines whose operators fit the shared scan (range and StrCmp equality atoms,
+ * bare-column projections, equi-joins on unique int32 keys, 0-3 group keys, one
+ * metric a, a-b, a+b or a*b over int32 fact columns) run fused; their
  * dimension pipelines are absorbed (outs[i] is NULL for an absorbed one and
  * its object is not cached); every other plan runs as svm_pipeline_run would.
  * Returns the number of fused fact pipelines (>= 0) or a negative status. */
@@ -123,6 +125,11 @@ int svm_server_submit(const uint8_t* const* plans, const size_t* lens, int n, ui
 /* Waits up to timeout_ms (<0: forever) for the fact pipeline's result; returns
  * SVM_ERR_BUSY on timeout.  out may be NULL.  The ticket is released on success. */
 int svm_server_wait(uint64_t ticket, int timeout_ms, svm_result** out);
+/* svm_server_wait that also reports the query's server-side latency: ms from its
+ * submission to its admission into a batch, and to its result being ready.
+ * Either pointer may be NULL. */
+int svm_server_wait_timed(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms,
+                          double* total_ms);
 /* Queue length, admitted queries, completed queries and HBM bytes in use. */
 int svm_server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes_in_use);
 int svm_server_stop(void);

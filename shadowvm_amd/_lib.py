@@ -67,6 +67,8 @@ SIGNATURES = {
     "svm_server_submit": (ctypes.c_int, [ctypes.POINTER(c_u8p), ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
                                          c_u64p]),
     "svm_server_wait": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "svm_server_wait_timed": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(c_vp),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "svm_server_stats": (ctypes.c_int, [c_u64p, c_u64p, c_u64p, c_u64p]),
     "svm_server_stop": (ctypes.c_int, []),
 }

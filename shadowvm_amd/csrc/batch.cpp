@@ -7,23 +7,33 @@
 //   1. recognises the template forms of the reference language that the shared
 //      scan interprets (range / equality atoms on int columns, StrCmp equality on
 //      strings, bare-column projections, metrics a, a-b, a+b, a*b);
-//   2. lowers each dimension pipeline feeding a fused query into a predicate list
-//      evaluated while building that query's packed map (no materialised object,
-//      no compaction, no host round trip), with group codes from per-table
-//      dictionaries cached across batches;
-//   3. runs one persistent kernel over (tile, query) work items in tile-major
-//      order, so concurrent items read the same tile through L2 / MALL;
-//   4. emits every query's groups, in key order, with two batched kernels.
+//   2. lowers each dimension pipeline feeding a fused query into a list of atoms
+//      (no materialised object, no compaction, no host round trip);
+//   3. groups the batch's joins into slots (dimension table, key column, fact
+//      column); each slot's dimension rows carry a profile — the packed dictionary
+//      codes of the columns the batch's queries filter or group on, deduplicated
+//      and cached with the table — and one small kernel evaluates every query's
+//      atoms on every profile into a 64-bit mask;
+//   4. runs one persistent kernel over tiles of the fact table: each row is read
+//      once, looks up one profile per slot and aggregates every (row, query) pair
+//      that passes into an LDS table keyed by (query, group);
+//   5. emits every query's groups, in key order, with two batched kernels.
 // Plans outside those forms run through the plain per-pipeline path unchanged.
 #include "batch.h"
 
+#include <algorithm>
+#include <array>
+#include <chrono>
 #include <climits>
-#include <cstring>
-#include <regex>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
+#include <regex>
 #include <set>
 #include <thread>
+#include <tuple>
 
 #include "exec.h"
 #include "kernels.h"
@@ -188,13 +198,6 @@ bool parse_metric(const std::string& t, int& op, int& a, int& b) {
   return true;
 }
 
-int width_for(uint64_t maxval) {
-  const int ws[] = {1, 2, 4, 8, 16};
-  for (int w : ws)
-    if (maxval <= ((1ull << w) - 1)) return w;
-  return 32;
-}
-
 // A dimension pipeline the shared scan can absorb: filter + map of bare columns.
 struct DimLowered {
   bool ok = false;
@@ -229,15 +232,155 @@ DimLowered lower_dim(const PipelineInfo& info, const Plan& p, Engine& E) {
   return d;
 }
 
+// ------------------------------------------------------------------ profiles
+// A string column's dictionary holds Hash31 values (the reference's group key of
+// a string); a dimension atom may be evaluated on a code only when every row with
+// that code has the same bytes.  Checked once per dictionary and cached.
+struct StrRep {
+  std::weak_ptr<DictEntry> owner;
+  DevBuf rep;  // first row of each code
+  bool exact = false;
+};
+std::mutex g_prof_mu;
+std::map<const DictEntry*, std::shared_ptr<StrRep>> g_strrep;
+
+std::shared_ptr<StrRep> string_rep(const TablePtr& t, int col, const std::shared_ptr<DictEntry>& d, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  auto it = g_strrep.find(d.get());
+  if (it != g_strrep.end() && it->second->owner.lock() == d) return it->second;
+  auto r = std::make_shared<StrRep>();
+  r->owner = d;
+  r->rep = DevBuf((uint64_t)d->n * 4 + 4);
+  if (t->rows && d->n) {
+    k::first_rows(d->codes.as<int32_t>(), t->rows, 0, d->n, r->rep.as<int32_t>(), s);
+    DevBuf flag(4);
+    k::fill_u32(flag.p, 1, 0, s);
+    bk::launch_string_exact((const char*)t->col_ptr(col), (int)t->cols[col].size, t->rows, d->codes.as<int32_t>(),
+                            r->rep.as<int32_t>(), flag.as<unsigned>(), s);
+    unsigned h = 1;
+    SVM_HIP(hipMemcpyAsync(&h, flag.p, 4, hipMemcpyDeviceToHost, s));
+    SVM_HIP(hipStreamSynchronize(s));
+    r->exact = h == 0;
+  } else {
+    r->exact = true;
+  }
+  g_strrep[d.get()] = r;
+  return r;
+}
+
+int bits_for(uint32_t n) {
+  int b = 0;
+  while (b < 31 && (1ull << b) < n) b++;
+  return b;
+}
+
+// Profiles of one (dimension table, key column, attribute set): every dimension
+// row's attribute codes packed into `raw`, deduplicated into P sorted profiles, and
+// the profile of each key offset.
+struct Profile {
+  std::weak_ptr<Table> owner;
+  std::vector<int> attrs, shift, bits;
+  std::vector<std::shared_ptr<DictEntry>> dicts;
+  uint32_t P = 0;
+  bool ok = false;
+  int32_t kmin = 0;
+  uint32_t range = 0;
+  DevBuf praw;  // P packed codes, ascending
+  DevBuf pkey;  // per key offset: uint8 when narrow (P < 255), else uint16; all ones = no row
+  bool narrow = false;
+};
+using ProfileKey = std::tuple<const Table*, int, std::vector<int>>;
+std::map<ProfileKey, std::shared_ptr<Profile>> g_profiles;
+
+constexpr uint64_t kMaxProfileRange = 1ull << 27;
+
+std::shared_ptr<Profile> profile_of(const TablePtr& t, int kcol, const std::vector<int>& attrs, hipStream_t s) {
+  ProfileKey key(t.get(), kcol, attrs);
+  {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    auto it = g_profiles.find(key);
+    if (it != g_profiles.end() && it->second->owner.lock() == t) return it->second;
+  }
+  auto p = std::make_shared<Profile>();
+  p->owner = t;
+  p->attrs = attrs;
+  KeyEntry ke = table_key(t, kcol, s);
+  const uint64_t n = t->rows;
+  const uint64_t range = n ? (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1) : 0;
+  int total = 0;
+  for (int a : attrs) {
+    auto d = table_dict(t, a, s);
+    p->dicts.push_back(d);
+    p->shift.push_back(total);
+    p->bits.push_back(bits_for(d->n));
+    total += p->bits.back();
+  }
+  p->kmin = ke.kmin;
+  p->range = (uint32_t)std::min<uint64_t>(range, kMaxProfileRange);
+  if (ke.unique && total <= 31 && range <= kMaxProfileRange && (int)attrs.size() <= bk::kMaxAttr) {
+    if (n) {
+      bk::RawArgs ra;
+      memset(&ra, 0, sizeof(ra));
+      ra.nattr = (int)attrs.size();
+      for (size_t k = 0; k < attrs.size(); ++k) {
+        ra.codes[k] = p->dicts[k]->codes.as<int32_t>();
+        ra.shift[k] = p->shift[k];
+      }
+      DevBuf raw(n * 4 + 4), pcode(n * 4 + 4);
+      bk::launch_profile_raw(ra, n, raw.as<int32_t>(), s);
+      DevBuf praw(n * 4 + 4);
+      p->P = (uint32_t)k::dict_build(raw.as<int32_t>(), n, praw.as<int32_t>(), s);
+      if (p->P <= bk::kMaxProfilesPerSlot) {
+        p->narrow = p->P < 0xFFu;
+        const uint64_t pbytes = p->narrow ? range : range * 2;
+        p->pkey = DevBuf(pbytes + 256);
+        // keys without a dimension row (and the entry past the range) hold P
+        const uint32_t fill = p->narrow ? p->P * 0x01010101u : p->P | p->P << 16;
+        k::fill_u32(p->pkey.p, pbytes / 4 + 2, fill, s);
+        k::codes_of(raw.as<int32_t>(), n, praw.as<int32_t>(), p->P, pcode.as<int32_t>(), s);
+        bk::launch_profile_scatter((const int32_t*)t->col_ptr(kcol), n, ke.kmin, pcode.as<int32_t>(), p->pkey.p,
+                                   p->narrow ? 1 : 0, s);
+        p->praw = DevBuf((uint64_t)p->P * 4 + 4);
+        SVM_HIP(hipMemcpyAsync(p->praw.p, praw.p, (uint64_t)p->P * 4, hipMemcpyDeviceToDevice, s));
+        p->ok = true;
+      }
+      SVM_HIP(hipStreamSynchronize(s));
+    } else {
+      p->pkey = DevBuf(256);
+      k::fill_u32(p->pkey.p, 64, 0u, s);  // P = 0
+      p->praw = DevBuf(4);
+      p->ok = true;
+    }
+  }
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_profiles[key] = p;
+  return p;
+}
+
+// ------------------------------------------------------------------ lowering
+struct JoinL {
+  TablePtr dim;
+  int kcol = 0, fcol = 0;
+  std::vector<Atom> atoms;  // on dimension columns
+  int key = -1;             // key component read through this join
+  int attr = -1;            // its dimension column
+  std::vector<int> attrs;   // atom columns and the key column, ascending
+};
+struct KeyL {
+  int join = 0;
+  std::shared_ptr<DictEntry> dict;
+  uint32_t stride = 1;
+};
 struct FactLowered {
   bool ok = false;
   TablePtr src, base;
-  bk::Query q;
-  std::vector<bk::MapBuild> builds;     // maps filled in later (arena offsets)
-  std::vector<uint64_t> build_words;
-  std::vector<std::shared_ptr<DictEntry>> dicts;  // key components in key order
-  std::vector<uint32_t> strides;
+  long long row0 = 0, row1 = 0;
+  std::vector<std::array<int, 3>> preds;  // fact column, lo, hi
+  int mop = 0, ma = 0, mb = 0;
   int K = 0;
+  uint64_t G = 1;
+  std::vector<KeyL> keys;
+  std::vector<JoinL> joins;
   std::vector<std::string> deferred;    // dimension plan ids this query absorbs
 };
 
@@ -283,10 +426,8 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
     refs.push_back(d);
   }
   auto src_int = [&](int col) { return col >= 0 && col < (int)src->cols.size() && src->cols[col].type == Int32; };
-  bk::Query& q = f.q;
-  memset(&q, 0, sizeof(q));
-  q.row0 = (long long)src->row0;
-  q.row1 = (long long)(src->row0 + src->rows);
+  f.row0 = (long long)src->row0;
+  f.row1 = (long long)(src->row0 + src->rows);
   // fact predicates: intersect ranges per column
   std::vector<Atom> atoms;
   if (!parse_conj(p.filter_exp, atoms)) return f;
@@ -299,14 +440,13 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
   }
   if (ranges.size() > (size_t)bk::kMaxPred) return f;
   for (auto& kv : ranges) {
-    q.pcol[q.npred] = kv.first;
-    q.plo[q.npred] = (int)std::max<long long>(kv.second.first, INT_MIN);
-    q.phi[q.npred] = (int)std::min<long long>(kv.second.second, INT_MAX);
-    if (kv.second.first > kv.second.second) { q.plo[q.npred] = 1; q.phi[q.npred] = 0; }
-    q.npred++;
+    if (kv.second.first > kv.second.second) f.preds.push_back({kv.first, 1, 0});
+    else
+      f.preds.push_back({kv.first, (int)std::max<long long>(kv.second.first, INT_MIN),
+                         (int)std::min<long long>(kv.second.second, INT_MAX)});
   }
   // metric
-  if (!parse_metric(p.metric_exp[0], q.mop, q.ma, q.mb) || !src_int(q.ma) || !src_int(q.mb)) return f;
+  if (!parse_metric(p.metric_exp[0], f.mop, f.ma, f.mb) || !src_int(f.ma) || !src_int(f.mb)) return f;
   // keys: one component per dimension, read through that dimension's join
   std::vector<int> key_obj(K), key_base(K);
   for (size_t k = 0; k < K; ++k) {
@@ -322,67 +462,55 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
       if (key_obj[k2] == ki.obj) return f;
   }
   uint64_t G = 1;
-  f.dicts.resize(K);
-  f.strides.resize(K);
+  f.keys.resize(K);
   for (int k = (int)K - 1; k >= 0; --k) {
-    f.dicts[k] = table_dict(refs[key_obj[k] - 1].base, key_base[k], s);
-    f.strides[k] = (uint32_t)G;
-    G *= f.dicts[k]->n;
+    f.keys[k].dict = table_dict(refs[key_obj[k] - 1].base, key_base[k], s);
+    f.keys[k].stride = (uint32_t)G;
+    G *= f.keys[k].dict->n;
     if (G > (1ull << 22)) return f;
   }
   f.K = (int)K;
-  q.G = (unsigned)(K ? G : 1);
-  q.agg = K == 0 ? bk::BAGG_SCALAR : G <= (uint64_t)bk::kDenseG ? bk::BAGG_DENSE : bk::BAGG_HASH;
+  f.G = K ? G : 1;
   // joins
   std::set<int> seen;
+  std::set<std::tuple<const Table*, int, int>> slots;
+  uint64_t profiles = 0;
   for (size_t j = 0; j < p.join_rkey.size(); ++j) {
     KeyIndex L = p.join_lkey[j], R = p.join_rkey[j];
     if (L.obj != 0 || !src_int(L.col) || R.obj < 1 || R.obj > (int)refs.size()) return f;
     if (!seen.insert(R.obj).second) return f;
     const DimLowered& d = refs[R.obj - 1];
     if (R.col < 0 || R.col >= (int)d.proj.size()) return f;
-    int rcol = d.proj[R.col];
-    if (d.base->cols[rcol].type != Int32) return f;
+    JoinL jl;
+    jl.dim = d.base;
+    jl.kcol = d.proj[R.col];
+    jl.fcol = L.col;
+    if (d.base->cols[jl.kcol].type != Int32) return f;
     if (d.base->max_tuple_count && d.base->rows > d.base->max_tuple_count) return f;
-    KeyEntry ke = table_key(d.base, rcol, s);
-    if (!ke.unique) return f;
-    bk::MapBuild b;
-    memset(&b, 0, sizeof(b));
-    b.keys = (const int32_t*)d.base->col_ptr(rcol);
-    b.n = d.base->rows;
-    b.kmin = ke.kmin;
-    uint64_t range = d.base->rows ? (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1) : 0;
-    if (range > (1ull << 28)) return f;
-    b.range = (unsigned)range;
-    for (auto& a : d.atoms) {
-      bk::DimPred& dp = b.pred[b.npred++];
-      dp.col = d.base->col_ptr(a.col);
-      dp.kind = a.kind;
-      dp.type = d.base->cols[a.col].type;
-      dp.size = (int)d.base->cols[a.col].size;
-      dp.lo = (int)std::max<long long>(a.lo, INT_MIN);
-      dp.hi = (int)std::min<long long>(a.hi, INT_MAX);
-      if (a.lo > a.hi) { dp.lo = 1; dp.hi = 0; }
-      dp.n = a.n;
-      memset(dp.lit, 0, sizeof(dp.lit));
-      memcpy(dp.lit, a.lit.data(), a.lit.size());
+    if (!slots.insert(std::make_tuple(d.base.get(), jl.kcol, jl.fcol)).second) return f;
+    jl.atoms = d.atoms;
+    std::set<int> attrs;
+    for (auto& a : jl.atoms) {
+      attrs.insert(a.col);
+      if (a.kind != bk::DP_RANGE) {
+        auto dict = table_dict(d.base, a.col, s);
+        if (!string_rep(d.base, a.col, dict, s)->exact) return f;
+      }
     }
-    int comp = -1;
     for (size_t k = 0; k < K; ++k)
-      if (key_obj[k] == R.obj) comp = (int)k;
-    b.codes = comp >= 0 ? f.dicts[comp]->codes.as<int32_t>() : nullptr;
-    b.wbits = comp >= 0 ? width_for(f.dicts[comp]->n) : 1;
-    q.jcol[q.njoin] = L.col;
-    q.jkmin[q.njoin] = b.kmin;
-    q.jrange[q.njoin] = b.range;
-    q.jw[q.njoin] = b.wbits;
-    q.jcode[q.njoin] = comp >= 0 ? 1 : 0;
-    q.jstride[q.njoin] = comp >= 0 ? f.strides[comp] : 0;
-    q.njoin++;
-    uint64_t per = 32 / (uint64_t)b.wbits;
-    f.build_words.push_back(((uint64_t)b.range + per - 1) / per);
-    f.builds.push_back(b);
+      if (key_obj[k] == R.obj) {
+        jl.key = (int)k;
+        jl.attr = key_base[k];
+        attrs.insert(jl.attr);
+        f.keys[k].join = (int)j;
+      }
+    jl.attrs.assign(attrs.begin(), attrs.end());
+    auto prof = profile_of(d.base, jl.kcol, jl.attrs, s);
+    if (!prof->ok) return f;
+    profiles += prof->P + 1;  // and the no-row profile
+    f.joins.push_back(std::move(jl));
   }
+  if (profiles > (uint64_t)bk::kMaxProfiles) return f;
   // every key component's dimension must be joined
   for (size_t k = 0; k < K; ++k)
     if (!seen.count(key_obj[k])) return f;
@@ -394,12 +522,30 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
 
 }  // namespace
 
-// Device buffers reused across batches (grown on demand, never shrunk), so a
-// steady stream of batches allocates nothing.
+// Fixed layout of one pass's device descriptors: every pointer a launch of the
+// pass receives stays the same from batch to batch, so a captured launch
+// sequence replays for any batch once its descriptors are uploaded in place.
+constexpr size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+struct PassLayout {
+  static constexpr size_t o_args = 0;
+  static constexpr size_t o_margs = al(sizeof(bk::ScanArgs));
+  static constexpr size_t o_qd = o_margs + al(sizeof(bk::MaskArgs));
+  static constexpr size_t o_row = o_qd + al(bk::kMaxBatch * sizeof(bk::QDesc));
+  static constexpr size_t o_terms = o_row + al(bk::kMaxBatch * 16);
+  static constexpr size_t o_emit = o_terms + al(bk::kMaxTerms * sizeof(bk::MaskTerm));
+  static constexpr size_t total = o_emit + al(bk::kMaxBatch * sizeof(bk::Emit));
+};
+// Emission blocks of one pass: 256 flag words per block, G <= 2^22 per query.
+constexpr uint64_t kMaxEmitBlocks = (uint64_t)bk::kMaxBatch * (((1u << 22) / 32 + 255) / 256);
+
+// Device buffers and captured launch sequences reused across batches (buffers are
+// grown on demand and never shrunk), so a steady stream of batches allocates
+// nothing and launches one graph per pass.
 struct Scratch {
   std::vector<DevBuf> bufs;
   void* host = nullptr;  // pinned staging for descriptors
   size_t host_bytes = 0;
+  std::map<std::tuple<int, unsigned, int>, hipGraphExec_t> graphs;  // (variant, LDS profiles, grid)
   void* get(size_t slot, uint64_t bytes) {
     if (bufs.size() <= slot) bufs.resize(slot + 1);
     if (bufs[slot].bytes < bytes) bufs[slot] = DevBuf(bytes + bytes / 4 + 256);
@@ -416,132 +562,420 @@ struct Scratch {
     return (char*)host;
   }
   ~Scratch() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     if (host) (void)hipHostFree(host);
   }
 };
 
-// HBM a fused query holds while it runs: its maps and group tables.
+// HBM a fused query holds while it runs: its group table and its share of the
+// batch's profile masks (profiles themselves are cached with the dimension tables).
 static uint64_t fused_bytes(const FactLowered& f) {
-  uint64_t b = 4096 + (uint64_t)f.q.G * 8 + ((uint64_t)f.q.G + 31) / 32 * 4;
-  for (uint64_t w : f.build_words) b += w * 4 + 16;
+  return 4096 + f.G * 8 + (f.G + 31) / 32 * 4 + (uint64_t)bk::kMaxProfiles * 8 / bk::kMaxBatch;
+}
+
+namespace {
+
+// One shared scan: at most kMaxBatch queries whose joins fit kMaxSlot slots.
+struct SlotB {
+  TablePtr dim;
+  int kcol = 0, fcol = 0;
+  std::set<int> attrs;
+  std::shared_ptr<Profile> prof;
+};
+struct PassB {
+  std::vector<int> qi;
+  std::vector<SlotB> slots;
+  std::vector<std::array<int, 3>> metrics;
+  int npred = 0;
+};
+
+int slot_bits(const SlotB& sl, hipStream_t s) {
+  int b = 0;
+  for (int a : sl.attrs) b += bits_for(table_dict(sl.dim, a, s)->n);
   return b;
 }
 
-// Fused execution of lowered queries that share one base fact table.
-static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr<svm_result>*>& outs,
-                      Engine& E, hipStream_t s, Scratch& sc) {
-  if (qs.empty()) return;
-  TablePtr base = qs[0]->base;
-  const int nq = (int)qs.size();
-  const long long tile = 65536;
-  // arenas
-  uint64_t map_words = 0, sum_words = 0, flag_words = 0, nbuild = 0, max_rows = 0;
-  for (auto* f : qs) {
-    for (size_t j = 0; j < f->builds.size(); ++j) {
-      map_words += (f->build_words[j] + 3) & ~3ull;
-      max_rows = std::max<uint64_t>(max_rows, f->builds[j].n);
+// Adds query i to the pass if every limit of one shared scan still holds.
+bool pass_add(PassB& P, const FactLowered& f, int i, hipStream_t s) {
+  if ((int)P.qi.size() >= bk::kMaxBatch) return false;
+  if (P.npred + (int)f.preds.size() > bk::kMaxFactPred) return false;
+  PassB T = P;
+  for (auto& j : f.joins) {
+    SlotB* sl = nullptr;
+    for (auto& x : T.slots)
+      if (x.dim == j.dim && x.kcol == j.kcol && x.fcol == j.fcol) sl = &x;
+    if (!sl) {
+      if ((int)T.slots.size() >= bk::kMaxSlot) return false;
+      T.slots.push_back(SlotB{j.dim, j.kcol, j.fcol, {}, nullptr});
+      sl = &T.slots.back();
     }
-    nbuild += f->builds.size();
-    sum_words += f->q.G;
-    flag_words += (f->q.G + 31) / 32;
+    sl->attrs.insert(j.attrs.begin(), j.attrs.end());
+    if ((int)sl->attrs.size() > bk::kMaxAttr || slot_bits(*sl, s) > 31) return false;
   }
-  hipEvent_t e0, e1;
-  SVM_HIP(hipEventCreate(&e0));
-  SVM_HIP(hipEventCreate(&e1));
-  SVM_HIP(hipEventRecord(e0, s));
-  unsigned* maps = (unsigned*)sc.get(0, map_words * 4 + 16);
+  std::array<int, 3> m = {f.mop, f.ma, f.mop == bk::MOP_COL ? f.ma : f.mb};
+  if (std::find(T.metrics.begin(), T.metrics.end(), m) == T.metrics.end()) {
+    if ((int)T.metrics.size() >= bk::kMaxMetric) return false;
+    T.metrics.push_back(m);
+  }
+  T.npred += (int)f.preds.size();
+  T.qi.push_back(i);
+  P = std::move(T);
+  return true;
+}
+
+// Rebuilds a pass from a subset of its queries.
+PassB pass_of(const std::vector<FactLowered*>& qs, const std::vector<int>& idx, hipStream_t s) {
+  PassB P;
+  for (int i : idx)
+    if (!pass_add(P, *qs[i], i, s)) fail(SVM_ERR_INVALID, "shared scan limits violated by a subset");
+  return P;
+}
+
+}  // namespace
+
+namespace {
+
+// Autotuner of the shared scan: per batch shape (slots, queries, rows scanned) the
+// fastest (variant, tile rows) among the candidates, timed on the batch itself the
+// first time the shape is seen.  A trial runs the whole scan; the pass is cleared
+// again before the run whose results are kept.
+struct BatchTune {
+  int variant = 0;
+  long long tile = 16384;
+  int bits = 0;
+};
+std::mutex g_btune_mu;
+std::map<std::tuple<int, int, int, bool>, BatchTune> g_btune;
+constexpr long long kTileChoices[] = {16384, 65536};
+constexpr uint64_t kBatchTuneMinRows = 1ull << 22;
+
+int ilog2(uint64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    r++;
+  }
+  return r;
+}
+
+unsigned prof_cap(unsigned nprof) { return nprof <= 1024 ? 1024u : nprof <= 2048 ? 2048u : 4096u; }
+
+// Persistent grid: as many workgroups per CU as LDS and 20 waves per CU (five per
+// SIMD at the scan's register count) allow.
+int scan_grid(const Engine& E, int variant, unsigned cap) {
+  const size_t lds = bk::shared_scan_lds(variant, cap) + 64;
+  const int by_lds = (int)((160 * 1024) / lds);
+  const int by_waves = 20 / (bk::scan_variant_block(variant) / 64);
+  return E.num_cus * std::max(1, std::min(by_lds, std::min(by_waves, 8)));
+}
+
+bool env_off(const char* name) {
+  const char* v = getenv(name);
+  return v && v[0] == '0';
+}
+
+}  // namespace
+
+static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
+                     std::vector<std::unique_ptr<svm_result>*>& outs, Engine& E, hipStream_t s, Scratch& sc,
+                     bool tune) {
+  bk::prepare_shared_scan();
+  TablePtr base = qs[P.qi[0]]->base;
+  const int nq = (int)P.qi.size();
+  // slots, with the profile of their attribute union
+  unsigned nprof = 0;
+  for (auto& sl : P.slots) {
+    std::vector<int> attrs(sl.attrs.begin(), sl.attrs.end());
+    sl.prof = profile_of(sl.dim, sl.kcol, attrs, s);
+    if (!sl.prof->ok) fail(SVM_ERR_UNSUPPORTED, "dimension profile unavailable");
+    nprof += sl.prof->P + 1;
+  }
+  if (nprof > (unsigned)bk::kMaxProfiles) fail(SVM_ERR_INVALID, "pass exceeds the LDS profile budget");
+  auto slot_index = [&](const JoinL& j) {
+    for (size_t k = 0; k < P.slots.size(); ++k)
+      if (P.slots[k].dim == j.dim && P.slots[k].kcol == j.kcol && P.slots[k].fcol == j.fcol) return (int)k;
+    fail(SVM_ERR_INVALID, "join without a slot");
+    return -1;
+  };
+  auto attr_pos = [&](const Profile& pr, int col) {
+    for (size_t k = 0; k < pr.attrs.size(); ++k)
+      if (pr.attrs[k] == col) return (int)k;
+    fail(SVM_ERR_INVALID, "attribute missing from a profile");
+    return -1;
+  };
+  // buffers: group tables grow with the batch, everything else has a fixed size
+  uint64_t sum_words = 0, flag_words = 0;
+  for (int i : P.qi) {
+    sum_words += qs[i]->G;
+    flag_words += (qs[i]->G + 31) / 32;
+  }
+  unsigned long long* masks = (unsigned long long*)sc.get(0, (uint64_t)bk::kMaxProfiles * 8);
   unsigned long long* sums = (unsigned long long*)sc.get(1, sum_words * 8 + 8);
   unsigned* flags = (unsigned*)sc.get(2, flag_words * 4 + 4);
   unsigned* counter = (unsigned*)sc.get(3, 16);
-  k::fill_u32(maps, map_words + 4, 0, s);
-  k::fill_u32(sums, sum_words * 2 + 2, 0, s);
-  k::fill_u32(flags, flag_words + 1, 0, s);
-  k::fill_u32(counter, 4, 0, s);
-  std::vector<bk::MapBuild> builds;
-  std::vector<bk::Query> qd;
-  std::vector<bk::Emit> em;
-  uint64_t mo = 0, so = 0, fo = 0, blocks = 0;
-  for (auto* f : qs) {
-    bk::Query q = f->q;
-    for (size_t j = 0; j < f->builds.size(); ++j) {
-      bk::MapBuild b = f->builds[j];
-      b.map = maps + mo;
-      q.jmap[j] = b.map;
-      mo += (f->build_words[j] + 3) & ~3ull;
-      builds.push_back(b);
+  char* dd = (char*)sc.get(4, PassLayout::total);
+  unsigned long long* counts = (unsigned long long*)sc.get(5, kMaxEmitBlocks * 8);
+  unsigned long long* offsets = (unsigned long long*)sc.get(6, kMaxEmitBlocks * 8);
+  unsigned long long* totals = (unsigned long long*)sc.get(7, bk::kMaxBatch * 8);
+  uint64_t bit_words = 0;
+  for (auto& sl : P.slots) bit_words += ((uint64_t)sl.prof->range + 63) / 64 * 2;
+  unsigned* kbits = (unsigned*)sc.get(10, bit_words * 4 + 8);
+  const auto* d_args = (const bk::ScanArgs*)(dd + PassLayout::o_args);
+  const auto* d_margs = (const bk::MaskArgs*)(dd + PassLayout::o_margs);
+  const auto* d_emit = (const bk::Emit*)(dd + PassLayout::o_emit);
+  // descriptors, written in place in the pinned staging area
+  char* host = sc.pinned(PassLayout::total);
+  auto* A = (bk::ScanArgs*)(host + PassLayout::o_args);
+  auto* M = (bk::MaskArgs*)(host + PassLayout::o_margs);
+  auto* qd = (bk::QDesc*)(host + PassLayout::o_qd);
+  auto* qrow = (long long*)(host + PassLayout::o_row);
+  auto* terms = (bk::MaskTerm*)(host + PassLayout::o_terms);
+  auto* em = (bk::Emit*)(host + PassLayout::o_emit);
+  memset(host, 0, PassLayout::total);
+  A->nq = nq;
+  A->nslot = (int)P.slots.size();
+  A->nmetric = (int)P.metrics.size();
+  for (size_t c = 0; c < base->cols.size() && c < (size_t)bk::kMaxCols; ++c) A->cols[c] = base->cols[c].dev;
+  unsigned lds = 0;
+  for (size_t k = 0; k < P.slots.size(); ++k) {
+    const SlotB& sl = P.slots[k];
+    bk::Slot& S = A->slot[k];
+    S.fcol = sl.fcol;
+    S.kmin = sl.prof->kmin;
+    S.range = sl.prof->range;
+    S.P = sl.prof->P;
+    S.pkey = sl.prof->pkey.p;
+    S.narrow = sl.prof->narrow ? 1 : 0;
+    S.praw = sl.prof->praw.as<int32_t>();
+    S.mask = masks + lds;
+    S.bits = kbits;
+    kbits += ((uint64_t)S.range + 63) / 64 * 2;
+    S.lds = (int)lds;
+    lds += sl.prof->P + 1;
+  }
+  for (size_t m = 0; m < P.metrics.size(); ++m) {
+    A->mop[m] = P.metrics[m][0];
+    A->ma[m] = P.metrics[m][1];
+    A->mb[m] = P.metrics[m][2];
+  }
+  std::vector<std::vector<bk::FactPred>> by_col(bk::kMaxCols);
+  std::vector<std::vector<bk::MaskTerm>> slot_terms(P.slots.size());
+  long long row_lo = LLONG_MAX, row_hi = 0;
+  uint64_t so = 0, fo = 0, blocks = 0;
+  for (int q = 0; q < nq; ++q) {
+    const FactLowered& f = *qs[P.qi[q]];
+    qrow[2 * q] = f.row0;
+    qrow[2 * q + 1] = f.row1;
+    if (f.row1 > f.row0) {
+      row_lo = std::min(row_lo, f.row0);
+      row_hi = std::max(row_hi, f.row1);
     }
-    q.sums = sums + so;
-    q.flags = flags + fo;
-    bk::Emit e;
-    memset(&e, 0, sizeof(e));
-    e.flags = q.flags;
-    e.sums = q.sums;
-    e.G = q.G;
-    e.K = f->K;
-    for (int k = 0; k < f->K; ++k) {
-      e.n[k] = f->dicts[k]->n;
-      e.stride[k] = f->strides[k];
-      e.dict[k] = f->dicts[k]->dict.as<int32_t>();
+    unsigned qcols = 1u << f.ma | 1u << f.mb;
+    for (auto& pr : f.preds) {
+      by_col[pr[0]].push_back(bk::FactPred{q, pr[1], pr[2]});
+      qcols |= 1u << pr[0];
+    }
+    for (auto& j : f.joins) qcols |= 1u << j.fcol;
+    A->qcols[q] = qcols;
+    bk::QDesc& d = qd[q];
+    d.K = f.K;
+    std::array<int, 3> m = {f.mop, f.ma, f.mop == bk::MOP_COL ? f.ma : f.mb};
+    d.metric = (int)(std::find(P.metrics.begin(), P.metrics.end(), m) - P.metrics.begin());
+    d.sum_off = (unsigned)so;
+    d.flag_off = (unsigned)fo;
+    for (auto& j : f.joins) {
+      const int sk = slot_index(j);
+      A->slot[sk].joined |= 1ull << q;
+      const Profile& pr = *P.slots[sk].prof;
+      bk::MaskTerm t;
+      memset(&t, 0, sizeof(t));
+      t.q = q;
+      for (auto& a : j.atoms) {
+        bk::MaskAtom& ma = t.atom[t.natom++];
+        const int pos = attr_pos(pr, a.col);
+        ma.kind = a.kind;
+        ma.shift = pr.shift[pos];
+        ma.bits = pr.bits[pos];
+        ma.lo = (int)std::max<long long>(a.lo, INT_MIN);
+        ma.hi = (int)std::min<long long>(a.hi, INT_MAX);
+        if (a.lo > a.hi) { ma.lo = 1; ma.hi = 0; }
+        ma.n = a.n;
+        ma.dict = pr.dicts[pos]->dict.as<int32_t>();
+        if (a.kind != bk::DP_RANGE) {
+          ma.col = (const char*)j.dim->col_ptr(a.col);
+          ma.size = (int)j.dim->cols[a.col].size;
+          ma.rep = string_rep(j.dim, a.col, pr.dicts[pos], s)->rep.as<int32_t>();
+        }
+        memcpy(ma.lit, a.lit.data(), std::min<size_t>(a.lit.size(), sizeof(ma.lit) - 1));
+      }
+      slot_terms[sk].push_back(t);
+      if (j.key >= 0) {
+        const int pos = attr_pos(pr, j.attr);
+        d.key[j.key] = (unsigned)A->slot[sk].lds | (unsigned)sk << 13 | (unsigned)pr.shift[pos] << 16 |
+                       (unsigned)pr.bits[pos] << 21;
+        d.stride[j.key] = f.keys[j.key].stride;
+      }
+    }
+    bk::Emit& e = em[q];
+    e.flags = flags + fo;
+    e.sums = sums + so;
+    e.G = (unsigned)f.G;
+    e.K = f.K;
+    for (int k = 0; k < f.K; ++k) {
+      e.n[k] = f.keys[k].dict->n;
+      e.stride[k] = f.keys[k].stride;
+      e.dict[k] = f.keys[k].dict->dict.as<int32_t>();
     }
     e.block0 = blocks;
-    e.nblocks = ((q.G + 31) / 32 + 255) / 256;
+    e.nblocks = ((f.G + 31) / 32 + 255) / 256;
     blocks += e.nblocks;
-    so += q.G;
-    fo += (q.G + 31) / 32;
-    qd.push_back(q);
-    em.push_back(e);
+    so += f.G;
+    fo += (f.G + 31) / 32;
   }
-  // work items: tile-major so concurrent workgroups share a tile through the caches
-  std::vector<bk::Item> items;
-  long long tmax = 0;
-  for (auto& q : qd) tmax = std::max(tmax, (q.row1 + tile - 1) / tile);
-  for (long long t = 0; t < tmax; ++t)
-    for (int i = 0; i < nq; ++i)
-      if (qd[i].row0 < (t + 1) * tile && qd[i].row1 > t * tile && qd[i].row1 > qd[i].row0)
-        items.push_back({(int)t, i});
-  // descriptors in one upload
-  size_t ob = 0, oq = ob + builds.size() * sizeof(bk::MapBuild);
-  oq = (oq + 255) & ~255ull;
-  size_t oi = oq + qd.size() * sizeof(bk::Query);
-  oi = (oi + 255) & ~255ull;
-  size_t oe = oi + items.size() * sizeof(bk::Item);
-  oe = (oe + 255) & ~255ull;
-  size_t total = oe + em.size() * sizeof(bk::Emit);
-  char* host = sc.pinned(total + 256);
-  memcpy(host + ob, builds.data(), builds.size() * sizeof(bk::MapBuild));
-  memcpy(host + oq, qd.data(), qd.size() * sizeof(bk::Query));
-  memcpy(host + oi, items.data(), items.size() * sizeof(bk::Item));
-  memcpy(host + oe, em.data(), em.size() * sizeof(bk::Emit));
-  char* dd = (char*)sc.get(4, total + 256);
-  SVM_HIP(hipMemcpyAsync(dd, host, total, hipMemcpyHostToDevice, s));
-  hipEvent_t eb, es;
+  int np = 0;
+  for (int c = 0; c < bk::kMaxCols; ++c) {
+    A->pred0[c] = np;
+    for (auto& fp : by_col[c]) A->pred[np++] = fp;
+  }
+  A->pred0[bk::kMaxCols] = np;
+  int nt = 0;
+  for (size_t k = 0; k < P.slots.size(); ++k) {
+    bk::MaskSlot& ms = M->slots[k];
+    ms.P = A->slot[k].P;
+    ms.praw = A->slot[k].praw;
+    ms.mask = masks + A->slot[k].lds;
+    ms.t0 = nt;
+    for (auto& t : slot_terms[k]) terms[nt++] = t;
+    ms.t1 = nt;
+  }
+  M->terms = (const bk::MaskTerm*)(dd + PassLayout::o_terms);
+  M->nslot = (int)P.slots.size();
+  A->nprof = nprof;
+  A->qd = (const bk::QDesc*)(dd + PassLayout::o_qd);
+  A->qrow = (const long long*)(dd + PassLayout::o_row);
+  A->sums = sums;
+  A->flags = flags;
+  A->sum_words = sum_words;
+  A->flag_words = flag_words;
+  A->total_blocks = blocks;
+  A->counter = counter;
+  if (row_lo == LLONG_MAX) row_lo = row_hi = 0;
+  A->row_lo = row_lo & ~3ll;
+  A->row_hi = row_hi;
+  const unsigned cap = prof_cap(nprof);
+  auto set_tile = [&](long long tile) {
+    A->tile_rows = tile;
+    A->ntiles = (unsigned)((row_hi - A->row_lo + tile - 1) / tile);
+  };
+  // autotuner: the launch shape of this batch shape
+  const uint64_t union_rows = (uint64_t)(row_hi - A->row_lo);
+  const bool mask32 = nq <= 32;  // query masks of 32 bits suffice
+  const auto tkey = std::make_tuple(A->nslot, ilog2((uint64_t)nq), ilog2(union_rows + 1), mask32);
+  BatchTune ch;
+  ch.variant = mask32 ? 1 : 0;  // 512 threads, gated lookups
+  bool tuned = false;
+  {
+    std::lock_guard<std::mutex> g(g_btune_mu);
+    auto it = g_btune.find(tkey);
+    if (it != g_btune.end()) {
+      ch = it->second;
+      tuned = true;
+    }
+  }
+  if (!tuned) ch.tile = union_rows > (1ull << 26) ? 65536 : 16384;
+  if (!tuned && tune && union_rows >= kBatchTuneMinRows && !env_off("SVM_AUTOTUNE")) {
+    hipEvent_t t0, t1;
+    SVM_HIP(hipEventCreate(&t0));
+    SVM_HIP(hipEventCreate(&t1));
+    auto trial = [&](const BatchTune& c) {
+      set_tile(c.tile);
+      A->use_bits = c.bits;
+      SVM_HIP(hipMemcpyAsync(dd, host, PassLayout::total, hipMemcpyHostToDevice, s));
+      bk::launch_pass_clear(d_args, s);
+      bk::launch_profile_masks(d_margs, s);
+      bk::launch_slot_bits(d_args, s);
+      SVM_HIP(hipEventRecord(t0, s));
+      bk::launch_shared_scan(d_args, c.variant, cap, scan_grid(E, c.variant, cap), s);
+      SVM_HIP(hipEventRecord(t1, s));
+      SVM_HIP(hipStreamSynchronize(s));
+      float ms = 0;
+      SVM_HIP(hipEventElapsedTime(&ms, t0, t1));
+      if (E.cfg.verbose)
+        fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots, %s lookups%s, key bits %d, tile %lld rows: %.3f ms\n",
+                bk::scan_variant_block(c.variant), bk::scan_variant_hash_slots(c.variant),
+                bk::scan_variant_parallel(c.variant) ? "parallel" : "gated",
+                bk::scan_variant_prefetch(c.variant) ? ", prefetch" : "", c.bits, c.tile, ms);
+      return ms;
+    };
+    float best = 1e30f;
+    ch.variant = -1;
+    for (int v = 0; v < bk::scan_variants(); ++v)
+      for (long long tile : kTileChoices) {
+        if (bk::scan_variant_mask32(v) != mask32) continue;
+        BatchTune c{v, tile, 0};
+        const float ms = trial(c);
+        if (ms < best) {
+          best = ms;
+          ch = c;
+        }
+      }
+    BatchTune c = ch;
+    c.bits = 1;
+    if (trial(c) < best) ch = c;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    std::lock_guard<std::mutex> g(g_btune_mu);
+    g_btune[tkey] = ch;
+  }
+  A->use_bits = ch.bits;
+  set_tile(ch.tile);
+  const int grid = scan_grid(E, ch.variant, cap);
+  // the pass: upload descriptors, then clear -> masks -> scan -> emission count
+  hipEvent_t e0, e1, eb, es;
+  SVM_HIP(hipEventCreate(&e0));
+  SVM_HIP(hipEventCreate(&e1));
   SVM_HIP(hipEventCreate(&eb));
   SVM_HIP(hipEventCreate(&es));
-  bk::launch_map_build((const bk::MapBuild*)(dd + ob), (int)builds.size(), max_rows, s);
-  SVM_HIP(hipEventRecord(eb, s));
-  const void* cols[bk::kMaxCols] = {nullptr};
-  for (size_t c = 0; c < base->cols.size() && c < (size_t)bk::kMaxCols; ++c) cols[c] = base->cols[c].dev;
-  if (!items.empty())
-    bk::launch_shared_scan((const bk::Query*)(dd + oq), (const bk::Item*)(dd + oi), (unsigned)items.size(),
-                           counter, cols, tile, E.num_cus * 4, s);
-  SVM_HIP(hipEventRecord(es, s));
-  unsigned long long* counts = (unsigned long long*)sc.get(5, blocks * 8 + 8);
-  unsigned long long* offsets = (unsigned long long*)sc.get(6, blocks * 8 + 8);
-  unsigned long long* totals = (unsigned long long*)sc.get(7, nq * 8 + 8);
-  bk::launch_emit_count((const bk::Emit*)(dd + oe), nq, blocks, counts, offsets, totals, s);
+  SVM_HIP(hipMemcpyAsync(dd, host, PassLayout::total, hipMemcpyHostToDevice, s));
+  SVM_HIP(hipEventRecord(e0, s));
+  const bool graph = !env_off("SVM_BATCH_GRAPH");
+  if (graph) {
+    hipGraphExec_t& ge = sc.graphs[std::make_tuple(ch.variant, cap, grid)];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      SVM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      bk::launch_pass_clear(d_args, s);
+      bk::launch_profile_masks(d_margs, s);
+      bk::launch_slot_bits(d_args, s);
+      bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
+      bk::launch_emit_count(d_emit, d_args, counts, offsets, totals, s);
+      SVM_HIP(hipStreamEndCapture(s, &g));
+      SVM_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      SVM_HIP(hipGraphDestroy(g));
+    }
+    SVM_HIP(hipGraphLaunch(ge, s));
+  } else {
+    bk::launch_pass_clear(d_args, s);
+    bk::launch_profile_masks(d_margs, s);
+    bk::launch_slot_bits(d_args, s);
+    SVM_HIP(hipEventRecord(eb, s));
+    bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
+    SVM_HIP(hipEventRecord(es, s));
+    bk::launch_emit_count(d_emit, d_args, counts, offsets, totals, s);
+  }
   std::vector<unsigned long long> tot(nq);
   SVM_HIP(hipMemcpyAsync(tot.data(), totals, nq * 8, hipMemcpyDeviceToHost, s));
   SVM_HIP(hipStreamSynchronize(s));
+  // emission of rows (sized by the counts just read)
   uint64_t rows = 0;
   for (int i = 0; i < nq; ++i) {
     em[i].out0 = rows;
     rows += tot[i];
   }
-  memcpy(host + oe, em.data(), em.size() * sizeof(bk::Emit));
-  SVM_HIP(hipMemcpyAsync(dd + oe, host + oe, em.size() * sizeof(bk::Emit), hipMemcpyHostToDevice, s));
+  SVM_HIP(hipMemcpyAsync(dd + PassLayout::o_emit, em, nq * sizeof(bk::Emit), hipMemcpyHostToDevice, s));
   int32_t* keys = (int32_t*)sc.get(8, rows * 4 * bk::kMaxKey + 16);
   int64_t* vals = (int64_t*)sc.get(9, rows * 8 + 8);
-  bk::launch_emit_rows((const bk::Emit*)(dd + oe), nq, blocks, offsets, keys, vals, rows, s);
+  bk::launch_emit_rows(d_emit, nq, blocks, offsets, keys, vals, rows, s);
   std::vector<int32_t> hk(rows * bk::kMaxKey + 1);
   std::vector<int64_t> hv(rows + 1);
   if (rows) {
@@ -550,40 +984,80 @@ static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr
   }
   SVM_HIP(hipEventRecord(e1, s));
   SVM_HIP(hipStreamSynchronize(s));
-  float ms = 0, ms_build = 0, ms_scan = 0;
-  SVM_HIP(hipEventElapsedTime(&ms, e0, e1));
-  SVM_HIP(hipEventElapsedTime(&ms_build, e0, eb));
-  SVM_HIP(hipEventElapsedTime(&ms_scan, eb, es));
+  float ms_all = 0;
+  SVM_HIP(hipEventElapsedTime(&ms_all, e0, e1));
   if (E.cfg.verbose) {
-    uint64_t slice_rows = 0, dim_rows = 0;
-    for (auto& q : qd) slice_rows += (uint64_t)(q.row1 - q.row0);
-    for (auto& b : builds) dim_rows += b.n;
-    fprintf(stderr, "[svm] batch of %d: gpu %.3f ms (map builds %.3f over %llu dim rows, shared scan %.3f over "
-                    "%llu slice rows / %zu items, emit %.3f)\n",
-            nq, ms, ms_build, (unsigned long long)dim_rows, ms_scan, (unsigned long long)slice_rows,
-            items.size(), ms - ms_build - ms_scan);
+    uint64_t slice_rows = 0;
+    for (int q = 0; q < nq; ++q) slice_rows += (uint64_t)(qrow[2 * q + 1] - qrow[2 * q]);
+    char phases[128] = "";
+    if (!graph) {
+      float mb = 0, msc = 0;
+      SVM_HIP(hipEventElapsedTime(&mb, e0, eb));
+      SVM_HIP(hipEventElapsedTime(&msc, eb, es));
+      snprintf(phases, sizeof(phases), ", masks %.3f, scan %.3f", mb, msc);
+    }
+    fprintf(stderr, "[svm] shared scan of %d queries: gpu %.3f ms%s (%s; %u profiles in %zu slots; %llu rows / %llu "
+                    "slice rows; %d threads, %d LDS slots, %s lookups, key bits %d, tile %lld, %u tiles, grid %d)\n",
+            nq, ms_all, phases, graph ? "graph" : "eager", nprof, P.slots.size(), (unsigned long long)union_rows,
+            (unsigned long long)slice_rows, bk::scan_variant_block(ch.variant), bk::scan_variant_hash_slots(ch.variant),
+            bk::scan_variant_parallel(ch.variant) ? "parallel" : "gated", ch.bits, (long long)ch.tile, A->ntiles, grid);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(eb);
   (void)hipEventDestroy(es);
-  for (int i = 0; i < nq; ++i) {
+  for (int q = 0; q < nq; ++q) {
+    const int i = P.qi[q];
     auto r = std::make_unique<svm_result>();
     const int K = qs[i]->K;
     r->kind = K == 0 ? 2 : 1;
-    r->rows = (int64_t)tot[i];
+    r->rows = (int64_t)tot[q];
     for (int k = 0; k < K; ++k) {
       r->types.push_back({Int32, 4});
-      std::vector<uint8_t> col(tot[i] * 4);
-      if (tot[i]) memcpy(col.data(), hk.data() + (size_t)k * rows + em[i].out0, tot[i] * 4);
+      std::vector<uint8_t> col(tot[q] * 4);
+      if (tot[q]) memcpy(col.data(), hk.data() + (size_t)k * rows + em[q].out0, tot[q] * 4);
       r->cols.push_back(std::move(col));
     }
     r->types.push_back({Int64, 8});
-    std::vector<uint8_t> col(tot[i] * 8);
-    if (tot[i]) memcpy(col.data(), hv.data() + em[i].out0, tot[i] * 8);
+    std::vector<uint8_t> col(tot[q] * 8);
+    if (tot[q]) memcpy(col.data(), hv.data() + em[q].out0, tot[q] * 8);
     r->cols.push_back(std::move(col));
-    r->scan_ms = ms;  // the whole batch completes together
+    r->scan_ms = ms_all;  // the whole pass completes together
     *outs[i] = std::move(r);
+  }
+}
+
+// Fused execution of lowered queries that share one base fact table: greedy
+// passes within the limits of one shared scan; a pass whose profiles exceed the
+// LDS staging budget is split in two.
+static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr<svm_result>*>& outs,
+                      Engine& E, hipStream_t s, Scratch& sc, bool tune) {
+  if (qs.empty()) return;
+  std::vector<PassB> passes;
+  PassB cur;
+  for (int i = 0; i < (int)qs.size(); ++i) {
+    if (pass_add(cur, *qs[i], i, s)) continue;
+    passes.push_back(std::move(cur));
+    cur = PassB();
+    if (!pass_add(cur, *qs[i], i, s)) fail(SVM_ERR_INVALID, "a lowered query exceeds the shared scan limits");
+  }
+  if (!cur.qi.empty()) passes.push_back(std::move(cur));
+  while (!passes.empty()) {
+    PassB P = std::move(passes.back());
+    passes.pop_back();
+    uint64_t nprof = 0;
+    for (auto& sl : P.slots) {
+      std::vector<int> attrs(sl.attrs.begin(), sl.attrs.end());
+      auto pr = profile_of(sl.dim, sl.kcol, attrs, s);
+      nprof += pr->ok ? pr->P + 1 : (uint64_t)bk::kMaxProfiles + 1;
+    }
+    if (nprof > (uint64_t)bk::kMaxProfiles && P.qi.size() > 1) {
+      const size_t h = P.qi.size() / 2;
+      passes.push_back(pass_of(qs, std::vector<int>(P.qi.begin(), P.qi.begin() + h), s));
+      passes.push_back(pass_of(qs, std::vector<int>(P.qi.begin() + h, P.qi.end()), s));
+      continue;
+    }
+    run_pass(qs, P, outs, E, s, sc, tune);
   }
 }
 
@@ -651,7 +1125,7 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
       fo.push_back(&res[i]);
     }
     static Scratch* scratch = new Scratch();  // reused by every batch (guarded by run_mu)
-    run_fused(fq, fo, E, s, *scratch);
+    run_fused(fq, fo, E, s, *scratch, true);
   }
   int nfused = 0;
   for (int i = 0; i < n; ++i) nfused += fused[i] ? 1 : 0;
@@ -676,6 +1150,7 @@ struct Ticket {
   int status = SVM_OK;
   std::string error;
   std::unique_ptr<svm_result> result;
+  std::chrono::steady_clock::time_point submitted, admitted, finished;
 };
 
 struct Server {
@@ -693,10 +1168,13 @@ struct Server {
 std::unique_ptr<Server> g_server;
 std::mutex g_server_mu;
 constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a query on the plain path
-constexpr uint64_t kSharedScanMaxRows = 64ull << 20;  // larger slices take the JIT-specialised path
 
+// A served query is self-contained (its dimension pipelines, then its fact
+// pipeline): the objects it materialises are its intermediates and are dropped
+// once its result is on the host, so a long-running server does not accumulate them.
 void run_plain_query(Ticket& t, hipStream_t s) {
   std::unique_ptr<svm_result> last;
+  Engine& E = engine();
   for (auto& j : t.jobs) {
     RunOptions opt;
     opt.stream = s;
@@ -704,6 +1182,7 @@ void run_plain_query(Ticket& t, hipStream_t s) {
     last = run_pipeline(j.info, opt);
   }
   t.result = std::move(last);
+  for (auto& j : t.jobs) E.store.drop(j.info.id);
 }
 
 void worker_main(Server* S, int index) {
@@ -738,6 +1217,8 @@ void worker_main(Server* S, int index) {
       }
       S->bytes_in_use += need;
       S->running += batch.size();
+      const auto now = std::chrono::steady_clock::now();
+      for (auto& t : batch) t->admitted = now;
     }
     try {
       if (batch.size() == 1 && (!batch[0]->fused || S->mode == 1)) {
@@ -749,7 +1230,9 @@ void worker_main(Server* S, int index) {
           fq.push_back(&t->fact);
           fo.push_back(&t->result);
         }
-        run_fused(fq, fo, E, s, sc);
+        // shapes not tuned by an earlier batch_run take the default launch shape:
+        // a serving batch does not stall on tuning trials
+        run_fused(fq, fo, E, s, sc, false);
       }
     } catch (const Error& e) {
       for (auto& t : batch) { t->status = e.code; t->error = e.what(); }
@@ -761,7 +1244,11 @@ void worker_main(Server* S, int index) {
       S->bytes_in_use -= need;
       S->running -= batch.size();
       S->completed += batch.size();
-      for (auto& t : batch) t->done = true;
+      const auto now = std::chrono::steady_clock::now();
+      for (auto& t : batch) {
+        t->finished = now;
+        t->done = true;
+      }
     }
     S->cv_done.notify_all();
     S->cv_work.notify_all();
@@ -809,16 +1296,13 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
     t->fact = lower_fact(t->jobs[n - 1], dims, E, S->submit_stream);
     // every dimension pipeline of the query must be absorbed by the fused fact pipeline
     t->fused = t->fact.ok && t->fact.deferred.size() == (size_t)(n - 1);
-    // a query over a large slice runs faster on its own specialised kernel (the
-    // shared scan interprets descriptors); it still overlaps with batches on
-    // another worker
-    if (t->fused && (uint64_t)(t->fact.q.row1 - t->fact.q.row0) > kSharedScanMaxRows) t->fused = false;
   }
   t->need = t->fused ? fused_bytes(t->fact) : kPlainNeed;
   if (t->need > S->budget) fail(SVM_ERR_BUSY, "query needs more HBM than the serving budget");
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) fail(SVM_ERR_STATE, "server stopping");
+    t->submitted = std::chrono::steady_clock::now();
     t->id = S->next_id++;
     S->tickets[t->id] = t;
     S->pending.push_back(t);
@@ -828,7 +1312,7 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
   return SVM_OK;
 }
 
-int server_wait(uint64_t ticket, int timeout_ms, svm_result** out) {
+int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms, double* total_ms) {
   Server* S = g_server.get();
   if (!S) fail(SVM_ERR_STATE, "server not running");
   std::unique_lock<std::mutex> lk(S->mu);
@@ -840,6 +1324,11 @@ int server_wait(uint64_t ticket, int timeout_ms, svm_result** out) {
   else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
     return SVM_ERR_BUSY;
   S->tickets.erase(ticket);
+  auto ms = [&](std::chrono::steady_clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(a - t->submitted).count();
+  };
+  if (queued_ms) *queued_ms = ms(t->admitted);
+  if (total_ms) *total_ms = ms(t->finished);
   if (t->status != SVM_OK) fail(t->status, t->error);
   if (out) *out = t->result.release();
   return SVM_OK;

@@ -1,7 +1,9 @@
-// batch_kernels.hip — parameter-driven kernels of the batched path: fused
-// dimension filter + map build, the shared scan over (tile, query) work items, and
-// batched ordered group emission.
+// batch_kernels.hip — parameter-driven kernels of the batched path: dimension
+// profiles and their per-batch query masks, the shared scan in which every fact
+// row is read once for all queries of the batch, and batched ordered group emission.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "batch.h"
 #include "svm_internal.h"
@@ -14,6 +16,8 @@ namespace {
 constexpr int kBlock = 256;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
+// The reference's StrCmp: compare up to n bytes, stopping at the first NUL of
+// either side (define.hpp StrCmp).
 __device__ __forceinline__ int ref_strcmp(const char* a, const char* b, int n) {
   for (int k = 0; k < n; ++k) {
     char x = a[k], y = b[k];
@@ -23,190 +27,431 @@ __device__ __forceinline__ int ref_strcmp(const char* a, const char* b, int n) {
   return 0;
 }
 
-__device__ __forceinline__ bool dim_pass(const DimPred& p, unsigned long long r) {
-  const char* v = p.col + r * (unsigned long long)p.size;
-  if (p.kind == DP_RANGE) {
-    int x = p.type == Int16 ? (int)*(const short*)v : p.type == Byte ? (int)*(const signed char*)v : *(const int*)v;
-    return x >= p.lo && x <= p.hi;
+// ------------------------------------------------------------------ profiles
+__global__ void k_profile_raw(RawArgs a, uint64_t n, int32_t* __restrict__ raw) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned v = 0;
+    for (int k = 0; k < a.nattr; ++k) v |= (unsigned)a.codes[k][r] << a.shift[k];
+    raw[r] = (int32_t)v;
   }
-  int c = ref_strcmp(v, p.lit, p.n);
-  return p.kind == DP_STREQ ? c == 0 : c != 0;
 }
 
-__global__ void k_map_build(const MapBuild* __restrict__ builds) {
-  const MapBuild& b = builds[blockIdx.y];
-  const int sh = b.wbits == 1 ? 5 : b.wbits == 2 ? 4 : b.wbits == 4 ? 3 : b.wbits == 8 ? 2 : b.wbits == 16 ? 1 : 0;
-  for (unsigned long long r = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; r < b.n;
-       r += (unsigned long long)gridDim.x * blockDim.x) {
+__global__ void k_profile_scatter(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin,
+                                  const int32_t* __restrict__ pcode, void* __restrict__ pkey, int narrow) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned off = (unsigned)(keys[r] - kmin);
+    if (narrow) ((uint8_t*)pkey)[off] = (uint8_t)pcode[r];
+    else ((uint16_t*)pkey)[off] = (uint16_t)pcode[r];
+  }
+}
+
+// Profile of key offset k of a slot; P when no dimension row has the key.
+__device__ __forceinline__ unsigned slot_profile(const Slot& S, unsigned k) {
+  return S.narrow ? (unsigned)((const uint8_t*)S.pkey)[k] : (unsigned)((const uint16_t*)S.pkey)[k];
+}
+
+// Per batch and slot, one bit per key offset: the key has a dimension row whose
+// profile passes some query that joins the slot.  One wave per 64 keys.
+__global__ void k_slot_bits(const ScanArgs* __restrict__ A) {
+  const unsigned lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (int s = 0; s < A->nslot; ++s) {
+    const Slot& S = A->slot[s];
+    const uint64_t chunks = ((uint64_t)S.range + 63) / 64;
+    for (uint64_t c = wave; c < chunks; c += nwaves) {
+      const uint64_t k = c * 64 + lane;
+      bool pass = false;
+      if (k < S.range) {
+        const unsigned p = slot_profile(S, (unsigned)k);
+        pass = p < S.P && (S.mask[p] & S.joined) != 0ull;
+      }
+      const unsigned long long b = __ballot(pass);
+      if (lane == 0) {
+        S.bits[2 * c] = (unsigned)b;
+        S.bits[2 * c + 1] = (unsigned)(b >> 32);
+      }
+    }
+  }
+}
+
+__global__ void k_string_exact(const char* __restrict__ col, int size, uint64_t n, const int32_t* __restrict__ codes,
+                               const int32_t* __restrict__ rep, unsigned* flag) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    const char* a = col + r * (uint64_t)size;
+    const char* b = col + (uint64_t)rep[codes[r]] * (uint64_t)size;
+    bool same = true;
+    for (int k = 0; k < size; ++k) same = same && a[k] == b[k];
+    if (!same) atomicOr(flag, 1u);
+  }
+}
+
+// One thread per (slot, profile); the grid covers kMaxSlot x kMaxProfiles so the
+// launch does not depend on the batch.
+__global__ void k_profile_masks(const MaskArgs* __restrict__ M) {
+  if ((int)blockIdx.y >= M->nslot) return;
+  const MaskSlot& S = M->slots[blockIdx.y];
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.P) return;
+  const unsigned raw = (unsigned)S.praw[p];
+  unsigned long long m = 0ull;
+  for (int t = S.t0; t < S.t1; ++t) {
+    const MaskTerm& T = M->terms[t];
     bool ok = true;
-    for (int p = 0; p < b.npred; ++p) ok = ok && dim_pass(b.pred[p], r);
-    if (!ok) continue;
-    unsigned off = (unsigned)(b.keys[r] - b.kmin);
-    if (off >= b.range) continue;
-    unsigned val = b.codes ? (unsigned)b.codes[r] + 1u : 1u;
-    unsigned per = 32u >> (5 - sh);  // entries per word (1 when wbits == 32)
-    unsigned pos = sh == 0 ? 0u : (off & (per - 1u)) * (unsigned)b.wbits;
-    atomicOr(&b.map[off >> sh], val << pos);
+    for (int k = 0; k < T.natom && ok; ++k) {
+      const MaskAtom& a = T.atom[k];
+      const unsigned code = a.bits ? (raw >> a.shift) & ((1u << a.bits) - 1u) : 0u;
+      if (a.kind == DP_RANGE) {
+        const int v = a.dict[code];
+        ok = v >= a.lo && v <= a.hi;
+      } else {
+        const char* str = a.col + (uint64_t)a.rep[code] * (uint64_t)a.size;
+        const int c = ref_strcmp(str, a.lit, a.n);
+        ok = a.kind == DP_STREQ ? c == 0 : c != 0;
+      }
+    }
+    if (ok) m |= 1ull << T.q;
+  }
+  S.mask[p] = m;
+}
+
+// --------------------------------------------------------------- shared scan
+// Value of fact column c (uniform across the wave) for element e of this lane: an
+// indexed register read (s_set_gpr_idx), one VALU move instead of a select chain.
+__device__ __forceinline__ int pick(const i32x4 (&x)[kMaxCols], int c, int e) { return x[c][e]; }
+
+// a[i] for a lane-varying i < n, as a chain of selects kept in registers (the
+// empty asm hides that the operands come from one array, which would otherwise be
+// turned into an indexed load from scratch memory).
+template <int N>
+__device__ __forceinline__ unsigned sel(const unsigned (&a)[N], unsigned i, int n) {
+  unsigned r = a[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    if (k < n) {
+      unsigned c = a[k];
+      asm volatile("" : "+v"(c));
+      r = i == (unsigned)k ? c : r;
+    }
+  }
+  return r;
+}
+
+__device__ __forceinline__ int sel_i(const int (&a)[kMaxMetric], int i, int n) {
+  int r = a[0];
+#pragma unroll
+  for (int k = 1; k < kMaxMetric; ++k) {
+    if (k < n) {
+      int c = a[k];
+      asm volatile("" : "+v"(c));
+      r = i == k ? c : r;
+    }
+  }
+  return r;
+}
+
+// Query masks of the shared scan: 32 bits when a scan holds at most 32 queries
+// (half the mask arithmetic per row), else 64.
+__device__ __forceinline__ int first_bit(unsigned b) { return __ffs(b) - 1; }
+__device__ __forceinline__ int first_bit(unsigned long long b) { return __ffsll((long long)b) - 1; }
+template <typename M>
+__device__ __forceinline__ M lds_mask(const unsigned long long* pm, unsigned i) {
+  if (sizeof(M) == 4) return (M)((const unsigned*)pm)[2 * i];  // low half
+  return (M)pm[i];
+}
+
+// Profiles of a slot for the four rows of a lane.  A key without a dimension row
+// maps to profile P, whose mask is 0; rows that no joining query wants read entry
+// `range`, which holds P as well, so the four loads issue together unconditionally.
+template <typename M>
+__device__ __forceinline__ void slot_lookup(const Slot& S, const i32x4 (&x)[kMaxCols], const M (&gate)[4],
+                                            bool use_bits, unsigned (&c)[4]) {
+  const unsigned range = S.range;
+  const M joined = (M)S.joined;
+  unsigned idx[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const unsigned off = (unsigned)(pick(x, S.fcol, e) - S.kmin);
+    idx[e] = (gate[e] & joined) != (M)0 && off < range ? off : range;
+  }
+  if (use_bits) {
+    unsigned w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = S.bits[idx[e] >> 5];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) idx[e] = (w[e] >> (idx[e] & 31u)) & 1u ? idx[e] : range;
+  }
+  if (S.narrow) {
+    const uint8_t* pk = (const uint8_t*)S.pkey;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[e] = pk[idx[e]];
+  } else {
+    const uint16_t* pk = (const uint16_t*)S.pkey;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[e] = pk[idx[e]];
   }
 }
 
-__device__ __forceinline__ unsigned map_get(const unsigned* m, unsigned off, int w, int sh) {
-  if (w == 32) return m[off];
-  unsigned per = 32u >> (5 - sh);
-  return (m[off >> sh] >> ((off & (per - 1u)) * (unsigned)w)) & ((1u << w) - 1u);
-}
-
-struct ColPtrs {
-  const void* c[kMaxCols];
-};
-
-__device__ __forceinline__ int pick(const i32x4 (&x)[kMaxCols], int c, int e) {
-  int v = x[0][e];
+// Drops from each row's query mask the joining queries whose filter its profile fails.
+template <typename M>
+__device__ __forceinline__ void slot_apply(const Slot& S, const unsigned long long* pmask, const unsigned (&c)[4],
+                                           M (&pass)[4]) {
+  const M notj = ~(M)S.joined;
 #pragma unroll
-  for (int k = 1; k < kMaxCols; ++k) v = c == k ? x[k][e] : v;
-  return v;
+  for (int e = 0; e < 4; ++e) pass[e] &= lds_mask<M>(pmask, (unsigned)S.lds + c[e]) | notj;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shared_scan(const Query* __restrict__ qs, const Item* __restrict__ items,
-                                                        unsigned nitems, unsigned* counter, ColPtrs cols,
-                                                        long long tile_rows) {
-  __shared__ unsigned long long lsum[kDenseG];
-  __shared__ unsigned lflag[kDenseG / 32];
-  __shared__ unsigned hkey[kHashSlots];
-  __shared__ unsigned long long hsum[kHashSlots];
-  __shared__ long long red[kBlock / 64];
-  __shared__ unsigned cur;
+// 128-bit group v of every fact column in colmask (zeros for the others).
+__device__ __forceinline__ void load_groups(const ScanArgs* __restrict__ A, unsigned colmask, long long v,
+                                            i32x4 (&x)[kMaxCols]) {
+#pragma unroll
+  for (int c = 0; c < kMaxCols; ++c) {
+    if (colmask & (1u << c)) x[c] = __builtin_nontemporal_load((const i32x4*)A->cols[c] + v);
+    else x[c] = i32x4{0, 0, 0, 0};
+  }
+}
+
+// Work item: a tile of the base table.  Each 128-bit group of the columns any
+// query reads is loaded once; per row, the slice, fact-predicate and slot lookups
+// produce the mask of queries the row passes, and each lane then walks its own
+// (row, query) pairs, accumulating into one LDS hash table keyed by (query, group)
+// that overflows to the global group tables.
+template <int BLOCK, int HBITS, bool PAR, bool PF, typename M>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shared_scan(
+    const ScanArgs* __restrict__ A) {
+  constexpr unsigned kSlots = 1u << HBITS;  // LDS (query, group) accumulator
+  extern __shared__ unsigned long long lds64[];
+  __shared__ unsigned long long tmask[2];
+  __shared__ unsigned cur, used, tcols;
+  const int nq = A->nq, nslot = A->nslot, nmetric = A->nmetric;
+  const bool use_bits = A->use_bits != 0;
+  const unsigned nprof = A->nprof;
+  unsigned long long* hsum = lds64;
+  unsigned long long* pmask = hsum + kSlots;
+  unsigned* hkey = (unsigned*)(pmask + nprof);
+  unsigned* praw = hkey + kSlots;
+  QDesc* qd = (QDesc*)(praw + ((nprof + 3u) & ~3u));
+  unsigned long long* const gsum = A->sums;
+  unsigned* const gflag = A->flags;
+
+  // each slot's P profiles, then its no-row profile P (mask 0)
+  for (int s = 0; s < nslot; ++s) {
+    const unsigned P = A->slot[s].P, off = (unsigned)A->slot[s].lds;
+    const unsigned long long* m = A->slot[s].mask;
+    const int32_t* r = A->slot[s].praw;
+    for (unsigned p = threadIdx.x; p <= P; p += BLOCK) {
+      pmask[off + p] = p < P ? m[p] : 0ull;
+      praw[off + p] = p < P ? (unsigned)r[p] : 0u;
+    }
+  }
+  for (unsigned i = threadIdx.x; i < (unsigned)nq * (unsigned)(sizeof(QDesc) / 4); i += BLOCK)
+    ((unsigned*)qd)[i] = ((const unsigned*)A->qd)[i];
+  for (unsigned i = threadIdx.x; i < (unsigned)kSlots; i += BLOCK) {
+    hkey[i] = 0u;
+    hsum[i] = 0ull;
+  }
+  if (threadIdx.x == 0) used = 0u;
+  // a scalar query yields its one row even when no row passes
+  if (blockIdx.x == 0 && (int)threadIdx.x < nq && A->qd[threadIdx.x].K == 0)
+    atomicOr(gflag + A->qd[threadIdx.x].flag_off, 1u);
+  __syncthreads();
+
+  auto flush = [&]() {
+    for (unsigned i = threadIdx.x; i < (unsigned)kSlots; i += BLOCK) {
+      const unsigned k = hkey[i];
+      if (k) {
+        const unsigned key = k - 1u, q = key >> 22, gid = key & ((1u << 22) - 1u);
+        atomicAdd(gsum + qd[q].sum_off + gid, hsum[i]);
+        atomicOr(gflag + qd[q].flag_off + (gid >> 5), 1u << (gid & 31u));
+        hkey[i] = 0u;
+        hsum[i] = 0ull;
+      }
+    }
+  };
+
   for (;;) {
-    if (threadIdx.x == 0) cur = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned it = cur;
-    if (it >= nitems) break;
-    const Item w = items[it];
-    const Query& Q = qs[w.q];
-    const long long lo = max((long long)w.tile * tile_rows, Q.row0);
-    const long long hi = min(((long long)w.tile + 1) * tile_rows, Q.row1);
-    const int agg = Q.agg, npred = Q.npred, njoin = Q.njoin, mop = Q.mop, ma = Q.ma, mb = Q.mb;
-    const unsigned G = Q.G;
-    unsigned mask = 1u << ma;
-    if (mop != MOP_COL) mask |= 1u << mb;
-    for (int p = 0; p < npred; ++p) mask |= 1u << Q.pcol[p];
-    for (int j = 0; j < njoin; ++j) mask |= 1u << Q.jcol[j];
-    int pcol[kMaxPred], plo[kMaxPred], phi[kMaxPred];
-#pragma unroll
-    for (int p = 0; p < kMaxPred; ++p) {
-      pcol[p] = Q.pcol[p];
-      plo[p] = Q.plo[p];
-      phi[p] = Q.phi[p];
-    }
-    int jcol[kMaxJoin], jkmin[kMaxJoin], jw[kMaxJoin], jsh[kMaxJoin], jcode[kMaxJoin];
-    unsigned jrange[kMaxJoin], jstride[kMaxJoin];
-    const unsigned* jmap[kMaxJoin];
-#pragma unroll
-    for (int j = 0; j < kMaxJoin; ++j) {
-      jcol[j] = Q.jcol[j];
-      jkmin[j] = Q.jkmin[j];
-      jw[j] = Q.jw[j];
-      jsh[j] = jw[j] == 1 ? 5 : jw[j] == 2 ? 4 : jw[j] == 4 ? 3 : jw[j] == 8 ? 2 : jw[j] == 16 ? 1 : 0;
-      jcode[j] = Q.jcode[j];
-      jrange[j] = Q.jrange[j];
-      jstride[j] = Q.jstride[j];
-      jmap[j] = Q.jmap[j];
-    }
-    unsigned long long* gsum = Q.sums;
-    unsigned* gflag = Q.flags;
-    if (agg == BAGG_DENSE) {
-      for (unsigned g = threadIdx.x; g < G; g += kBlock) lsum[g] = 0ull;
-      for (unsigned g = threadIdx.x; g < (G + 31u) / 32u; g += kBlock) lflag[g] = 0u;
-    } else if (agg == BAGG_HASH) {
-      for (unsigned g = threadIdx.x; g < (unsigned)kHashSlots; g += kBlock) { hkey[g] = 0u; hsum[g] = 0ull; }
-    }
-    __syncthreads();
-    long long acc = 0;
-    for (long long v = (lo >> 2) + threadIdx.x; v < ((hi + 3) >> 2); v += kBlock) {
-      i32x4 x[kMaxCols];
-#pragma unroll
-      for (int c = 0; c < kMaxCols; ++c) {
-        if (mask & (1u << c)) x[c] = __builtin_nontemporal_load((const i32x4*)cols.c[c] + v);
-        else x[c] = i32x4{0, 0, 0, 0};
+    if (threadIdx.x < 64) {
+      unsigned t0 = 0u;
+      if (threadIdx.x == 0) t0 = atomicAdd(A->counter, 1u);
+      t0 = __shfl(t0, 0);
+      const long long lo = A->row_lo + (long long)t0 * A->tile_rows;
+      const long long hi = min(lo + A->tile_rows, A->row_hi);
+      bool full = false, part = false;
+      unsigned cm = 0u;
+      if (t0 < A->ntiles && (int)threadIdx.x < nq) {
+        const long long r0 = A->qrow[2 * threadIdx.x], r1 = A->qrow[2 * threadIdx.x + 1];
+        full = r0 <= lo && r1 >= hi;
+        part = !full && r0 < hi && r1 > lo;
+        if (full || part) cm = A->qcols[threadIdx.x];
       }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long long i = v * 4 + e;
-        bool cond = i >= lo && i < hi;
-        for (int p = 0; p < npred && cond; ++p) {
-          int val = pick(x, pcol[p], e);
-          cond = val >= plo[p] && val <= phi[p];
-        }
-        unsigned gid = 0;
-        for (int j = 0; j < njoin && cond; ++j) {
-          unsigned off = (unsigned)(pick(x, jcol[j], e) - jkmin[j]);
-          unsigned pv = off < jrange[j] ? map_get(jmap[j], off, jw[j], jsh[j]) : 0u;
-          cond = pv != 0u;
-          if (jcode[j]) gid += (pv - 1u) * jstride[j];
-        }
-        if (!cond) continue;
-        int a = pick(x, ma, e);
-        int b = mop == MOP_COL ? 0 : pick(x, mb, e);
-        int r = mop == MOP_COL ? a
-              : mop == MOP_SUB ? (int)((unsigned)a - (unsigned)b)
-              : mop == MOP_ADD ? (int)((unsigned)a + (unsigned)b)
-                               : (int)((unsigned)a * (unsigned)b);
-        long long m = (long long)r;
-        if (agg == BAGG_SCALAR) {
-          acc += m;
-        } else if (agg == BAGG_DENSE) {
-          atomicAdd(&lsum[gid], (unsigned long long)m);
-          atomicOr(&lflag[gid >> 5], 1u << (gid & 31u));
-        } else {
-          unsigned h = (gid * 2654435761u) >> 22;  // 10 bits
-          int slot = -1;
-          for (int t = 0; t < 16; ++t) {
-            unsigned k = hkey[h];
-            if (k == gid + 1u) { slot = (int)h; break; }
-            if (k == 0u) {
-              unsigned old = atomicCAS(&hkey[h], 0u, gid + 1u);
-              if (old == 0u || old == gid + 1u) { slot = (int)h; break; }
-            }
-            h = (h + 1u) & (kHashSlots - 1u);
-          }
-          if (slot >= 0) atomicAdd(&hsum[slot], (unsigned long long)m);
-          else {
-            atomicAdd(&gsum[gid], (unsigned long long)m);
-            atomicOr(&gflag[gid >> 5], 1u << (gid & 31u));
-          }
-        }
-      }
-    }
-    if (agg == BAGG_SCALAR) {
-      for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d);
-      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    }
-    __syncthreads();
-    if (agg == BAGG_SCALAR) {
+      const unsigned long long fm = __ballot(full), pm = __ballot(part);
+      for (int d = 32; d > 0; d >>= 1) cm |= __shfl_xor(cm, d);
       if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int q = 0; q < kBlock / 64; ++q) t += red[q];
-        atomicAdd(&gsum[0], (unsigned long long)t);
-        atomicOr(&gflag[0], 1u);
-      }
-    } else if (agg == BAGG_DENSE) {
-      for (unsigned g = threadIdx.x; g < G; g += kBlock)
-        if ((lflag[g >> 5] >> (g & 31u)) & 1u) atomicAdd(&gsum[g], lsum[g]);
-      for (unsigned g = threadIdx.x; g < (G + 31u) / 32u; g += kBlock)
-        if (lflag[g]) atomicOr(&gflag[g], lflag[g]);
-    } else {
-      for (unsigned s = threadIdx.x; s < (unsigned)kHashSlots; s += kBlock) {
-        unsigned k = hkey[s];
-        if (k) {
-          atomicAdd(&gsum[k - 1u], hsum[s]);
-          atomicOr(&gflag[(k - 1u) >> 5], 1u << ((k - 1u) & 31u));
-        }
+        cur = t0;
+        tmask[0] = fm;
+        tmask[1] = pm;
+        tcols = cm;  // columns the tile's queries read
       }
     }
     __syncthreads();
+    const unsigned t = cur;
+    if (t >= A->ntiles) break;
+    const M full = (M)tmask[0], part = (M)tmask[1], active = full | part;
+    const unsigned colmask = tcols;
+    const long long lo = A->row_lo + (long long)t * A->tile_rows;  // a multiple of 4
+    const long long hi = min(lo + A->tile_rows, A->row_hi);
+    if (active) {
+      const long long vend = (hi + 3) >> 2;
+      long long v = (lo >> 2) + threadIdx.x;
+      i32x4 x[kMaxCols];
+      if (v < vend) load_groups(A, colmask, v, x);
+      for (; v < vend; v += BLOCK) {
+        // PF: the next group's loads are issued before this group's lookups
+        i32x4 xn[kMaxCols];
+        if (PF) load_groups(A, colmask, v + BLOCK < vend ? v + BLOCK : v, xn);
+        else {
+#pragma unroll
+          for (int c = 0; c < kMaxCols; ++c) xn[c] = x[c];
+        }
+        const long long left = hi - v * 4;  // rows of this group inside the tile (>= 1)
+        M pass[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          M m = e < left ? full : (M)0;
+          if (e < left) {
+            const long long i = v * 4 + e;
+            for (M pb = part; pb; pb &= pb - (M)1) {
+              const int q = first_bit(pb);
+              if (i >= A->qrow[2 * q] && i < A->qrow[2 * q + 1]) m |= (M)1 << q;
+            }
+          }
+          pass[e] = m;
+        }
+        // fact predicates of the tile's queries, grouped by column
+#pragma unroll
+        for (int c = 0; c < kMaxCols; ++c) {
+          const int p1 = A->pred0[c + 1];
+          for (int p = A->pred0[c]; p < p1; ++p) {
+            const int q = A->pred[p].q;
+            if (!((active >> q) & (M)1)) continue;
+            const int plo = A->pred[p].lo, phi = A->pred[p].hi;
+            const M keep = ~((M)1 << q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (x[c][e] < plo || x[c][e] > phi) pass[e] &= keep;
+          }
+        }
+        // joins: one profile lookup per slot for the rows some joining query still
+        // wants.  PAR issues every slot's lookups before combining any (one memory
+        // latency per row group); otherwise each slot is gated by the slots before
+        // it (fewer lookups when the first joins are selective).
+        unsigned cid[kMaxSlot][2];  // two rows' 16-bit profile ids per register
+        if (PAR) {
+          unsigned cs[kMaxSlot][4];
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cs[s][e] = 0u;
+            if (s < nslot && ((M)A->slot[s].joined & active)) slot_lookup<M>(A->slot[s], x, pass, use_bits, cs[s]);
+          }
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) {
+            if (s < nslot && ((M)A->slot[s].joined & active)) slot_apply<M>(A->slot[s], pmask, cs[s], pass);
+            cid[s][0] = cs[s][0] | cs[s][1] << 16;
+            cid[s][1] = cs[s][2] | cs[s][3] << 16;
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) {
+            unsigned cs[4] = {0u, 0u, 0u, 0u};
+            if (s < nslot && ((M)A->slot[s].joined & active)) {
+              slot_lookup<M>(A->slot[s], x, pass, use_bits, cs);
+              slot_apply<M>(A->slot[s], pmask, cs, pass);
+            }
+            cid[s][0] = cs[0] | cs[1] << 16;
+            cid[s][1] = cs[2] | cs[3] << 16;
+          }
+        }
+        if (pass[0] | pass[1] | pass[2] | pass[3]) {
+        // distinct metric expressions
+        int mval[kMaxMetric][4];
+#pragma unroll
+        for (int k = 0; k < kMaxMetric; ++k) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) mval[k][e] = 0;
+          if (k < nmetric) {
+            const int op = A->mop[k], ca = A->ma[k], cb = A->mb[k];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const unsigned a = (unsigned)pick(x, ca, e), b = (unsigned)pick(x, cb, e);
+              mval[k][e] = op == MOP_COL ? (int)a : op == MOP_SUB ? (int)(a - b) : op == MOP_ADD ? (int)(a + b)
+                                                                                    : (int)(a * b);
+            }
+          }
+        }
+        // this lane's (row, query) pairs
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          unsigned cs[kMaxSlot];
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) cs[s] = (cid[s][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+          int ms[kMaxMetric];
+#pragma unroll
+          for (int k = 0; k < kMaxMetric; ++k) ms[k] = mval[k][e];
+          for (M bits = pass[e]; bits; bits &= bits - (M)1) {
+            const unsigned q = (unsigned)first_bit(bits);
+            const QDesc& d = qd[q];
+            unsigned gid = 0u;
+            for (int k = 0; k < d.K; ++k) {
+              const unsigned kw = d.key[k];
+              const unsigned c = sel<kMaxSlot>(cs, (kw >> 13) & 7u, nslot);
+              const unsigned raw = praw[(kw & 8191u) + c];
+              const unsigned nb = (kw >> 21) & 31u;
+              gid += ((raw >> ((kw >> 16) & 31u)) & ((1u << nb) - 1u)) * d.stride[k];
+            }
+            const unsigned long long mv = (unsigned long long)(long long)sel_i(ms, d.metric, nmetric);
+            const unsigned key = ((q << 22) | gid) + 1u;
+            unsigned h = (key * 2654435761u) >> (32 - HBITS);
+            bool done = false;
+            for (int probe = 0; probe < 8; ++probe) {
+              unsigned k = hkey[h];
+              if (k == 0u) {
+                const unsigned old = atomicCAS(&hkey[h], 0u, key);
+                if (old == 0u) atomicAdd(&used, 1u);
+                k = old == 0u ? key : old;
+              }
+              if (k == key) {
+                atomicAdd(&hsum[h], mv);
+                done = true;
+                break;
+              }
+              h = (h + 1u) & (unsigned)(kSlots - 1);
+            }
+            if (!done) {
+              atomicAdd(gsum + d.sum_off + gid, mv);
+              atomicOr(gflag + d.flag_off + (gid >> 5), 1u << (gid & 31u));
+            }
+          }
+        }
+        }  // some row of the lane passes
+        // the next group: already in flight (PF) or requested now
+#pragma unroll
+        for (int c = 0; c < kMaxCols; ++c) x[c] = xn[c];
+        if (!PF && v + BLOCK < vend) load_groups(A, colmask, v + BLOCK, x);
+      }
+    }
+    __syncthreads();
+    if (used > (unsigned)kSlots / 2u) {
+      flush();
+      __syncthreads();
+      if (threadIdx.x == 0) used = 0u;
+    }
   }
+  __syncthreads();
+  flush();
 }
 
+// ------------------------------------------------------------------ emission
 // Block b of the flattened emission grid belongs to the query whose block range holds it.
 __device__ __forceinline__ int owner(const Emit* e, int nq, unsigned long long b) {
   int lo = 0, hi = nq - 1;
@@ -217,9 +462,22 @@ __device__ __forceinline__ int owner(const Emit* e, int nq, unsigned long long b
   return lo;
 }
 
-__global__ void k_emit_count(const Emit* __restrict__ e, int nq, unsigned long long total_blocks,
+// Zeroes the pass's group tables and its tile counter.
+__global__ void k_pass_clear(const ScanArgs* __restrict__ A) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long* sums = A->sums;
+  unsigned* flags = A->flags;
+  for (uint64_t i = t0; i < A->sum_words; i += stride) sums[i] = 0ull;
+  for (uint64_t i = t0; i < A->flag_words; i += stride) flags[i] = 0u;
+  if (t0 == 0) *A->counter = 0u;
+}
+
+__global__ void k_emit_count(const Emit* __restrict__ e, const ScanArgs* __restrict__ A,
                              unsigned long long* counts) {
   __shared__ unsigned part[kBlock / 64];
+  const int nq = A->nq;
+  const unsigned long long total_blocks = A->total_blocks;
   for (unsigned long long b = blockIdx.x; b < total_blocks; b += gridDim.x) {
     int q = owner(e, nq, b);
     unsigned long long w = (b - e[q].block0) * kBlock + threadIdx.x;
@@ -238,10 +496,11 @@ __global__ void k_emit_count(const Emit* __restrict__ e, int nq, unsigned long l
 }
 
 // One workgroup per query: exclusive scan of its block counts, global row offsets.
-__global__ void k_emit_scan(const Emit* __restrict__ e, int nq, const unsigned long long* counts,
-                            unsigned long long* offsets, unsigned long long* totals) {
+__global__ void k_emit_scan(const Emit* __restrict__ e, const ScanArgs* __restrict__ A,
+                            const unsigned long long* counts, unsigned long long* offsets,
+                            unsigned long long* totals) {
   const int q = blockIdx.x;
-  if (q >= nq) return;
+  if (q >= A->nq) return;
   __shared__ unsigned long long buf[kBlock];
   unsigned long long carry = 0;
   for (unsigned long long base = 0; base < e[q].nblocks; base += kBlock) {
@@ -298,32 +557,107 @@ __global__ void k_emit_rows(const Emit* __restrict__ e, int nq, unsigned long lo
   }
 }
 
+inline unsigned grid_rows(uint64_t n) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g == 0) g = 1;
+  return (unsigned)(g > 4096 ? 4096 : g);
+}
+
 }  // namespace
 
-void launch_map_build(const MapBuild* d_builds, int nbuild, unsigned long long max_rows, hipStream_t s) {
-  if (nbuild <= 0) return;
-  unsigned long long gx = (max_rows + kBlock - 1) / kBlock;
-  if (gx > 1024) gx = 1024;
-  if (gx == 0) gx = 1;
-  hipLaunchKernelGGL(k_map_build, dim3((unsigned)gx, (unsigned)nbuild), dim3(kBlock), 0, s, d_builds);
-  hip_check(hipGetLastError(), "k_map_build");
+void launch_profile_raw(const RawArgs& a, uint64_t n, int32_t* raw, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_profile_raw, dim3(grid_rows(n)), dim3(kBlock), 0, s, a, n, raw);
+  hip_check(hipGetLastError(), "k_profile_raw");
 }
 
-void launch_shared_scan(const Query* d_q, const Item* d_items, unsigned nitems, unsigned* d_counter,
-                        const void* const* cols, long long tile_rows, int grid, hipStream_t s) {
-  ColPtrs cp;
-  for (int c = 0; c < kMaxCols; ++c) cp.c[c] = cols[c];
-  hipLaunchKernelGGL(k_shared_scan, dim3(grid), dim3(kBlock), 0, s, d_q, d_items, nitems, d_counter, cp,
-                     tile_rows);
-  hip_check(hipGetLastError(), "k_shared_scan");
+void launch_profile_scatter(const int32_t* keys, uint64_t n, int32_t kmin, const int32_t* pcode, void* pkey,
+                            int narrow, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_profile_scatter, dim3(grid_rows(n)), dim3(kBlock), 0, s, keys, n, kmin, pcode, pkey, narrow);
+  hip_check(hipGetLastError(), "k_profile_scatter");
 }
 
-void launch_emit_count(const Emit* d_e, int nq, unsigned long long total_blocks, unsigned long long* d_counts,
+void launch_string_exact(const char* col, int size, uint64_t n, const int32_t* codes, const int32_t* rep,
+                         unsigned* flag, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_string_exact, dim3(grid_rows(n)), dim3(kBlock), 0, s, col, size, n, codes, rep, flag);
+  hip_check(hipGetLastError(), "k_string_exact");
+}
+
+// Every launch below except the emission of rows has a grid and arguments that do
+// not depend on the batch (sizes are read from the device descriptors), so the
+// sequence clear -> masks -> scan -> emission count can be captured once in a
+// hipGraph and replayed for any batch whose descriptors are uploaded in place.
+constexpr unsigned kClearGrid = 1024;
+constexpr unsigned kEmitGrid = 2048;
+
+void launch_pass_clear(const ScanArgs* d_args, hipStream_t s) {
+  hipLaunchKernelGGL(k_pass_clear, dim3(kClearGrid), dim3(kBlock), 0, s, d_args);
+  hip_check(hipGetLastError(), "k_pass_clear");
+}
+
+void launch_profile_masks(const MaskArgs* d_margs, hipStream_t s) {
+  hipLaunchKernelGGL(k_profile_masks, dim3(kMaxProfiles / kBlock, kMaxSlot), dim3(kBlock), 0, s, d_margs);
+  hip_check(hipGetLastError(), "k_profile_masks");
+}
+
+void launch_slot_bits(const ScanArgs* d_args, hipStream_t s) {
+  hipLaunchKernelGGL(k_slot_bits, dim3(kClearGrid), dim3(kBlock), 0, s, d_args);
+  hip_check(hipGetLastError(), "k_slot_bits");
+}
+
+namespace {
+struct Variant {
+  int block, hbits;
+  bool par, pf, mask32;
+  const void* fn;
+};
+#define SVM_SCAN_VARIANT(B, H, PAR, PF, M32) \
+  { B, H, PAR, PF, M32, (const void*)k_shared_scan<B, H, PAR, PF, std::conditional_t<M32, unsigned, unsigned long long>> }
+const Variant kVariants[] = {
+    SVM_SCAN_VARIANT(512, 11, false, false, false), SVM_SCAN_VARIANT(512, 11, false, false, true),
+    SVM_SCAN_VARIANT(512, 11, true, false, false),  SVM_SCAN_VARIANT(512, 11, true, false, true),
+    SVM_SCAN_VARIANT(512, 11, false, true, false),  SVM_SCAN_VARIANT(512, 11, false, true, true),
+    SVM_SCAN_VARIANT(256, 10, false, false, false), SVM_SCAN_VARIANT(256, 10, false, false, true),
+};
+#undef SVM_SCAN_VARIANT
+}  // namespace
+
+int scan_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+int scan_variant_block(int v) { return kVariants[v].block; }
+int scan_variant_hash_slots(int v) { return 1 << kVariants[v].hbits; }
+bool scan_variant_parallel(int v) { return kVariants[v].par; }
+bool scan_variant_prefetch(int v) { return kVariants[v].pf; }
+bool scan_variant_mask32(int v) { return kVariants[v].mask32; }
+
+size_t shared_scan_lds(int variant, unsigned nprof_cap) {
+  return ((size_t)1 << kVariants[variant].hbits) * 12 + (size_t)nprof_cap * 12 + (size_t)kMaxBatch * sizeof(QDesc);
+}
+
+void prepare_shared_scan() {
+  static const bool done = [] {
+    for (int v = 0; v < scan_variants(); ++v)
+      hip_check(hipFuncSetAttribute(kVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shared_scan_lds(v, kMaxProfiles)),
+                "k_shared_scan LDS limit");
+    return true;
+  }();
+  (void)done;
+}
+
+void launch_shared_scan(const ScanArgs* d_args, int variant, unsigned nprof_cap, int grid, hipStream_t s) {
+  const size_t lds = shared_scan_lds(variant, nprof_cap);
+  const ScanArgs* arg = d_args;
+  void* params[] = {&arg};
+  hip_check(hipLaunchKernel(kVariants[variant].fn, dim3(grid), dim3(kVariants[variant].block), params, lds, s),
+            "k_shared_scan");
+}
+
+void launch_emit_count(const Emit* d_e, const ScanArgs* d_args, unsigned long long* d_counts,
                        unsigned long long* d_offsets, unsigned long long* d_totals, hipStream_t s) {
-  if (nq <= 0) return;
-  unsigned g = (unsigned)(total_blocks < 4096 ? (total_blocks ? total_blocks : 1) : 4096);
-  hipLaunchKernelGGL(k_emit_count, dim3(g), dim3(kBlock), 0, s, d_e, nq, total_blocks, d_counts);
-  hipLaunchKernelGGL(k_emit_scan, dim3(nq), dim3(kBlock), 0, s, d_e, nq, d_counts, d_offsets, d_totals);
+  hipLaunchKernelGGL(k_emit_count, dim3(kEmitGrid), dim3(kBlock), 0, s, d_e, d_args, d_counts);
+  hipLaunchKernelGGL(k_emit_scan, dim3(kMaxBatch), dim3(kBlock), 0, s, d_e, d_args, d_counts, d_offsets, d_totals);
   hip_check(hipGetLastError(), "k_emit_count");
 }
 

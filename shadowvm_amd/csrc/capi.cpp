@@ -9,7 +9,7 @@ namespace svm {
 int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs);
 int server_start(int workers, uint64_t budget, int max_batch, int mode);
 int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64_t* ticket);
-int server_wait(uint64_t ticket, int timeout_ms, svm_result** out);
+int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms, double* total_ms);
 int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes);
 int server_stop();
 }  // namespace svm
@@ -246,7 +246,14 @@ int svm_server_submit(const uint8_t* const* plans, const size_t* lens, int n, ui
 int svm_server_wait(uint64_t ticket, int timeout_ms, svm_result** out) {
   if (out) *out = nullptr;
   int rc = SVM_OK;
-  int g = guard([&] { rc = server_wait(ticket, timeout_ms, out); });
+  int g = guard([&] { rc = server_wait(ticket, timeout_ms, out, nullptr, nullptr); });
+  if (g == SVM_OK && rc == SVM_ERR_BUSY) t_err = "timed out waiting for ticket";
+  return g != SVM_OK ? g : rc;
+}
+int svm_server_wait_timed(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms, double* total_ms) {
+  if (out) *out = nullptr;
+  int rc = SVM_OK;
+  int g = guard([&] { rc = server_wait(ticket, timeout_ms, out, queued_ms, total_ms); });
   if (g == SVM_OK && rc == SVM_ERR_BUSY) t_err = "timed out waiting for ticket";
   return g != SVM_OK ? g : rc;
 }

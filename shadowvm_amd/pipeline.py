@@ -88,7 +88,7 @@ def table_view(id, base_id, row0, rows):
 def batch_run(plans, want_result=True):
     """svm_batch_run over plan dicts; returns (results list, number fused).
     Absorbed dimension pipelines give None."""
-    datas = [serialize(p) for p in plans]
+    datas = [_bytes(p) for p in plans]
     n = len(datas)
     bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
     arr = (ctypes.POINTER(ctypes.c_uint8) * n)(*[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
@@ -99,8 +99,13 @@ def batch_run(plans, want_result=True):
     return res, fused
 
 
+def _bytes(plan):
+    """PipelineInfo bytes of a plan dict (bytes pass through unchanged)."""
+    return bytes(plan) if isinstance(plan, (bytes, bytearray)) else serialize(plan)
+
+
 def _plan_array(plans):
-    datas = [serialize(p) for p in plans]
+    datas = [_bytes(p) for p in plans]
     bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
     arr = (ctypes.POINTER(ctypes.c_uint8) * len(datas))(
         *[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
@@ -125,6 +130,17 @@ def server_wait(ticket, timeout_ms=-1):
     out = ctypes.c_void_p()
     _lib.check(_lib.lib().svm_server_wait(ticket, timeout_ms, ctypes.byref(out)))
     return _take_result(out) if out.value else None
+
+
+def server_wait_timed(ticket, timeout_ms=-1, want_result=True):
+    """svm_server_wait_timed: (result or None, queued ms, total ms), the latencies
+    measured by the server from the query's submission."""
+    out = ctypes.c_void_p()
+    q, t = ctypes.c_double(), ctypes.c_double()
+    _lib.check(_lib.lib().svm_server_wait_timed(ticket, timeout_ms, ctypes.byref(out) if want_result else None,
+                                                ctypes.byref(q), ctypes.byref(t)))
+    res = _take_result(out) if (want_result and out.value) else None
+    return res, q.value, t.value
 
 
 def server_stats():
@@ -213,8 +229,9 @@ def _take_result(ptr) -> Result:
 
 
 def run_plan(plan: dict, want_result=True) -> Result | None:
-    """Serialize a plan dict to PipelineInfo bytes and run it (svm_pipeline_run)."""
-    data = serialize(plan)
+    """Serialize a plan dict to PipelineInfo bytes (or take the bytes) and run it
+    (svm_pipeline_run)."""
+    data = _bytes(plan)
     buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
     out = ctypes.c_void_p()
     _lib.check(_lib.lib().svm_pipeline_run(buf, len(data), ctypes.byref(out) if want_result else None))

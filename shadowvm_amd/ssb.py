@@ -289,7 +289,7 @@ def batch_queries(wl: Workload, n=64, seed=1, window=60_000_000, min_rows=1_000_
     return out
 
 
-def serve_queries(wl: Workload, n, seed=1, light_frac=0.9, light_rows=60_000_000, prefix="sq"):
+def serve_queries(wl: Workload, n, seed=1, light_frac=0.9, light_rows=60_000_000, prefix="sq", id_digits=0):
     """Serving mix over the four-way tables: light queries (one join, <= 0.1% of
     rows pass, <= 100 groups, a slice of at most `light_rows` rows anywhere in the
     fact table) and heavy queries (the full four-way star join over the whole
@@ -300,7 +300,7 @@ def serve_queries(wl: Workload, n, seed=1, light_frac=0.9, light_rows=60_000_000
     fact_id, fact_rows = ids[GEN_FACT3], wl.fact_rows
     out = []
     for qn in range(n):
-        qid = f"{prefix}{qn}"
+        qid = f"{prefix}{qn:0{id_digits}d}" if id_digits else f"{prefix}{qn}"
         if rng.random() >= light_frac:
             # heavy: the four-way star join with seeded constants, whole table
             h = four_way(seed=rng.randint(1, 1 << 30), fact_rows=fact_rows, prefix=qid)
