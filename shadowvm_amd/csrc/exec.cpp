@@ -10,6 +10,8 @@
 // the scan (LDS tables + 64-bit atomics) followed by an ordered group emission.
 #include "exec.h"
 
+#include "loader.h"
+
 #include <climits>
 #include <cstring>
 #include <set>
@@ -111,8 +113,6 @@ struct Ctx {
 };
 
 void resolve(Ctx& c) {
-  if (c.plan.has_loader)
-    fail(SVM_ERR_UNSUPPORTED, "Loader pipelines (CSV ingest) are not part of the device path");
   if (c.info.pre_pipeline_id.empty())
     fail(SVM_ERR_NOT_FOUND, "pipeline '" + c.info.id + "' names no source object (pre_pipeline_id[0])");
   c.src = c.E.store.get(c.info.pre_pipeline_id[0]);
@@ -729,6 +729,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
 std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOptions& opt) {
   Engine& E = engine();
   Plan plan = decode_plan(info);
+  if (plan.has_loader) return run_load(info, plan, opt);
   Ctx c{E, plan, info, opt.stream ? opt.stream : E.stream, nullptr, {}, {}};
   // runs on one stream are sequential and each ends synchronised, so the previous
   // run's temporaries on this stream are free

@@ -1,0 +1,158 @@
+// loader.cpp — the Loader operator on the device (see loader.h).
+//
+// The reference maps each file of the Source path as one bulk (a directory gives
+// one bulk per file, object/object.cpp:122-169) and parses it on the host, one
+// task per bulk, into a sink bulk (core/operator.cpp:24-48).  Here every file is
+// mapped, its rows counted on the host (memchr), its bytes copied to HBM and
+// parsed there; the files' rows are concatenated, in file-name order, into one
+// resident table cached under the pipeline id.
+#include "loader.h"
+
+#include <dirent.h>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace svm {
+
+namespace {
+
+struct Mapped {
+  std::string path;
+  int fd = -1;
+  const char* data = nullptr;
+  uint64_t size = 0;
+  uint64_t rows = 0;
+  ~Mapped() {
+    if (data) munmap((void*)data, size);
+    if (fd >= 0) close(fd);
+  }
+};
+
+std::vector<std::string> source_files(const std::string& path) {
+  struct stat st;
+  if (path.empty() || stat(path.c_str(), &st) != 0) fail(SVM_ERR_NOT_FOUND, "Loader source '" + path + "' not found");
+  std::vector<std::string> files;
+  if (S_ISDIR(st.st_mode)) {
+    DIR* d = opendir(path.c_str());
+    if (!d) fail(SVM_ERR_NOT_FOUND, "cannot open directory '" + path + "'");
+    while (struct dirent* e = readdir(d)) {
+      std::string name = e->d_name;
+      if (name.empty() || name[0] == '.') continue;  // as object/object.cpp:133
+      std::string full = path + (path.back() == '/' ? "" : "/") + name;
+      struct stat fs;
+      if (stat(full.c_str(), &fs) == 0 && S_ISREG(fs.st_mode)) files.push_back(full);
+    }
+    closedir(d);
+    std::sort(files.begin(), files.end());  // a stable order (readdir's is the file system's)
+  } else {
+    files.push_back(path);
+  }
+  return files;
+}
+
+}  // namespace
+
+std::unique_ptr<svm_result> run_load(const PipelineInfo& info, const Plan& plan, const RunOptions& opt) {
+  Engine& E = engine();
+  const auto t0 = std::chrono::steady_clock::now();
+  if (plan.source_type != "file") fail(SVM_ERR_INVALID, "a Loader pipeline reads a Source of type file");
+  if (plan.col_sep.empty() || plan.row_sep.empty())
+    fail(SVM_ERR_INVALID, "Loader needs col_spilter and row_spilter (core/operator.cpp:16-17)");
+  const char col_sep = plan.col_sep[0], row_sep = plan.row_sep[0];
+  const ObjectInfo& schema = !plan.output_info.column_list.empty() ? plan.output_info
+                             : !plan.loader_info.column_list.empty() ? plan.loader_info
+                                                                     : plan.source_info;
+  const int ncols = (int)schema.column_list.size();
+  if (ncols == 0 || ncols > ld::kMaxLoadCols)
+    fail(SVM_ERR_INVALID, "Loader schema must have 1.." + std::to_string(ld::kMaxLoadCols) + " columns");
+  std::vector<std::pair<int, int64_t>> oschema;
+  for (auto& c : schema.column_list) {
+    if (c.type != Byte && c.type != Int16 && c.type != Int32 && c.type != Int64 && c.type != String)
+      fail(SVM_ERR_UNSUPPORTED, "Loader column type " + std::to_string(c.type) + " is not on the integer path");
+    int64_t vs = c.val_size > 0 ? c.val_size : (c.type == Int64 ? 8 : c.type == Int16 ? 2 : c.type == Byte ? 1 : 4);
+    if (c.type == Int32 && vs != 4) fail(SVM_ERR_INVALID, "Int32 column with val_size != 4");
+    oschema.push_back({c.type, vs});
+  }
+  // map the files and count their rows
+  std::vector<std::unique_ptr<Mapped>> files;
+  uint64_t total = 0;
+  for (auto& f : source_files(plan.source_path)) {
+    auto m = std::make_unique<Mapped>();
+    m->path = f;
+    m->fd = open(f.c_str(), O_RDONLY);
+    if (m->fd < 0) fail(SVM_ERR_NOT_FOUND, "cannot open '" + f + "'");
+    struct stat st;
+    fstat(m->fd, &st);
+    m->size = (uint64_t)st.st_size;
+    if (m->size) {
+      void* p = mmap(nullptr, m->size, PROT_READ, MAP_PRIVATE, m->fd, 0);
+      if (p == MAP_FAILED) fail(SVM_ERR_NO_MEMORY, "cannot map '" + f + "'");
+      m->data = (const char*)p;
+      for (const char* q = m->data; (q = (const char*)memchr(q, row_sep, m->data + m->size - q)) != nullptr; ++q)
+        m->rows++;
+    }
+    total += m->rows;
+    files.push_back(std::move(m));
+  }
+  hipStream_t s = opt.stream ? opt.stream : E.stream;
+  TablePtr out = E.store.create(info.id, oschema, total);  // zeroed: fields a row lacks read 0
+  out->max_tuple_count = E.cfg.tuple_per_bulk;
+  ld::ParseArgs a;
+  memset(&a, 0, sizeof(a));
+  a.col_sep = col_sep;
+  a.ncols = ncols;
+  for (int c = 0; c < ncols; ++c) {
+    a.type[c] = oschema[c].first;
+    a.size[c] = (int)oschema[c].second;
+    a.col[c] = out->cols[c].dev;
+  }
+  uint64_t row0 = 0;
+  for (auto& m : files) {
+    if (!m->size) continue;
+    DevBuf buf(m->size + 64);
+    SVM_HIP(hipMemcpyAsync(buf.p, m->data, m->size, hipMemcpyHostToDevice, s));
+    SVM_HIP(hipMemsetAsync((char*)buf.p + m->size, 0, 64, s));
+    const uint64_t nchunks = (m->size + ld::kChunk - 1) / ld::kChunk;
+    DevBuf counts(nchunks * 4 + 4), offs(nchunks * 8 + 16), ends(m->rows * 8 + 8);
+    ld::count_separators(buf.as<char>(), m->size, row_sep, counts.as<unsigned>(), nchunks, s);
+    k::exclusive_scan_u32(counts.as<uint32_t>(), nchunks, offs.as<unsigned long long>(),
+                          offs.as<unsigned long long>() + nchunks, s);
+    ld::separator_positions(buf.as<char>(), m->size, row_sep, offs.as<unsigned long long>(),
+                            ends.as<unsigned long long>(), nchunks, s);
+    a.buf = buf.as<char>();
+    a.n = m->size;
+    a.ends = ends.as<unsigned long long>();
+    a.rows = m->rows;
+    a.row0 = row0;
+    ld::parse_rows(a, s);
+    SVM_HIP(hipStreamSynchronize(s));  // before the mapping and the buffers go
+    row0 += m->rows;
+  }
+  auto res = std::make_unique<svm_result>();
+  res->kind = 0;
+  res->rows = (int64_t)total;
+  res->types = oschema;
+  if (opt.want_host) {
+    for (int c = 0; c < ncols; ++c) {
+      std::vector<uint8_t> h(total * (uint64_t)oschema[c].second);
+      if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out->cols[c].dev, h.size(), hipMemcpyDeviceToHost));
+      res->cols.push_back(std::move(h));
+    }
+  }
+  E.store.put(info.id, out);
+  res->scan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (E.cfg.verbose)
+    fprintf(stderr, "[svm] %s: loaded %llu rows from %zu file(s) in %.3f ms\n", info.id.c_str(),
+            (unsigned long long)total, files.size(), res->scan_ms);
+  return res;
+}
+
+}  // namespace svm

@@ -63,10 +63,21 @@ Plan decode_plan(const PipelineInfo& info) {
         p.has_source = true;
         const std::string* t = arg(op, "type");
         p.source_type = (t && !t->empty() && (*t)[0] == 'f') ? "file" : "object";
+        const std::string* path = arg(op, "path");
+        if (path) p.source_path = *path;
         p.source_info = op.output;
         break;
       }
-      case OpLoader: p.has_loader = true; break;
+      case OpLoader: {
+        // core/operator.cpp:13-21: the separators are the first characters of the args
+        p.has_loader = true;
+        const std::string* cs = arg(op, "col_spilter");
+        const std::string* rs = arg(op, "row_spilter");
+        if (cs) p.col_sep = *cs;
+        if (rs) p.row_sep = *rs;
+        p.loader_info = op.output;
+        break;
+      }
       case OpSink:
         p.has_sink = true;
         p.output_info = op.output;

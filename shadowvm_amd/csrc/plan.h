@@ -21,7 +21,9 @@ struct Plan {
   std::string id;
   bool has_source = false, has_sink = false, has_loader = false;
   std::string source_type;  // "object" | "file"
-  ObjectInfo source_info, output_info;
+  std::string source_path;  // Source{type: file, path}
+  std::string col_sep, row_sep;  // Loader{col_spilter, row_spilter}: their first characters are used
+  ObjectInfo source_info, output_info, loader_info;
   std::vector<std::string> filter_exp, select_exp, metric_exp;
   std::vector<KeyIndex> reduce_key, join_lkey, join_rkey;
   bool is_reduce = false;

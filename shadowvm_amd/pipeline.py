@@ -255,8 +255,9 @@ class PipelineTest:
         return self
 
     def plan(self, input_obj, ref_obj=()):
+        """The PipelineInfo dict; a Loader pipeline (file Source) takes input_obj None."""
         p = dict(self.pipe_info)
-        p["pre_pipeline_id"] = [input_obj] + list(ref_obj)
+        p["pre_pipeline_id"] = ([] if input_obj is None else [input_obj]) + list(ref_obj)
         return p
 
     def Run(self, input_obj, ref_obj=(), codegen=True, print_=False, isGPU=True, want_result=True):
