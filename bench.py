@@ -222,6 +222,50 @@ def serve_sweep(wl, rates, seconds, seed, workers_batched, workers_plain, p99_bo
     return res
 
 
+def bench_load(rows, seed, tmpdir):
+    """The Loader on SSB-style files: the single-join fact table (4 int32 columns,
+    dbgen's '|' layout with a trailing separator) written from the generator into
+    four files, loaded into HBM, checked equal to the generated table."""
+    import numpy as np
+    from shadowvm_amd import pipeline as P
+    from shadowvm_amd import ssb
+    wl = ssb.single_join(seed=seed, fact_rows=rows, prefix="bl")
+    spec = next(t for t in wl.tables if t.rows == rows)
+    types, sizes, cols = ssb.host_table(wl, spec)
+    d = os.path.join(tmpdir, "svm_load_bench")
+    os.makedirs(d, exist_ok=True)
+    nbytes, parts = 0, 4
+    for k in range(parts):
+        a, b = rows * k // parts, rows * (k + 1) // parts
+        block = np.stack([c[a:b] for c in cols], axis=1)
+        text = "".join("|".join(map(str, r)) + "|\n" for r in block.tolist())
+        with open(os.path.join(d, f"lineorder.tbl.{k}"), "w") as f:
+            f.write(text)
+        nbytes += len(text)
+    sch = P.create_schema(types, sizes)
+    pt = P.PipelineTest("bl_loaded")
+    pt.AddOperator(P.SOURCE, {"type": "file", "path": d}, sch)
+    pt.AddOperator(P.LOADER, {"col_spilter": "|", "row_spilter": "\n"}, sch)
+    pt.AddOperator(P.SINK, {}, sch)
+    pt.Run(None, want_result=False)  # warm: code paths, allocator
+    t0 = time.perf_counter()
+    r = pt.Run(None, want_result=False)
+    wall = time.perf_counter() - t0
+    res = P.run_plan(pt.plan(None), want_result=True)
+    exact = res.rows == rows and all(np.array_equal(np.asarray(g), np.asarray(w)) for g, w in zip(res.columns, cols))
+    for k in range(parts):
+        os.remove(os.path.join(d, f"lineorder.tbl.{k}"))
+    out = {"rows": rows, "bytes": nbytes, "files": parts, "wall_ms": wall * 1e3,
+           "rows_per_s": rows / wall, "gb_per_s": nbytes / wall / 1e9,
+           "host_map_count_ms": res.build_ms, "copy_ms": res.finish_ms, "parse_ms": res.scan_ms,
+           "parse_gb_per_s": nbytes / (res.scan_ms / 1e3) / 1e9 if res.scan_ms else None,
+           "equal_to_generated": exact}
+    log(f"[load] {rows} rows / {nbytes/1e6:.0f} MB text in {wall*1e3:.1f} ms ({out['gb_per_s']:.2f} GB/s): "
+        f"host {res.build_ms:.1f} ms, copy {res.finish_ms:.1f} ms, GPU parse {res.scan_ms:.2f} ms "
+        f"({out['parse_gb_per_s'] or 0:.0f} GB/s); equal to generated: {exact}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +282,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-p99-ms", type=float, default=50.0, help="latency bound of the capacity comparison")
     ap.add_argument("--serve-workers", default="2,4", help="worker streams: continuous batching, plain")
+    ap.add_argument("--load-rows", type=int, default=4_000_000, help="Loader benchmark size (0: skip)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-pipeline timing lines from libsvm")
@@ -297,6 +342,8 @@ def main():
                          "submission a distinct query; latency measured by the server from submission to "
                          "result; HBM admission budget 32 GB")
 
+    loaded = bench_load(args.load_rows, args.seed, "/tmp") if args.load_rows else None
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -333,6 +380,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "loaded": loaded,
     }
     print(json.dumps(out), flush=True)
 

@@ -121,6 +121,58 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
   }
 }
 
+// The aggregation of one passing row (group id, metrics, accumulate), emitted inside
+// `if (cond) {` with the row's f<c>, p<j> and JI<o> in scope.
+void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
+  const int M = (int)s.metrics.size();
+  // group id
+  if (s.agg != AGG_SCALAR) {
+    o << "        unsigned gid = 0u;\n";
+    for (size_t q = 0; q < s.keys.size(); ++q) {
+      const KeyShape& K = s.keys[q];
+      std::string stride = "(unsigned)a.iv[" + I(32 + 4 * (int)q + 1) + "]";
+      if (K.src == KEY_DIMCODE)
+        o << "        gid += (p" << K.join << " - 1u) * " << stride << ";\n";
+      else if (K.src == KEY_DIMROW)
+        o << "        gid += (unsigned)((const int*)a.ptr[" << SVM_SLOT_CODE + q << "])[JI" << K.obj
+          << "] * " << stride << ";\n";
+      else
+        o << "        gid += (unsigned)(f" << K.col << " - a.iv[" << 32 + 4 * q << "]) * " << stride
+          << ";\n";
+    }
+  }
+  for (int m = 0; m < M; ++m)
+    o << "        const svm_i64 mv" << m << " = (svm_i64)(" << s.metric_ctype[m] << ")(" << s.metrics[m]
+      << ");\n";
+  if (s.agg == AGG_SCALAR) {
+    for (int m = 0; m < M; ++m) o << "        acc" << m << " += mv" << m << ";\n";
+  } else if (s.agg == AGG_DENSE) {
+    for (int m = 0; m < M; ++m)
+      o << "        atomicAdd(&lsum[gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+    o << "        atomicOr(&lflag[gid >> 5], 1u << (gid & 31u));\n";
+  } else {
+    o << "        unsigned h = (gid * 2654435761u) >> (32 - SVM_HBITS);\n"
+      << "        int slot = -1;\n"
+      << "        for (int t = 0; t < " << s.hash_probes << "; ++t) {\n"
+      << "          unsigned k = hkey[h];\n"
+      << "          if (k == gid + 1u) { slot = (int)h; break; }\n"
+      << "          if (k == 0u) {\n"
+      << "            unsigned old = atomicCAS(&hkey[h], 0u, gid + 1u);\n"
+      << "            if (old == 0u || old == gid + 1u) { slot = (int)h; break; }\n"
+      << "          }\n"
+      << "          h = (h + 1u) & (SVM_HSLOTS - 1u);\n"
+      << "        }\n"
+      << "        if (slot >= 0) {\n";
+    for (int m = 0; m < M; ++m)
+      o << "          atomicAdd(&hsum[slot * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+    o << "        } else {\n";
+    for (int m = 0; m < M; ++m)
+      o << "          atomicAdd(&gsum[(size_t)gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+    o << "          atomicOr(&gflag[gid >> 5], 1u << (gid & 31u));\n"
+      << "        }\n";
+  }
+}
+
 }  // namespace
 
 std::string gen_scan_source(const KernelShape& s) {
@@ -186,59 +238,49 @@ std::string gen_scan_source(const KernelShape& s) {
     }
   }
   for (int u = 0; u < U; ++u) {
+    if (!s.flat) {
     o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
       << "      const long long i = v" << u << "_ * 4 + e;\n";
     for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
     o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
     emit_row_body_joins(o, s, "      ");
     o << "      if (cond) {\n";
-    // group id
-    if (s.agg != AGG_SCALAR) {
-      o << "        unsigned gid = 0u;\n";
-      for (size_t q = 0; q < s.keys.size(); ++q) {
-        const KeyShape& K = s.keys[q];
-        std::string stride = "(unsigned)a.iv[" + I(32 + 4 * (int)q + 1) + "]";
-        if (K.src == KEY_DIMCODE)
-          o << "        gid += (p" << K.join << " - 1u) * " << stride << ";\n";
-        else if (K.src == KEY_DIMROW)
-          o << "        gid += (unsigned)((const int*)a.ptr[" << SVM_SLOT_CODE + q << "])[JI" << K.obj
-            << "] * " << stride << ";\n";
-        else
-          o << "        gid += (unsigned)(f" << K.col << " - a.iv[" << 32 + 4 * q << "]) * " << stride
-            << ";\n";
-      }
-    }
-    for (int m = 0; m < M; ++m)
-      o << "        const svm_i64 mv" << m << " = (svm_i64)(" << s.metric_ctype[m] << ")(" << s.metrics[m]
-        << ");\n";
-    if (s.agg == AGG_SCALAR) {
-      for (int m = 0; m < M; ++m) o << "        acc" << m << " += mv" << m << ";\n";
-    } else if (s.agg == AGG_DENSE) {
-      for (int m = 0; m < M; ++m)
-        o << "        atomicAdd(&lsum[gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
-      o << "        atomicOr(&lflag[gid >> 5], 1u << (gid & 31u));\n";
-    } else {
-      o << "        unsigned h = (gid * 2654435761u) >> (32 - SVM_HBITS);\n"
-        << "        int slot = -1;\n"
-        << "        for (int t = 0; t < " << s.hash_probes << "; ++t) {\n"
-        << "          unsigned k = hkey[h];\n"
-        << "          if (k == gid + 1u) { slot = (int)h; break; }\n"
-        << "          if (k == 0u) {\n"
-        << "            unsigned old = atomicCAS(&hkey[h], 0u, gid + 1u);\n"
-        << "            if (old == 0u || old == gid + 1u) { slot = (int)h; break; }\n"
-        << "          }\n"
-        << "          h = (h + 1u) & (SVM_HSLOTS - 1u);\n"
-        << "        }\n"
-        << "        if (slot >= 0) {\n";
-      for (int m = 0; m < M; ++m)
-        o << "          atomicAdd(&hsum[slot * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
-      o << "        } else {\n";
-      for (int m = 0; m < M; ++m)
-        o << "          atomicAdd(&gsum[(size_t)gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
-      o << "          atomicOr(&gflag[gid >> 5], 1u << (gid & 31u));\n"
-        << "        }\n";
-    }
+    emit_aggregate(o, s);
     o << "      }\n    }\n";
+    } else {
+    // flat: the four rows' filters and map lookups are straight-line code (a row
+    // that needs no lookup reads entry 0), so their loads are in flight together;
+    // the passing rows are then aggregated one by one
+    o << "    bool cnd" << u << "[4];\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+      << "      const long long i = v" << u << "_ * 4 + e;\n";
+    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n"
+      << "      cond = cond && (" << s.filter << ");\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) {
+      const JoinShape& J = s.joins[j];
+      o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+        << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
+        << "        unsigned p = svm_map_get<" << J.wbits << ">(m" << j << ", in ? off : 0u);\n"
+        << "        p = in ? p : 0u;\n"
+        << "        cond = p != 0u;\n"
+        << "        pp" << j << "_" << u << "[e] = p; }\n";
+    }
+    o << "      cnd" << u << "[e] = cond;\n    }\n";
+    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+      << "      if (!cnd" << u << "[e]) continue;\n"
+      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
+    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) {
+      o << "      const unsigned p" << j << " = pp" << j << "_" << u << "[e];\n      (void)p" << j << ";\n";
+      if (s.joins[j].payload == PAY_ROW)
+        o << "      const long long JI" << s.joins[j].robj << " = (long long)p" << j << " - 1;\n";
+    }
+    o << "      {\n";
+    emit_aggregate(o, s);
+    o << "      }\n    }\n";
+    }
   }
   o << "  }\n";
   // flush

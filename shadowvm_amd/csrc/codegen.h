@@ -48,6 +48,7 @@ struct KernelShape {
   int hash_probes = 16;
   int unroll = 1;                        // int4 groups per thread per iteration
   bool prefetch = false;                 // load the next iteration's groups before processing
+  bool flat = false;                     // branch-free filter + joins for the four rows, then aggregate
   // select
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs

@@ -311,6 +311,7 @@ struct TuneChoice {
   int block = 256, unroll = 1, bpc = 0;  // bpc 0: as many as LDS allows (<= 8)
   bool stage = true;                      // small join maps copied into LDS
   bool prefetch = false;                  // software-pipelined column loads
+  bool flat = false;                      // branch-free filter + joins for a lane's four rows
 };
 struct TuneState {
   bool done = false;
@@ -320,9 +321,11 @@ struct TuneState {
 std::mutex g_tune_mu;
 std::map<uint64_t, TuneState> g_tune;
 const TuneChoice kTuneCandidates[] = {
-    {256, 1, 0, true, false},   {256, 2, 0, true, false},  {512, 1, 0, true, false},
-    {512, 1, 0, false, false},  {1024, 1, 0, false, false}, {512, 1, 0, true, true},
-    {512, 1, 0, false, true},   {1024, 1, 0, false, true}};
+    {256, 1, 0, true, false, false},  {256, 2, 0, true, false, false},  {512, 1, 0, true, false, false},
+    {512, 1, 0, false, false, false}, {1024, 1, 0, false, false, false}, {512, 1, 0, true, true, false},
+    {512, 1, 0, false, true, false},  {512, 2, 0, true, false, false},  {1024, 1, 0, true, false, false},
+    {256, 1, 0, true, false, true},   {512, 1, 0, true, false, true},   {512, 2, 0, true, false, true},
+    {1024, 1, 0, true, false, true},  {512, 1, 0, false, false, true}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -479,6 +482,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2.block = t.block;
     k2.unroll = t.unroll;
     k2.prefetch = t.prefetch;
+    k2.flat = t.flat;
     for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
     words = t.stage ? lds_words : agg_words;
   };
@@ -510,13 +514,15 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
       if (c.E.cfg.verbose) {
         fprintf(stderr, "[svm] autotune %s rows=%llu:", c.info.id.c_str(), (unsigned long long)rows);
         for (auto& t : st.trials)
-          fprintf(stderr, " (%d,%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc,
-                  (int)t.first.stage, (int)t.first.prefetch, t.second);
+          fprintf(stderr, " (%d,%d,%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc,
+                  (int)t.first.stage, (int)t.first.prefetch, (int)t.first.flat, t.second);
         fprintf(stderr, "\n");
       }
     }
     choice = st.best;
   }
+  // SVM_SCAN_FLAT=1 / =0 forces the flat (branch-free join) form on or off
+  if (const char* fl = getenv("SVM_SCAN_FLAT")) choice.flat = fl[0] == '1';
   uint64_t run_words = lds_words;
   {
     KernelShape k2;
@@ -524,7 +530,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     ks = k2;
   }
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
-              (choice.prefetch ? " prefetch" : "");
+              (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
   hipFunction_t fn = nullptr;
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
   const double t_launch_host = host_ms(t_host0);

@@ -90,6 +90,13 @@ std::string compile(const std::string& src, uint64_t h) {
 
 std::string jit_code_object(const std::string& src) {
   uint64_t h = fnv1a(src);
+  // SVM_JIT_DUMP=<dir>: keep every generated source (for offline inspection)
+  const char* dump = getenv("SVM_JIT_DUMP");
+  if (dump && *dump) {
+    mkdir(dump, 0777);
+    std::ofstream f(std::string(dump) + "/" + std::to_string(h) + ".hip", std::ios::binary);
+    f << src;
+  }
   std::string path = cache_dir() + "/" + std::to_string(h) + ".co";
   std::string code = read_file(path);
   if (code.empty()) code = compile(src, h);

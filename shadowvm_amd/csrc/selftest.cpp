@@ -58,6 +58,10 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.prefetch = true;
     s.block = 512;
     out.push_back({"scan_hash_prefetch", gen_scan_source(s)});
+    s.prefetch = false;
+    s.unroll = 2;
+    s.flat = true;
+    out.push_back({"scan_hash_flat", gen_scan_source(s)});
   }
   {  // row-id payload, dimension attribute in the metric, source-column key
     KernelShape s;
@@ -72,6 +76,10 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metric_ctype = {"long long"};
     s.agg = AGG_HASH;
     out.push_back({"scan_row", gen_scan_source(s)});
+    s.flat = true;
+    s.filter = "StrCmp(((const char*)(FCOL2+(long long)i*10)),PS(0),PI(0))!=PI(1)";
+    s.fact_other = {2};
+    out.push_back({"scan_row_flat", gen_scan_source(s)});
   }
   {  // select with a string filter and projections, plus a row-id join
     KernelShape s;

@@ -49,8 +49,21 @@ def test_three_way_group(engine):
     assert 100 < got.rows <= 7 * 40
 
 
-def test_four_way_zipf_strings(engine):
+@pytest.mark.parametrize("flat", ["0", "1"])
+def test_four_way_zipf_strings(engine, monkeypatch, flat):
+    # both code forms of the scan: joins under per-row branches, and the flat form
+    # (the four rows' filters and map lookups straight-line, then aggregation)
+    monkeypatch.setenv("SVM_SCAN_FLAT", flat)
     wl = ssb.four_way(seed=7, fact_rows=3_000_000, n_cust=300_000, n_supp=20_000, n_part=140_000,
                       prefix="fw7")
+    got, _ = _run_both(engine, wl)
+    assert got.rows > 0
+
+
+@pytest.mark.parametrize("flat", ["0", "1"])
+def test_three_way_forms(engine, monkeypatch, flat):
+    # the three-way group-by in both code forms
+    monkeypatch.setenv("SVM_SCAN_FLAT", flat)
+    wl = ssb.three_way(seed=11, fact_rows=2_000_000, n_part=100_000, n_supp=10_000, prefix=f"tf{flat}")
     got, _ = _run_both(engine, wl)
     assert got.rows > 0
