@@ -1,10 +1,11 @@
 /*
- * svm_debug.h — inspection entry points of libsvm.so that need no GPU.
+ * svm_debug.h — inspection entry points of libsvm.so (the first two need no GPU).
  *
  * These have no counterpart in the reference; they expose what the reference
  * only prints (the decoded plan, core/pipeline.cu:11-80, and the generated code,
- * test/test.hpp:106-114) so the boundary and the code generator can be checked
- * on a machine without a GPU.
+ * test/test.hpp:106-114), so the boundary and the code generator can be checked
+ * on a machine without a GPU, and what it does silently (the inserts its join hash
+ * tables drop).
  */
 #ifndef SVM_DEBUG_H_
 #define SVM_DEBUG_H_
@@ -30,6 +31,13 @@ int svm_debug_jit_selftest(char* buf, size_t cap);
  * rows and width) with a plain 128-bit streaming copy kernel (svm_calib_copy).
  * A kernel of known traffic (read n, write n bytes) to calibrate HBM counters. */
 int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, int src_col);
+
+/* Inserts the reference's join hash table would drop (BuildHashTable,
+ * com/cpp/com.hpp:168-174: Hash of the key bytes % slots, at most 10 linear probes,
+ * silent drop) over every row of Int32 column `col` of table `id`, inserted in row
+ * order into `slots` slots (0: HashRate x the table's bulk capacity).  Computed on
+ * the GPU, as the joins do. */
+int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* dropped);
 
 #ifdef __cplusplus
 }

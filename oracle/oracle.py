@@ -65,14 +65,18 @@ class OracleError(Exception):
 # --------------------------------------------------------------------------- tables
 @dataclass
 class OTable:
-    """A columnar table on the host: one bulk for dimensions, ``tuple_per_bulk``
-    bulks for the fact side (bulking is done by the driver)."""
+    """A columnar table on the host.  As a source it is cut into bulks of the
+    driver's ``tuple_per_bulk`` rows; as a ref object only its bulk 0 (the first
+    ``bulk0_rows`` rows) is read."""
 
     types: list
     sizes: list
     columns: list  # np.ndarray per column, or None for an unmaterialised column
     max_tuple_count: int = 3_000_000
     rows: int = field(default=-1)
+    # rows of bulk 0, the only bulk a join reads (core/pipeline.cu:91); -1: the table
+    # is cut into bulks of max_tuple_count rows
+    bulk0_rows: int = field(default=-1)
 
     def __post_init__(self):
         if self.rows < 0:
@@ -81,6 +85,9 @@ class OTable:
                 if c is not None:
                     self.rows = int(c.shape[0])
                     break
+        if self.bulk0_rows < 0:
+            self.bulk0_rows = min(self.rows, self.max_tuple_count) if self.max_tuple_count else self.rows
+        self.bulk0_rows = min(self.bulk0_rows, self.rows)
         self._bufs = []
         for t, s, c in zip(self.types, self.sizes, self.columns):
             if c is None:
@@ -348,7 +355,8 @@ class _Ctx(ctypes.Structure):
 
 class _Res(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("ncols", ctypes.c_int32), ("rows", ctypes.c_uint64),
-                ("col", ctypes.c_void_p * MAX_COL), ("width", ctypes.c_int64 * MAX_COL)]
+                ("col", ctypes.c_void_p * MAX_COL), ("width", ctypes.c_int64 * MAX_COL),
+                ("bulk0_rows", ctypes.c_uint64)]
 
 
 _lock = threading.Lock()
@@ -415,7 +423,7 @@ def run_pipeline(plan: dict, source: OTable, refs=(), tuple_per_bulk=3_000_000, 
     ctx.tuple_per_bulk = tuple_per_bulk
     for o, t in enumerate([source] + refs):
         ob = ctx.obj[o]
-        ob.rows = t.rows
+        ob.rows = t.rows if o == 0 else t.bulk0_rows  # a ref object is read through bulk 0 only
         ob.max_tuple_ct = t.max_tuple_count
         ob.ncols = len(t.types)
         for c in range(len(t.types)):
@@ -454,7 +462,8 @@ def run_pipeline(plan: dict, source: OTable, refs=(), tuple_per_bulk=3_000_000, 
             out = _cols(d.output_info)
             tcols = out_cols + [None] * (len(out) - len(out_cols))
             result.table = OTable([t for t, _ in out], [s for _, s in out], tcols,
-                                  max_tuple_count=tuple_per_bulk, rows=int(res.rows))
+                                  max_tuple_count=tuple_per_bulk, rows=int(res.rows),
+                                  bulk0_rows=int(res.bulk0_rows))
         return result
     finally:
         lib.oracle_free(ctypes.byref(ctx), ctypes.byref(res))

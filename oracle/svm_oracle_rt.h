@@ -179,6 +179,7 @@ struct ORT_Result {
   uint64 rows;
   char* col[ORT_MAX_COL];
   int64_t width[ORT_MAX_COL];
+  uint64 bulk0_rows;  /* select: rows of sink bulk 0 = what source bulk 0 produced */
 };
 
 /* Per-bulk output written by the generated task. */
@@ -201,8 +202,10 @@ static inline double ort_ms_since(std::chrono::steady_clock::time_point t0) {
 template <typename T>
 inline void ort_build_join(ORT_Ctx* c, int o, int col) {
   ORT_Obj& ob = c->obj[o];
-  uint64 ht_size = HashRate * ob.max_tuple_ct;
-  uint64 rows = ob.rows < ob.max_tuple_ct ? ob.rows : ob.max_tuple_ct;
+  /* BuildHashTable takes unsigned sizes (com/cpp/com.hpp:169); ob.rows of a ref
+   * object is already its bulk 0 tuple count */
+  uint64 ht_size = (unsigned)(HashRate * ob.max_tuple_ct);
+  uint64 rows = ob.rows;
   if (c->ht[o] == nullptr) c->ht[o] = (int*)malloc(sizeof(int) * ht_size);
   c->ht_size[o] = ht_size;
   c->dropped[o] = BuildHashTable<T>((const T*)ob.col[col], (unsigned)rows,
@@ -310,6 +313,8 @@ static inline int ort_drive(ORT_Ctx* c, ORT_Result* res, ORT_TaskFn task, int ki
     for (uint64 b = 0; b < nbulk; ++b) total += sel_rows[b];
     res->ncols = nout;
     res->rows = total;
+    /* sink bulk b holds what source bulk b produced (core/pipeline.cu:161-162,294-296) */
+    res->bulk0_rows = nbulk ? sel_rows[0] : 0;
     for (int q = 0; q < nout; ++q) {
       res->width[q] = out_width[q];
       res->col[q] = (char*)malloc(total * (uint64)out_width[q] + 1);

@@ -71,6 +71,7 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "svm_server_stats": (ctypes.c_int, [c_u64p, c_u64p, c_u64p, c_u64p]),
     "svm_server_stop": (ctypes.c_int, []),
+    "svm_debug_hash_drops": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, c_u64p]),
 }
 
 _lib = None

@@ -486,7 +486,15 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
     jl.kcol = d.proj[R.col];
     jl.fcol = L.col;
     if (d.base->cols[jl.kcol].type != Int32) return f;
-    if (d.base->max_tuple_count && d.base->rows > d.base->max_tuple_count) return f;
+    // the absorbed dimension object is the filtered base; it is all in bulk 0 only
+    // when the base is (a join reads bulk 0 alone, core/pipeline.cu:91)
+    if (d.base->bulk0_rows() < d.base->rows) return f;
+    // When the base's own hash table would drop inserts, a query's dimension object (an
+    // in-order subset of the base's rows) may drop some too: the pass checks each query
+    // and leaves those that drop to the plain path.  Clusters wider than the check
+    // replays keep the query on the plain path from the start.
+    if (auto hc = table_drop_clusters(d.base, jl.kcol, pipeline_ht_slots(E), s))
+      if (hc->max_span > (uint32_t)bk::kCheckSpan) return f;
     if (!slots.insert(std::make_tuple(d.base.get(), jl.kcol, jl.fcol)).second) return f;
     jl.atoms = d.atoms;
     std::set<int> attrs;
@@ -533,7 +541,8 @@ struct PassLayout {
   static constexpr size_t o_row = o_qd + al(bk::kMaxBatch * sizeof(bk::QDesc));
   static constexpr size_t o_terms = o_row + al(bk::kMaxBatch * 16);
   static constexpr size_t o_emit = o_terms + al(bk::kMaxTerms * sizeof(bk::MaskTerm));
-  static constexpr size_t total = o_emit + al(bk::kMaxBatch * sizeof(bk::Emit));
+  static constexpr size_t o_chk = o_emit + al(bk::kMaxBatch * sizeof(bk::Emit));
+  static constexpr size_t total = o_chk + al(bk::kMaxSlot * sizeof(bk::DropCheck));
 };
 // Emission blocks of one pass: 256 flag words per block, G <= 2^22 per query.
 constexpr uint64_t kMaxEmitBlocks = (uint64_t)bk::kMaxBatch * (((1u << 22) / 32 + 255) / 256);
@@ -545,7 +554,8 @@ struct Scratch {
   std::vector<DevBuf> bufs;
   void* host = nullptr;  // pinned staging for descriptors
   size_t host_bytes = 0;
-  std::map<std::tuple<int, unsigned, int>, hipGraphExec_t> graphs;  // (variant, LDS profiles, grid)
+  // (variant, LDS profiles, grid, drop check)
+  std::map<std::tuple<int, unsigned, int, bool>, hipGraphExec_t> graphs;
   void* get(size_t slot, uint64_t bytes) {
     if (bufs.size() <= slot) bufs.resize(slot + 1);
     if (bufs[slot].bytes < bytes) bufs[slot] = DevBuf(bytes + bytes / 4 + 256);
@@ -678,7 +688,7 @@ bool env_off(const char* name) {
 
 static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
                      std::vector<std::unique_ptr<svm_result>*>& outs, Engine& E, hipStream_t s, Scratch& sc,
-                     bool tune) {
+                     bool tune, std::vector<int>& redo) {
   bk::prepare_shared_scan();
   TablePtr base = qs[P.qi[0]]->base;
   const int nq = (int)P.qi.size();
@@ -850,6 +860,33 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   }
   M->terms = (const bk::MaskTerm*)(dd + PassLayout::o_terms);
   M->nslot = (int)P.slots.size();
+  // slots whose dimension table drops inserts in the reference's hash table
+  auto* chk = (bk::DropCheck*)(host + PassLayout::o_chk);
+  std::vector<std::shared_ptr<const k::HtClusters>> held;
+  int nchk = 0;
+  for (size_t k = 0; k < P.slots.size(); ++k) {
+    auto hc = table_drop_clusters(P.slots[k].dim, P.slots[k].kcol, pipeline_ht_slots(E), s);
+    if (!hc) continue;
+    if (hc->max_span > (uint32_t)bk::kCheckSpan) fail(SVM_ERR_INVALID, "drop check wider than its replay window");
+    held.push_back(hc);
+    const bk::Slot& S = A->slot[k];
+    bk::DropCheck& D = chk[nchk++];
+    D.start = hc->start.as<unsigned>();
+    D.key = hc->key.as<int>();
+    D.home = hc->home.as<unsigned>();
+    D.clusters = hc->clusters;
+    D.kmin = S.kmin;
+    D.range = S.range;
+    D.P = S.P;
+    D.pkey = S.pkey;
+    D.narrow = S.narrow;
+    D.mask = S.mask;
+    D.joined = S.joined;
+  }
+  unsigned long long* dropq = (unsigned long long*)sc.get(11, 16);
+  A->chk = nchk ? (const bk::DropCheck*)(dd + PassLayout::o_chk) : nullptr;
+  A->nchk = nchk;
+  A->dropq = dropq;
   A->nprof = nprof;
   A->qd = (const bk::QDesc*)(dd + PassLayout::o_qd);
   A->qrow = (const long long*)(dd + PassLayout::o_row);
@@ -940,12 +977,13 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   SVM_HIP(hipEventRecord(e0, s));
   const bool graph = !env_off("SVM_BATCH_GRAPH");
   if (graph) {
-    hipGraphExec_t& ge = sc.graphs[std::make_tuple(ch.variant, cap, grid)];
+    hipGraphExec_t& ge = sc.graphs[std::make_tuple(ch.variant, cap, grid, nchk > 0)];
     if (!ge) {
       hipGraph_t g = nullptr;
       SVM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       bk::launch_pass_clear(d_args, s);
       bk::launch_profile_masks(d_margs, s);
+      if (nchk) bk::launch_drop_check(d_args, s);
       bk::launch_slot_bits(d_args, s);
       bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
       bk::launch_emit_count(d_emit, d_args, counts, offsets, totals, s);
@@ -957,6 +995,7 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   } else {
     bk::launch_pass_clear(d_args, s);
     bk::launch_profile_masks(d_margs, s);
+    if (nchk) bk::launch_drop_check(d_args, s);
     bk::launch_slot_bits(d_args, s);
     SVM_HIP(hipEventRecord(eb, s));
     bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
@@ -964,7 +1003,9 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     bk::launch_emit_count(d_emit, d_args, counts, offsets, totals, s);
   }
   std::vector<unsigned long long> tot(nq);
+  unsigned long long dq = 0;
   SVM_HIP(hipMemcpyAsync(tot.data(), totals, nq * 8, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipMemcpyAsync(&dq, dropq, 8, hipMemcpyDeviceToHost, s));
   SVM_HIP(hipStreamSynchronize(s));
   // emission of rows (sized by the counts just read)
   uint64_t rows = 0;
@@ -997,10 +1038,12 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       snprintf(phases, sizeof(phases), ", masks %.3f, scan %.3f", mb, msc);
     }
     fprintf(stderr, "[svm] shared scan of %d queries: gpu %.3f ms%s (%s; %u profiles in %zu slots; %llu rows / %llu "
-                    "slice rows; %d threads, %d LDS slots, %s lookups, key bits %d, tile %lld, %u tiles, grid %d)\n",
+                    "slice rows; %d threads, %d LDS slots, %s lookups, key bits %d, tile %lld, %u tiles, grid %d; "
+                    "%d slots checked for dropped inserts, %d queries drop)\n",
             nq, ms_all, phases, graph ? "graph" : "eager", nprof, P.slots.size(), (unsigned long long)union_rows,
             (unsigned long long)slice_rows, bk::scan_variant_block(ch.variant), bk::scan_variant_hash_slots(ch.variant),
-            bk::scan_variant_parallel(ch.variant) ? "parallel" : "gated", ch.bits, (long long)ch.tile, A->ntiles, grid);
+            bk::scan_variant_parallel(ch.variant) ? "parallel" : "gated", ch.bits, (long long)ch.tile, A->ntiles, grid,
+            nchk, __builtin_popcountll(dq));
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1008,6 +1051,10 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   (void)hipEventDestroy(es);
   for (int q = 0; q < nq; ++q) {
     const int i = P.qi[q];
+    if ((dq >> q) & 1ull) {
+      redo.push_back(i);  // its reference hash table drops an insert: the plain path is exact
+      continue;
+    }
     auto r = std::make_unique<svm_result>();
     const int K = qs[i]->K;
     r->kind = K == 0 ? 2 : 1;
@@ -1029,9 +1076,11 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
 
 // Fused execution of lowered queries that share one base fact table: greedy
 // passes within the limits of one shared scan; a pass whose profiles exceed the
-// LDS staging budget is split in two.
+// LDS staging budget is split in two.  Queries whose dimension objects the
+// reference's hash table would drop rows of get no result here: their indices (into
+// qs) are appended to `redo` for the plain path.
 static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr<svm_result>*>& outs,
-                      Engine& E, hipStream_t s, Scratch& sc, bool tune) {
+                      Engine& E, hipStream_t s, Scratch& sc, bool tune, std::vector<int>& redo) {
   if (qs.empty()) return;
   std::vector<PassB> passes;
   PassB cur;
@@ -1057,7 +1106,7 @@ static void run_fused(std::vector<FactLowered*>& qs, std::vector<std::unique_ptr
       passes.push_back(pass_of(qs, std::vector<int>(P.qi.begin() + h, P.qi.end()), s));
       continue;
     }
-    run_pass(qs, P, outs, E, s, sc, tune);
+    run_pass(qs, P, outs, E, s, sc, tune, redo);
   }
 }
 
@@ -1104,6 +1153,8 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
       for (auto& id : facts[i].deferred)
         if (!absorbed.count(id)) fused[i] = false;  // its dimension is materialised instead
   std::vector<std::unique_ptr<svm_result>> res(n);
+  std::map<std::string, int> job_of;
+  for (int i = 0; i < n; ++i) job_of[jobs[i].info.id] = i;
   const double t_lowered = host_ms();
   // plain plans in submission order (absorbed dimension pipelines are skipped)
   for (int i = 0; i < n; ++i) {
@@ -1125,7 +1176,21 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
       fo.push_back(&res[i]);
     }
     static Scratch* scratch = new Scratch();  // reused by every batch (guarded by run_mu)
-    run_fused(fq, fo, E, s, *scratch, true);
+    std::vector<int> redo;
+    run_fused(fq, fo, E, s, *scratch, true, redo);
+    for (int pos : redo) {
+      // its dimension pipelines, then the query, on the plain path; the dimension
+      // objects stay unmaterialised afterwards, like every absorbed one
+      const int i = gq.second[pos];
+      RunOptions dopt;
+      dopt.want_host = false;
+      for (auto& id : facts[i].deferred) run_pipeline(jobs[job_of.at(id)].info, dopt);
+      RunOptions opt;
+      opt.want_host = outs != nullptr;
+      res[i] = run_pipeline(jobs[i].info, opt);
+      for (auto& id : facts[i].deferred) E.store.drop(id);
+      fused[i] = false;
+    }
   }
   int nfused = 0;
   for (int i = 0; i < n; ++i) nfused += fused[i] ? 1 : 0;
@@ -1232,7 +1297,9 @@ void worker_main(Server* S, int index) {
         }
         // shapes not tuned by an earlier batch_run take the default launch shape:
         // a serving batch does not stall on tuning trials
-        run_fused(fq, fo, E, s, sc, false);
+        std::vector<int> redo;
+        run_fused(fq, fo, E, s, sc, false, redo);
+        for (int pos : redo) run_plain_query(*batch[pos], s);
       }
     } catch (const Error& e) {
       for (auto& t : batch) { t->status = e.code; t->error = e.what(); }

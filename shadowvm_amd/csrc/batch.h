@@ -103,6 +103,25 @@ struct FactPred {
   int lo, hi;
 };
 
+// A slot whose dimension table drops inserts in the reference's hash table
+// (core/pipeline.cu:88-133): the clusters of its inserts that drop something.  A
+// query's dimension object is an in-order subset of the table's rows, so it can drop
+// a row only inside one of these clusters, and only where it keeps at least 11 of
+// the cluster's rows (a drop needs 10 slots taken by earlier rows of the cluster).
+struct DropCheck {
+  const unsigned* start;           // cluster c holds entries [start[c], start[c+1])
+  const int* key;                  // key of each entry (row order inside a cluster)
+  const unsigned* home;            // home slot of each entry
+  unsigned clusters;
+  int kmin;
+  unsigned range, P;
+  const void* pkey;                // profile per key offset (uint8 when narrow, else uint16)
+  int narrow;
+  const unsigned long long* mask;  // query mask of each profile < P
+  unsigned long long joined;       // queries joining through this slot
+};
+constexpr int kCheckSpan = 256;  // widest cluster (in slots) a check replays
+
 struct ScanArgs {
   const void* cols[kMaxCols];
   unsigned qcols[kMaxBatch];    // fact columns each query reads
@@ -123,6 +142,9 @@ struct ScanArgs {
   long long row_hi;
   unsigned ntiles;
   unsigned* counter;
+  const DropCheck* chk;         // slots to check for dropped inserts
+  int nchk;
+  unsigned long long* dropq;    // bit q: query q's dimension object drops an insert
 };
 
 // Per-query group emission.
@@ -157,6 +179,9 @@ void launch_string_exact(const char* col, int size, uint64_t n, const int32_t* c
 void launch_pass_clear(const ScanArgs* d_args, hipStream_t s);
 void launch_profile_masks(const MaskArgs* d_margs, hipStream_t s);
 void launch_slot_bits(const ScanArgs* d_args, hipStream_t s);
+// Sets dropq bit q for every query whose dimension object on a checked slot drops an
+// insert (replays the insert order of the clusters where the query keeps >= 11 rows).
+void launch_drop_check(const ScanArgs* d_args, hipStream_t s);
 // Shared-scan variants (workgroup size x LDS accumulator size) the autotuner picks from.
 int scan_variants();
 int scan_variant_block(int v);

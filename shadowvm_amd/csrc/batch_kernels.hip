@@ -470,7 +470,66 @@ __global__ void k_pass_clear(const ScanArgs* __restrict__ A) {
   unsigned* flags = A->flags;
   for (uint64_t i = t0; i < A->sum_words; i += stride) sums[i] = 0ull;
   for (uint64_t i = t0; i < A->flag_words; i += stride) flags[i] = 0u;
-  if (t0 == 0) *A->counter = 0u;
+  if (t0 == 0) {
+    *A->counter = 0u;
+    if (A->dropq) *A->dropq = 0ull;
+  }
+}
+
+// Queries (of `joined`) that keep the dimension row of entry e of a checked slot.
+__device__ __forceinline__ unsigned long long check_member(const DropCheck& D, unsigned e) {
+  const unsigned off = (unsigned)(D.key[e] - D.kmin);
+  if (off >= D.range) return 0ull;
+  const unsigned p = D.narrow ? ((const unsigned char*)D.pkey)[off] : ((const unsigned short*)D.pkey)[off];
+  return p < D.P ? D.mask[p] & D.joined : 0ull;
+}
+
+__global__ void k_drop_check(const ScanArgs* __restrict__ A) {
+  const int nchk = A->nchk;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (int k = 0; k < nchk; ++k) {
+    const DropCheck& D = A->chk[k];
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < D.clusters; c += stride) {
+      const unsigned e0 = D.start[c], e1 = D.start[c + 1];
+      // members per query as bit-sliced 4-bit counters (c4: 16 or more)
+      unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+      unsigned base = 0xFFFFFFFFu;
+      for (unsigned e = e0; e < e1; ++e) {
+        base = min(base, D.home[e]);
+        unsigned long long carry = check_member(D, e), t;
+        t = c0 & carry; c0 ^= carry; carry = t;
+        t = c1 & carry; c1 ^= carry; carry = t;
+        t = c2 & carry; c2 ^= carry; carry = t;
+        t = c3 & carry; c3 ^= carry; carry = t;
+        c4 |= carry;
+      }
+      unsigned long long cand = c4 | (c3 & (c2 | (c1 & c0)));  // 11 or more
+      while (cand) {
+        const int q = __ffsll((long long)cand) - 1;
+        cand &= cand - 1ull;
+        // the query's inserts in row order; the cluster spans at most kCheckSpan
+        // slots from `base` (checked when the slot is set up), so no position wraps
+        unsigned long long occ[kCheckSpan / 64] = {};
+        for (unsigned e = e0; e < e1; ++e) {
+          if (!((check_member(D, e) >> q) & 1ull)) continue;
+          unsigned pos = D.home[e] - base;
+          bool placed = false;
+          for (int t = 0; t < 10; ++t, ++pos) {
+            const unsigned long long bit = 1ull << (pos & 63u);
+            if (!(occ[pos >> 6] & bit)) {
+              occ[pos >> 6] |= bit;
+              placed = true;
+              break;
+            }
+          }
+          if (!placed) {
+            atomicOr(A->dropq, 1ull << q);
+            break;
+          }
+        }
+      }
+    }
+  }
 }
 
 __global__ void k_emit_count(const Emit* __restrict__ e, const ScanArgs* __restrict__ A,
@@ -600,6 +659,11 @@ void launch_pass_clear(const ScanArgs* d_args, hipStream_t s) {
 void launch_profile_masks(const MaskArgs* d_margs, hipStream_t s) {
   hipLaunchKernelGGL(k_profile_masks, dim3(kMaxProfiles / kBlock, kMaxSlot), dim3(kBlock), 0, s, d_margs);
   hip_check(hipGetLastError(), "k_profile_masks");
+}
+
+void launch_drop_check(const ScanArgs* d_args, hipStream_t s) {
+  hipLaunchKernelGGL(k_drop_check, dim3(kClearGrid), dim3(kBlock), 0, s, d_args);
+  hip_check(hipGetLastError(), "k_drop_check");
 }
 
 void launch_slot_bits(const ScanArgs* d_args, hipStream_t s) {

@@ -3,6 +3,7 @@
 
 #include "exec.h"
 #include "kernels.h"
+#include "stats.h"
 #include "svm_internal.h"
 
 namespace svm {
@@ -277,6 +278,21 @@ int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, i
       fail(SVM_ERR_INVALID, "destination column must be an owned column of the same byte size");
     k::calib_copy(s->col_ptr(src_col), d->cols[dst_col].dev, bytes & ~15ull, E.stream);
     SVM_HIP(hipStreamSynchronize(E.stream));
+  });
+}
+
+// Debug: inserts the reference's join hash table drops over every row of an Int32
+// column inserted in row order (slots = 0: HashRate x the table's bulk capacity).
+int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* dropped) {
+  return guard([&] {
+    need_id(id);
+    if (!dropped) fail(SVM_ERR_INVALID, "dropped is NULL");
+    Engine& E = engine();
+    TablePtr t = E.store.get(id);
+    if (col < 0 || col >= (int)t->cols.size() || t->cols[col].type != Int32 || !t->cols[col].dev)
+      fail(SVM_ERR_INVALID, "not a materialised Int32 column");
+    if (!slots) slots = (uint64_t)(uint32_t)(kHashRate * (t->max_tuple_count ? t->max_tuple_count : t->rows));
+    *dropped = k::ht_kept((const int32_t*)t->col_ptr(col), t->rows, slots, nullptr, E.stream);
   });
 }
 

@@ -67,6 +67,7 @@ struct JoinBuild {
   uint64_t words = 0;
   DevBuf packed;
   uint32_t lds_off = 0;
+  uint64_t dropped = 0;  // inserts the reference's hash table drops
 };
 
 struct KeyBuild {
@@ -157,8 +158,8 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     if (c.schemas[0][L.col].first != Int32 || c.schemas[R.obj][R.col].first != Int32)
       fail(SVM_ERR_UNSUPPORTED, "join keys must be Int32 columns");
     const Table& D = *c.refs[R.obj - 1];
-    if (D.max_tuple_count && D.rows > D.max_tuple_count)
-      fail(SVM_ERR_UNSUPPORTED, "dimension '" + D.id + "' exceeds one bulk (only bulk 0 is joined, core/pipeline.cu:91)");
+    // only bulk 0 of the dimension is joined (core/pipeline.cu:91,294-295)
+    const uint64_t n0 = D.bulk0_rows();
     JoinBuild jb;
     jb.rcol = R.col;
     jb.shape.robj = R.obj;
@@ -186,7 +187,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     // dimension object's keys, cached per resident table: no host round trip)
     const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
-    if (D.rows) {
+    if (n0) {
       auto root = column_root(Dp, R.col);
       KeyEntry ke = table_key(root.first, root.second, c.s);
       uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
@@ -199,8 +200,14 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     jb.words = ((uint64_t)jb.range + per - 1) / per;
     jb.packed = DevBuf(jb.words * 4 + 16, c.s);
     if (jb.range) {
+      // the reference's table drops inserts that find no free slot in 10 probes; a
+      // dropped row is never found, so the map takes the first kept row of each key
+      const uint8_t* kept = join_kept(Dp, R.col, &jb.dropped, c.s);
       DevBuf first((uint64_t)jb.range * 4, c.s);
-      k::first_rows((const int32_t*)kcol, D.rows, jb.kmin, jb.range, first.as<int32_t>(), c.s);
+      k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s, kept);
+      if (jb.dropped && c.E.cfg.verbose)
+        fprintf(stderr, "[svm] %s: the hash table on '%s' drops %llu of %llu inserts\n", c.info.id.c_str(),
+                D.id.c_str(), (unsigned long long)jb.dropped, (unsigned long long)n0);
       k::pack_map(first.as<int32_t>(), jb.range, payload, codes_for_pack.as<int32_t>(), jb.shape.wbits,
                   jb.packed.as<uint32_t>(), c.s);
       // `first` and the codes are released in stream order after pack_map
@@ -325,7 +332,10 @@ const TuneChoice kTuneCandidates[] = {
     {512, 1, 0, false, false, false}, {1024, 1, 0, false, false, false}, {512, 1, 0, true, true, false},
     {512, 1, 0, false, true, false},  {512, 2, 0, true, false, false},  {1024, 1, 0, true, false, false},
     {256, 1, 0, true, false, true},   {512, 1, 0, true, false, true},   {512, 2, 0, true, false, true},
-    {1024, 1, 0, true, false, true},  {512, 1, 0, false, false, true}};
+    {1024, 1, 0, true, false, true},  {512, 1, 0, false, false, true},
+    // two workgroups per CU: a plain read of several columns streams fastest with
+    // few waves, a scan with dependent map lookups needs more to hide them
+    {512, 1, 2, true, false, false},  {512, 2, 2, true, false, false}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -678,6 +688,28 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     SVM_HIP(hipMemcpyAsync(&total, offs.as<unsigned long long>() + tiles, 8, hipMemcpyDeviceToHost, c.s));
     SVM_HIP(hipStreamSynchronize(c.s));
   }
+  // Sink bulk i holds what source bulk i produced (core/pipeline.cu:161-162,294-296), so
+  // the output's bulk 0 -- all a later join reads of it -- is the passing rows of the
+  // source's bulk 0.
+  uint64_t out_bulk0 = total;
+  const uint64_t src_bulk0 = c.src->bulk0_rows();
+  if (tiles && src_bulk0 < rows) {
+    const uint64_t tiles0 = (src_bulk0 + tile - 1) / tile;
+    out_bulk0 = 0;
+    if (tiles0) {
+      DevBuf counts0(tiles0 * 4 + 4, c.s), offs0(tiles0 * 8 + 16, c.s);
+      SvmArgs a0 = a;
+      a0.rows = (long long)src_bulk0;
+      a0.out[6] = offs0.p;
+      a0.out[7] = counts0.p;
+      launch(jit_function(src, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s,
+             a0);
+      k::exclusive_scan_u32(counts0.as<uint32_t>(), tiles0, offs0.as<unsigned long long>(),
+                            offs0.as<unsigned long long>() + tiles0, c.s);
+      SVM_HIP(hipMemcpyAsync(&out_bulk0, offs0.as<unsigned long long>() + tiles0, 8, hipMemcpyDeviceToHost, c.s));
+      SVM_HIP(hipStreamSynchronize(c.s));
+    }
+  }
   // output object: the sink schema; only projected columns are materialised (the
   // reference reserves the trailing column for the join hash table, core/pipeline.cu:94)
   std::vector<std::pair<int, int64_t>> oschema;
@@ -689,6 +721,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   }
   TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat, c.s);
   out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+  out->bulk0 = out_bulk0;
   for (int q = 0; q < P; ++q) {
     a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
     // a bare source column projected unchanged keeps a subset of that column's values

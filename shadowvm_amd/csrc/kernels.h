@@ -22,9 +22,15 @@ uint64_t dict_build(const int32_t* vals, uint64_t n, int32_t* dict, hipStream_t 
 // codes[r] = position of vals[r] in dict (binary search).
 void codes_of(const int32_t* vals, uint64_t n, const int32_t* dict, uint64_t nd, int32_t* codes,
               hipStream_t s);
-// first[key-kmin] = smallest row r with keys[r] == key (first initialised to INT32_MAX).
+// first[key-kmin] = smallest row r with keys[r] == key (first initialised to INT32_MAX),
+// over the rows with kept[r] != 0 when kept is given.
 void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int32_t* first,
-                hipStream_t s);
+                hipStream_t s, const uint8_t* kept = nullptr);
+// The reference's join hash table over keys[0..n) inserted in row order into `slots`
+// slots (Hash of the key bytes % slots, at most 10 linear probes, silent drop):
+// kept[r] = 1 if row r's insert found a slot, else 0.  Returns the number of dropped
+// inserts (synchronises).
+uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, hipStream_t s);
 // Packed map: slot = 0 if first == INT32_MAX else 1 / codes[row]+1 / row+1 by payload.
 void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
               uint32_t* packed, hipStream_t s);

@@ -90,11 +90,11 @@ __global__ void k_codes(const int32_t* __restrict__ vals, uint64_t n, const int3
 }
 
 __global__ void k_first_rows(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
-                             int32_t* first) {
+                             int32_t* first, const uint8_t* __restrict__ kept) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t off = (uint32_t)(keys[r] - kmin);
-    if (off < range) atomicMin(&first[off], (int32_t)r);
+    if (off < range && (!kept || kept[r])) atomicMin(&first[off], (int32_t)r);
   }
 }
 
@@ -257,9 +257,9 @@ void codes_of(const int32_t* vals, uint64_t n, const int32_t* dict, uint64_t nd,
 }
 
 void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int32_t* first,
-                hipStream_t s) {
+                hipStream_t s, const uint8_t* kept) {
   fill_u32(first, range, (uint32_t)INT_MAX, s);
-  if (n) k_first_rows<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first);
+  if (n) k_first_rows<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first, kept);
   LAUNCH_CHECK("first_rows");
 }
 

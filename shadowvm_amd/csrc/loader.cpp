@@ -131,6 +131,8 @@ std::unique_ptr<svm_result> run_load(const PipelineInfo& info, const Plan& plan,
   hipStream_t s = opt.stream ? opt.stream : E.stream;
   TablePtr out = E.store.create(info.id, oschema, total);  // zeroed: fields a row lacks read 0
   out->max_tuple_count = E.cfg.tuple_per_bulk;
+  // Loader::Host writes every row into the one sink bulk (core/operator.cpp:24-48)
+  out->bulk0 = total;
   ld::ParseArgs a;
   memset(&a, 0, sizeof(a));
   a.col_sep = col_sep;

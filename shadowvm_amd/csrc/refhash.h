@@ -1,0 +1,32 @@
+// refhash.h — the reference's join hash table, reduced to which rows it keeps.
+#pragma once
+
+#include "svm_internal.h"
+
+namespace svm {
+namespace k {
+
+// The clusters of a table's inserts that drop something (only there can a subset of
+// its rows, inserted in the same order, drop anything).  Entries are grouped by
+// cluster, in row order inside a cluster: entry e has the key key[e] and the home slot
+// home[e]; cluster c holds entries [start[c], start[c+1]).
+struct HtClusters {
+  uint64_t dropped = 0;     // inserts the full table drops
+  uint32_t clusters = 0;    // clusters with a drop
+  uint32_t entries = 0;     // their inserts
+  uint32_t max_rows = 0;    // most inserts in one of them
+  uint32_t max_span = 0;    // widest slot range of one of them (last home - first home + 10;
+                            // all ones when a cluster's probes wrap past the last slot)
+  DevBuf start, key, home;  // uint32 [clusters + 1], int32 [entries], uint32 [entries]
+};
+
+// The reference's hash table over keys[0..n), inserted in row order into `slots` slots
+// (Hash of the key's bytes % slots, at most 10 linear probes, silent drop;
+// com/cpp/cpu-runtime.hpp:80-91).  Writes kept[r] = 1 if row r's insert found a slot
+// (kept may be null) and, when `clusters` is given, the clusters that drop.  Returns
+// the number of dropped inserts (synchronises).
+uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, HtClusters* clusters,
+                    hipStream_t s);
+
+}  // namespace k
+}  // namespace svm

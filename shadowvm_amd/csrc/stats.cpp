@@ -13,6 +13,34 @@ namespace {
 std::mutex g_cache_mu;
 std::map<std::pair<const Table*, int>, std::shared_ptr<DictEntry>> g_dicts;
 std::map<std::pair<const Table*, int>, KeyEntry> g_keys;
+
+struct KeptEntry {
+  std::weak_ptr<Table> owner;
+  uint64_t dropped = 0;
+  DevBuf kept;  // one byte per row; empty when nothing is dropped
+};
+// (table, column, rows inserted, slots)
+std::map<std::tuple<const Table*, int, uint64_t, uint64_t>, KeptEntry> g_kept;
+
+struct ClusterEntry {
+  std::weak_ptr<Table> owner;
+  std::shared_ptr<const k::HtClusters> clusters;  // null: nothing dropped
+};
+std::map<std::tuple<const Table*, int, uint64_t>, ClusterEntry> g_clusters;
+
+// The first `rows` rows of column `col` of `t` inserted in row order into `slots`
+// slots (the caller holds g_cache_mu).
+const KeptEntry& kept_of(const TablePtr& t, int col, uint64_t rows, uint64_t slots, hipStream_t s) {
+  auto key = std::make_tuple(t.get(), col, rows, slots);
+  auto it = g_kept.find(key);
+  if (it != g_kept.end() && it->second.owner.lock() == t) return it->second;
+  KeptEntry e;
+  e.owner = t;
+  DevBuf kept(rows + 16, s);
+  e.dropped = k::ht_kept((const int32_t*)t->col_ptr(col), rows, slots, kept.as<uint8_t>(), s);
+  if (e.dropped) e.kept = std::move(kept);
+  return g_kept[key] = std::move(e);
+}
 }  // namespace
 
 std::pair<TablePtr, int> column_root(const TablePtr& t, int col) {
@@ -71,5 +99,50 @@ KeyEntry table_key(const TablePtr& t, int col, hipStream_t s) {
   g_keys[key] = e;
   return e;
 }
+
+const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStream_t s) {
+  *dropped = 0;
+  const uint64_t n = t->bulk0_rows();
+  // an insert can only fail with 10 slots already taken: the first 10 always succeed
+  if (n <= 10) return nullptr;
+  if (t->cols[col].type != Int32 || !t->col_ptr(col)) fail(SVM_ERR_INVALID, "join key column is not a materialised Int32 column");
+  // the reference passes HashRate * max_tuple_count_ as an unsigned int
+  // (com/cpp/com.hpp:169, core/pipeline.cu:105-106)
+  const uint64_t slots = (uint64_t)(uint32_t)(kHashRate * (t->max_tuple_count ? t->max_tuple_count : n));
+  if (!slots) fail(SVM_ERR_UNSUPPORTED, "join hash table of '" + t->id + "' has 0 slots (HashRate x bulk capacity wraps)");
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  // Inserting a subsequence of a row sequence keeps every slot it occupies occupied in
+  // the full sequence at the same point, so it drops only what the full sequence drops:
+  // a root column that drops nothing clears every table built from its rows.
+  auto root = column_root(t, col);
+  if (root.first != t || root.second != col) {
+    const TablePtr& R = root.first;
+    if (R->cols[root.second].type == Int32 && R->col_ptr(root.second) &&
+        kept_of(R, root.second, R->rows, slots, s).dropped == 0)
+      return nullptr;
+  }
+  const KeptEntry& e = kept_of(t, col, n, slots, s);
+  *dropped = e.dropped;
+  return e.dropped ? e.kept.as<uint8_t>() : nullptr;
+}
+
+std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int col, uint64_t slots,
+                                                         hipStream_t s) {
+  if (t->rows <= 10) return nullptr;
+  if (t->cols[col].type != Int32 || !t->col_ptr(col)) fail(SVM_ERR_INVALID, "join key column is not a materialised Int32 column");
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  auto key = std::make_tuple(t.get(), col, slots);
+  auto it = g_clusters.find(key);
+  if (it != g_clusters.end() && it->second.owner.lock() == t) return it->second.clusters;
+  ClusterEntry e;
+  e.owner = t;
+  auto hc = std::make_shared<k::HtClusters>();
+  k::ht_analyse((const int32_t*)t->col_ptr(col), t->rows, slots, nullptr, hc.get(), s);
+  if (hc->dropped) e.clusters = hc;
+  g_clusters[key] = e;
+  return e.clusters;
+}
+
+uint64_t pipeline_ht_slots(const Engine& E) { return (uint64_t)(uint32_t)(kHashRate * E.cfg.tuple_per_bulk); }
 
 }  // namespace svm

@@ -3,6 +3,7 @@
 // group-key values with every row's code.
 #pragma once
 
+#include "refhash.h"
 #include "svm_internal.h"
 
 namespace svm {
@@ -19,11 +20,31 @@ struct KeyEntry {
 };
 
 // The table and column a column's values were projected from (a select output's
-// bare-column projection keeps a subset of its source's values); the column itself
-// when it has no such origin.
+// bare-column projection keeps the passing rows of its source column, in source
+// order; a view is a row range of its base); the column itself when it has no such
+// origin.  The column's rows are an in-order subsequence of the root's rows.
 std::pair<TablePtr, int> column_root(const TablePtr& t, int col);
 
 std::shared_ptr<DictEntry> table_dict(const TablePtr& t, int col, hipStream_t s);
 KeyEntry table_key(const TablePtr& t, int col, hipStream_t s);
+
+constexpr uint64_t kHashRate = 5;  // com/cpp/define.hpp:29
+
+// Rows of bulk 0 of `t` that the reference's join hash table on Int32 column `col`
+// keeps (HashRate x max_tuple_count slots, rows inserted in row order, at most 10
+// probes, core/pipeline.cu:88-133): nullptr when it keeps every row, else one byte per
+// row (1 kept, 0 dropped) on the device, valid while `t` lives.  *dropped receives the
+// number of dropped inserts.  Cached per table.
+const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStream_t s);
+
+// The clusters of the reference's hash table over every row of Int32 column `col` of
+// `t` (row order, `slots` slots) that drop an insert, or nullptr when it drops
+// nothing (then no in-order subset of those rows drops anything either).  Cached.
+std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int col, uint64_t slots,
+                                                         hipStream_t s);
+
+// Slots of the hash table on a dimension object made by a pipeline (a bulk of
+// tuple_per_bulk rows): HashRate x tuple_per_bulk, as the reference's unsigned int.
+uint64_t pipeline_ht_slots(const Engine& E);
 
 }  // namespace svm

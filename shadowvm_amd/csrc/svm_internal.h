@@ -81,6 +81,15 @@ struct Table {
   uint64_t rows = 0;
   uint64_t row0 = 0;        // views: first row inside the base allocation (column dev = base)
   uint64_t max_tuple_count = 0;
+  // Rows of bulk 0, the only bulk a join reads (core/pipeline.cu:91,294-295).  A table
+  // cut into bulks of max_tuple_count rows has min(rows, max_tuple_count); a select
+  // output has the rows its source's bulk 0 produced (sink bulk i is the output of
+  // source bulk i, core/pipeline.cu:161-162,294-296); UINT64_MAX = the default rule.
+  uint64_t bulk0 = UINT64_MAX;
+  uint64_t bulk0_rows() const {
+    if (bulk0 != UINT64_MAX) return bulk0 < rows ? bulk0 : rows;
+    return max_tuple_count && max_tuple_count < rows ? max_tuple_count : rows;
+  }
   std::vector<Column> cols;
   ObjectInfo info;          // schema as ObjectInfo (ref object info_)
   TablePtr base;            // views keep their base table alive

@@ -153,6 +153,14 @@ def server_stop():
     _lib.check(_lib.lib().svm_server_stop())
 
 
+def hash_drops(id, col, slots=0):
+    """Inserts the reference's join hash table drops over column `col` of table `id`
+    inserted in row order into `slots` slots (0: HashRate x bulk capacity)."""
+    d = ctypes.c_uint64()
+    _lib.check(_lib.lib().svm_debug_hash_drops(id.encode(), col, slots, ctypes.byref(d)))
+    return int(d.value)
+
+
 def table_drop(id):
     _lib.check(_lib.lib().svm_table_drop(id.encode()))
 

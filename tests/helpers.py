@@ -4,6 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 from oracle import oracle as O
+from shadowvm_amd import pipeline as P
 from shadowvm_amd import ssb
 
 
@@ -37,3 +38,40 @@ def assert_same_columns(got, want):
         if not np.array_equal(g, w):
             bad = np.nonzero(g != w)[0][:5] if g.ndim == 1 else np.nonzero((g != w).any(axis=1))[0][:5]
             raise AssertionError(f"column {q} differs at rows {bad.tolist()}: {g[bad]} vs {w[bad]}")
+
+
+def edge_query(wl, qid, row0, rows, dims, fact_filter, keys, metric):
+    """A fused-form query over the four-way tables: dims is a list of
+    (dimension kind, filter, select, fact key column)."""
+    I, S = P.INT32, P.STRING
+    ids = {t.kind: t.id for t in wl.tables}
+    schemas = {
+        "date": (ids[ssb.GEN_DATE], [I, I, I], [4, 4, 4]),
+        "customer": (ids[ssb.GEN_CUST3], [I, S, S, S], [4, 10, 10, 10]),
+        "supplier": (ids[ssb.GEN_SUPP3], [I, S, S, S], [4, 10, 10, 10]),
+        "part": (ids[ssb.GEN_PART3], [I, I, I], [4, 4, 4]),
+    }
+    view = (f"{qid}_slice", ids[ssb.GEN_FACT3], row0, rows)
+    dpts, refs = [], []
+    for k, (kind, filt, sel, _) in enumerate(dims):
+        src, st, ss = schemas[kind]
+        pt = P.PipelineTest(f"{qid}_d{k}")
+        osch = P.create_schema([I, I, S], [4, 4, 40]) if kind in ("date", "part") else \
+            P.create_schema([I, S, S], [4, 10, 40])
+        pt.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema(st, ss))
+        if filt:
+            pt.AddOperator(P.FILTER, {"predication": filt}, osch)
+        pt.AddOperator(P.MAP, {"select": sel}, osch)
+        pt.AddOperator(P.SINK, {}, osch)
+        dpts.append((pt, src, []))
+        refs.append(pt.pipe_info["id"])
+    res = P.create_schema([I] * 12, [4] * 12)
+    fq = P.PipelineTest(qid)
+    fq.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I] * 6, [4] * 6))
+    if fact_filter:
+        fq.AddOperator(P.FILTER, {"predication": fact_filter}, res)
+    for k, (_, _, _, lcol) in enumerate(dims):
+        fq.AddOperator(P.JOINWITH, {"lkey": f"$(0.{lcol})", "rkey": f"$({k + 1}.0)"}, res)
+    fq.AddOperator(P.REDUCEBY, {"dim": keys, "metric": metric}, res)
+    fq.AddOperator(P.SINK, {}, res)
+    return ssb.Query(qid, view, dpts, (fq, view[0], refs), {})

@@ -7,7 +7,7 @@ from oracle import oracle as O
 from shadowvm_amd import pipeline as P
 from shadowvm_amd import ssb
 
-from .helpers import assert_same_columns, host_tables
+from .helpers import assert_same_columns, edge_query, host_tables
 
 pytestmark = pytest.mark.gpu
 
@@ -84,48 +84,9 @@ def test_batch_plain_matches_oracle(small_four_way):
             dims[pt.pipe_info["id"]] = O.run_pipeline(pt.plan(src, refs), tabs[src], [])
         fq, src, refs = q.fact
         want = O.run_pipeline(fq.plan(src, refs), view, [dims[r].table for r in refs])
-        if any(v for v in want.dropped.values()):
-            continue  # parity is defined only for drop-free reference hash tables
         got = res[p]
         assert got.rows == want.rows, q.qid
         assert_same_columns(got.columns, want.columns)
-
-
-def _edge_query(wl, qid, row0, rows, dims, fact_filter, keys, metric):
-    """A fused-form query over the four-way tables: dims is a list of
-    (dimension kind, filter, select, fact key column)."""
-    I, S = P.INT32, P.STRING
-    ids = {t.kind: t.id for t in wl.tables}
-    schemas = {
-        "date": (ids[ssb.GEN_DATE], [I, I, I], [4, 4, 4]),
-        "customer": (ids[ssb.GEN_CUST3], [I, S, S, S], [4, 10, 10, 10]),
-        "supplier": (ids[ssb.GEN_SUPP3], [I, S, S, S], [4, 10, 10, 10]),
-        "part": (ids[ssb.GEN_PART3], [I, I, I], [4, 4, 4]),
-    }
-    view = (f"{qid}_slice", ids[ssb.GEN_FACT3], row0, rows)
-    dpts, refs = [], []
-    for k, (kind, filt, sel, _) in enumerate(dims):
-        src, st, ss = schemas[kind]
-        pt = P.PipelineTest(f"{qid}_d{k}")
-        osch = P.create_schema([I, I, S], [4, 4, 40]) if kind in ("date", "part") else \
-            P.create_schema([I, S, S], [4, 10, 40])
-        pt.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema(st, ss))
-        if filt:
-            pt.AddOperator(P.FILTER, {"predication": filt}, osch)
-        pt.AddOperator(P.MAP, {"select": sel}, osch)
-        pt.AddOperator(P.SINK, {}, osch)
-        dpts.append((pt, src, []))
-        refs.append(pt.pipe_info["id"])
-    res = P.create_schema([I] * 12, [4] * 12)
-    fq = P.PipelineTest(qid)
-    fq.AddOperator(P.SOURCE, {"type": "object"}, P.create_schema([I] * 6, [4] * 6))
-    if fact_filter:
-        fq.AddOperator(P.FILTER, {"predication": fact_filter}, res)
-    for k, (_, _, _, lcol) in enumerate(dims):
-        fq.AddOperator(P.JOINWITH, {"lkey": f"$(0.{lcol})", "rkey": f"$({k + 1}.0)"}, res)
-    fq.AddOperator(P.REDUCEBY, {"dim": keys, "metric": metric}, res)
-    fq.AddOperator(P.SINK, {}, res)
-    return ssb.Query(qid, view, dpts, (fq, view[0], refs), {})
 
 
 def test_batch_edge_cases(small_four_way):
@@ -135,20 +96,20 @@ def test_batch_edge_cases(small_four_way):
     wl = small_four_way
     n = wl.fact_rows
     qs = [
-        _edge_query(wl, "eq0", 0, n, [("customer", 'StrCmp($(0.2),"NATION_03",10)!=0', "$(0.0), $(0.3)", 1)],
+        edge_query(wl, "eq0", 0, n, [("customer", 'StrCmp($(0.2),"NATION_03",10)!=0', "$(0.0), $(0.3)", 1)],
                     "", "$(1.1)", "$(0.4)+$(0.5)"),
-        _edge_query(wl, "eq1", 1_000_003, 2_000_001,
+        edge_query(wl, "eq1", 1_000_003, 2_000_001,
                     [("date", "$(0.1)>=1993 && $(0.1)<=1995", "$(0.0), $(0.1)", 0)],
                     "$(0.4)<=5000000", "$(1.1)", "$(0.4)*$(0.5)"),
-        _edge_query(wl, "eq2", 17, 0, [("date", "$(0.1)==1994", "$(0.0), $(0.1)", 0)], "", "", "$(0.4)"),
-        _edge_query(wl, "eq3", 5, 0, [("date", "", "$(0.0), $(0.1)", 0)], "", "$(1.1)", "$(0.5)"),
-        _edge_query(wl, "eq4", 0, n, [("supplier", 'StrCmp($(0.1),"NOWHERE",7)==0', "$(0.0), $(0.2)", 2)],
+        edge_query(wl, "eq2", 17, 0, [("date", "$(0.1)==1994", "$(0.0), $(0.1)", 0)], "", "", "$(0.4)"),
+        edge_query(wl, "eq3", 5, 0, [("date", "", "$(0.0), $(0.1)", 0)], "", "$(1.1)", "$(0.5)"),
+        edge_query(wl, "eq4", 0, n, [("supplier", 'StrCmp($(0.1),"NOWHERE",7)==0', "$(0.0), $(0.2)", 2)],
                     "", "$(1.1)", "$(0.4)"),
-        _edge_query(wl, "eq5", 3, n - 7,
+        edge_query(wl, "eq5", 3, n - 7,
                     [("customer", 'StrCmp($(0.3),"REGION_1",8)==0', "$(0.0), $(0.2)", 1),
                      ("customer", "", "$(0.0), $(0.3)", 2)],
                     "$(0.5)>=1000", "$(1.1), $(2.1)", "$(0.5)-$(0.4)"),
-        _edge_query(wl, "eq6", 0, n, [("part", "$(0.1)<=3", "$(0.0), $(0.2)", 3)], "", "", "$(0.4)-$(0.5)"),
+        edge_query(wl, "eq6", 0, n, [("part", "$(0.1)<=3", "$(0.0), $(0.2)", 3)], "", "", "$(0.4)-$(0.5)"),
     ]
     qs += ssb.batch_queries(wl, n=70, seed=9, window=6_000_000, min_rows=100_000, prefix="em")
     ssb.make_views(qs)

@@ -111,7 +111,7 @@ def _drops(rt, keys, slots):
 # Dropped inserts of primary keys 1..N inserted in order (the 10-probe cap).
 @pytest.mark.parametrize("n,slots,want", [(2556, 10_000_000, 0), (1_400_000, 10_000_000, 6910),
                                           (1_400_000, 15_000_000, 0), (800_000, 10_000_000, 630),
-                                          (800_000, 15_000_000, 0)])
+                                          (800_000, 15_000_000, 0), (3_000_000, 15_000_000, 746_032)])
 def test_drop_counts(rt, n, slots, want):
     d, _ = _drops(rt, np.arange(1, n + 1), slots)
     assert d == want
