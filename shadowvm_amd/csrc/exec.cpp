@@ -187,9 +187,10 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     // dimension object's keys, cached per resident table: no host round trip)
     const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
+    auto root = column_root(Dp, R.col);
+    KeyEntry ke;
     if (n0) {
-      auto root = column_root(Dp, R.col);
-      KeyEntry ke = table_key(root.first, root.second, c.s);
+      ke = table_key(root.first, root.second, c.s);
       uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
       if (range > kMaxDenseRange)
         fail(SVM_ERR_UNSUPPORTED, "join key range " + std::to_string(range) + " of '" + D.id + "' exceeds the dense map limit");
@@ -200,14 +201,27 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     jb.words = ((uint64_t)jb.range + per - 1) / per;
     jb.packed = DevBuf(jb.words * 4 + 16, c.s);
     if (jb.range) {
-      // the reference's table drops inserts that find no free slot in 10 probes; a
-      // dropped row is never found, so the map takes the first kept row of each key
-      const uint8_t* kept = join_kept(Dp, R.col, &jb.dropped, c.s);
+      // The reference's table drops inserts that find no free slot in 10 probes; a
+      // dropped row is never found, so the map takes the first kept row of each key.
+      // A dimension object drawn (in order) from a root table with unique keys drops
+      // only inside the root's dropping clusters: its map is built, then those
+      // clusters are replayed on the device.  Otherwise the kept rows are computed
+      // (and cached) for the object itself.
       DevBuf first((uint64_t)jb.range * 4, c.s);
-      k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s, kept);
-      if (jb.dropped && c.E.cfg.verbose)
-        fprintf(stderr, "[svm] %s: the hash table on '%s' drops %llu of %llu inserts\n", c.info.id.c_str(),
-                D.id.c_str(), (unsigned long long)jb.dropped, (unsigned long long)n0);
+      const bool derived = root.first != Dp || root.second != R.col;
+      const uint64_t slots = (uint64_t)(uint32_t)(kHashRate * (D.max_tuple_count ? D.max_tuple_count : n0));
+      std::shared_ptr<const k::HtClusters> hc;
+      if (derived && ke.unique && slots) hc = table_drop_clusters(root.first, root.second, slots, c.s);
+      if (derived && ke.unique && slots && (!hc || hc->max_span <= (uint32_t)k::kReplaySpan)) {
+        k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s);
+        if (hc && n0 > 10) k::ht_clear_dropped(*hc, jb.kmin, jb.range, first.as<int32_t>(), c.s);
+      } else {
+        const uint8_t* kept = join_kept(Dp, R.col, &jb.dropped, c.s);
+        k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s, kept);
+        if (jb.dropped && c.E.cfg.verbose)
+          fprintf(stderr, "[svm] %s: the hash table on '%s' drops %llu of %llu inserts\n", c.info.id.c_str(),
+                  D.id.c_str(), (unsigned long long)jb.dropped, (unsigned long long)n0);
+      }
       k::pack_map(first.as<int32_t>(), jb.range, payload, codes_for_pack.as<int32_t>(), jb.shape.wbits,
                   jb.packed.as<uint32_t>(), c.s);
       // `first` and the codes are released in stream order after pack_map

@@ -28,5 +28,25 @@ struct HtClusters {
 uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, HtClusters* clusters,
                     hipStream_t s);
 
+// False when an in-order subset of the rows behind `hc` (the subset's keys are
+// keys[0..n), all within [kmin, kmin + range)) certainly drops nothing: a subset drops
+// only inside a cluster that drops, and only where it holds at least 11 of the
+// cluster's inserts (a drop needs 10 slots taken by earlier inserts of the cluster).
+// Inserts are matched by key, so duplicate keys only make the answer more cautious.
+// Synchronises.
+bool ht_subset_may_drop(const HtClusters& hc, const int32_t* keys, uint64_t n, int32_t kmin, uint64_t range,
+                        hipStream_t s);
+
+// Widest cluster (in slots) replayed in registers by the subset kernels.
+constexpr int kReplaySpan = 256;
+
+// For a dimension object that is an in-order subset, with unique keys, of the rows
+// behind `hc`, whose dense map first[key - kmin] (range entries) holds the object's row
+// of each key or INT32_MAX: clears the entries of the keys whose inserts the
+// reference's table over the object's rows drops.  Replays only the clusters where
+// the object holds at least 11 rows; needs hc.max_span <= kReplaySpan.  No
+// synchronisation.
+void ht_clear_dropped(const HtClusters& hc, int32_t kmin, uint32_t range, int32_t* first, hipStream_t s);
+
 }  // namespace k
 }  // namespace svm

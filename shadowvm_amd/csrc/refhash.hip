@@ -23,6 +23,7 @@
 // atomic OR whose old value says whether the slot was free.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -146,6 +147,67 @@ __global__ void k_cluster_stats(uint32_t* start, uint32_t nc, uint32_t m, const 
   if (blockIdx.x == 0 && threadIdx.x == 0) start[nc] = m;
 }
 
+// bit (key - kmin) of every key in keys[0..n)
+__global__ void k_key_bits(const int32_t* __restrict__ keys, uint32_t n, int32_t kmin, uint32_t range,
+                           uint32_t* bits) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t off = (uint32_t)(keys[r] - kmin);
+    if (off < range) atomicOr(&bits[off >> 5], 1u << (off & 31u));
+  }
+}
+
+// *flag = 1 when a cluster holds kProbes + 1 entries whose key bit is set
+__global__ void k_cluster_hits(const uint32_t* __restrict__ start, uint32_t nc, const int32_t* __restrict__ key,
+                               int32_t kmin, uint32_t range, const uint32_t* __restrict__ bits, uint32_t* flag) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    uint32_t hits = 0;
+    for (uint32_t e = start[c]; e < start[c + 1]; ++e) {
+      const uint32_t off = (uint32_t)(key[e] - kmin);
+      hits += (off < range && ((bits[off >> 5] >> (off & 31u)) & 1u)) ? 1u : 0u;
+      if (hits > (uint32_t)kProbes) {
+        *flag = 1u;
+        break;
+      }
+    }
+  }
+}
+
+// Replays, in each dropping cluster where the object holds kProbes + 1 or more rows,
+// the object's inserts (entries whose key has a row in `first`) and clears the entry of
+// every key whose insert drops.
+__global__ void k_clear_dropped(const uint32_t* __restrict__ start, uint32_t nc, const int32_t* __restrict__ key,
+                                const uint32_t* __restrict__ home, int32_t kmin, uint32_t range, int32_t* first) {
+  constexpr int kWords = kReplaySpan / 64;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    const uint32_t e0 = start[c], e1 = start[c + 1];
+    uint32_t members = 0, base = 0xFFFFFFFFu;
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t off = (uint32_t)(key[e] - kmin);
+      if (off < range && first[off] != INT_MAX) {
+        ++members;
+        base = min(base, home[e]);
+      }
+    }
+    if (members <= (uint32_t)kProbes) continue;
+    unsigned long long occ[kWords] = {};
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t off = (uint32_t)(key[e] - kmin);
+      if (off >= range || first[off] == INT_MAX) continue;
+      uint32_t pos = home[e] - base;  // < kReplaySpan - kProbes + 1: the cluster spans <= kReplaySpan slots
+      bool placed = false;
+      for (int t = 0; t < kProbes; ++t, ++pos) {
+        const unsigned long long bit = 1ull << (pos & 63u);
+        if (!(occ[pos >> 6] & bit)) {
+          occ[pos >> 6] |= bit;
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) first[off] = INT_MAX;
+    }
+  }
+}
+
 int bits_of(uint64_t v) {
   int b = 1;
   while (b < 32 && (1ull << b) <= v) ++b;
@@ -249,6 +311,34 @@ uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* ke
   out->max_rows = st[0];
   out->max_span = st[1];
   return d;
+}
+
+bool ht_subset_may_drop(const HtClusters& hc, const int32_t* keys, uint64_t n, int32_t kmin, uint64_t range,
+                        hipStream_t s) {
+  if (!hc.dropped || n <= (uint64_t)kProbes) return false;
+  if (range == 0 || range > (1ull << 31) || n > 0x7FFFFFFFull) return true;
+  const uint64_t words = (range + 31) / 32;
+  DevBuf bits(words * 4 + 4, s), flag(8, s);
+  fill_u32(bits.p, words, 0u, s);
+  fill_u32(flag.p, 1, 0u, s);
+  k_key_bits<<<grid_for(n), kBlock, 0, s>>>(keys, (uint32_t)n, kmin, (uint32_t)range, bits.as<uint32_t>());
+  hip_check(hipGetLastError(), "ht_subset key bits");
+  k_cluster_hits<<<grid_for(hc.clusters), kBlock, 0, s>>>(hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(),
+                                                           kmin, (uint32_t)range, bits.as<uint32_t>(),
+                                                           flag.as<uint32_t>());
+  hip_check(hipGetLastError(), "ht_subset hits");
+  uint32_t f = 0;
+  SVM_HIP(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return f != 0;
+}
+
+void ht_clear_dropped(const HtClusters& hc, int32_t kmin, uint32_t range, int32_t* first, hipStream_t s) {
+  if (!hc.dropped || !hc.clusters) return;
+  if (hc.max_span > (uint32_t)kReplaySpan) fail(SVM_ERR_INVALID, "ht_clear_dropped: cluster wider than the replay window");
+  k_clear_dropped<<<grid_for(hc.clusters), kBlock, 0, s>>>(hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(),
+                                                           hc.home.as<uint32_t>(), kmin, range, first);
+  hip_check(hipGetLastError(), "ht_clear_dropped");
 }
 
 uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, hipStream_t s) {

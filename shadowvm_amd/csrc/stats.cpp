@@ -28,19 +28,6 @@ struct ClusterEntry {
 };
 std::map<std::tuple<const Table*, int, uint64_t>, ClusterEntry> g_clusters;
 
-// The first `rows` rows of column `col` of `t` inserted in row order into `slots`
-// slots (the caller holds g_cache_mu).
-const KeptEntry& kept_of(const TablePtr& t, int col, uint64_t rows, uint64_t slots, hipStream_t s) {
-  auto key = std::make_tuple(t.get(), col, rows, slots);
-  auto it = g_kept.find(key);
-  if (it != g_kept.end() && it->second.owner.lock() == t) return it->second;
-  KeptEntry e;
-  e.owner = t;
-  DevBuf kept(rows + 16, s);
-  e.dropped = k::ht_kept((const int32_t*)t->col_ptr(col), rows, slots, kept.as<uint8_t>(), s);
-  if (e.dropped) e.kept = std::move(kept);
-  return g_kept[key] = std::move(e);
-}
 }  // namespace
 
 std::pair<TablePtr, int> column_root(const TablePtr& t, int col) {
@@ -110,20 +97,48 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
   // (com/cpp/com.hpp:169, core/pipeline.cu:105-106)
   const uint64_t slots = (uint64_t)(uint32_t)(kHashRate * (t->max_tuple_count ? t->max_tuple_count : n));
   if (!slots) fail(SVM_ERR_UNSUPPORTED, "join hash table of '" + t->id + "' has 0 slots (HashRate x bulk capacity wraps)");
-  std::lock_guard<std::mutex> g(g_cache_mu);
-  // Inserting a subsequence of a row sequence keeps every slot it occupies occupied in
-  // the full sequence at the same point, so it drops only what the full sequence drops:
-  // a root column that drops nothing clears every table built from its rows.
-  auto root = column_root(t, col);
-  if (root.first != t || root.second != col) {
-    const TablePtr& R = root.first;
-    if (R->cols[root.second].type == Int32 && R->col_ptr(root.second) &&
-        kept_of(R, root.second, R->rows, slots, s).dropped == 0)
-      return nullptr;
+  const auto key = std::make_tuple(t.get(), col, n, slots);
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto it = g_kept.find(key);
+    if (it != g_kept.end() && it->second.owner.lock() == t) {
+      *dropped = it->second.dropped;
+      return it->second.dropped ? it->second.kept.as<uint8_t>() : nullptr;
+    }
   }
-  const KeptEntry& e = kept_of(t, col, n, slots, s);
-  *dropped = e.dropped;
-  return e.dropped ? e.kept.as<uint8_t>() : nullptr;
+  const int32_t* keys = (const int32_t*)t->col_ptr(col);
+  // Inserting a subsequence of a row sequence keeps every slot it occupies occupied in
+  // the full sequence at the same point, so it drops only inside the full sequence's
+  // dropping clusters: a root column that drops nothing, or a subset that holds at
+  // most 10 rows of each such cluster, drops nothing.
+  bool replay = true;
+  auto root = column_root(t, col);
+  if ((root.first != t || root.second != col) && root.first->cols[root.second].type == Int32 &&
+      root.first->col_ptr(root.second)) {
+    auto hc = table_drop_clusters(root.first, root.second, slots, s);
+    if (!hc) {
+      replay = false;
+    } else {
+      KeyEntry ke = table_key(root.first, root.second, s);
+      const uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
+      replay = k::ht_subset_may_drop(*hc, keys, n, ke.kmin, range, s);
+    }
+  }
+  KeptEntry e;
+  e.owner = t;
+  if (replay) {
+    DevBuf kept(n + 16);  // cached past this run: not from the stream's per-run arena
+    e.dropped = k::ht_kept(keys, n, slots, kept.as<uint8_t>(), s);
+    if (e.dropped) e.kept = std::move(kept);
+  }
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  auto it = g_kept.find(key);
+  if (it == g_kept.end() || it->second.owner.lock() != t) {
+    g_kept[key] = std::move(e);
+    it = g_kept.find(key);
+  }
+  *dropped = it->second.dropped;
+  return it->second.dropped ? it->second.kept.as<uint8_t>() : nullptr;
 }
 
 std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int col, uint64_t slots,

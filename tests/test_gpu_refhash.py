@@ -58,8 +58,12 @@ def test_tiny_bulk0_and_drops(engine):
         P.table_upload("tc_fact", [I, I], [4, 4], fact)
         dim_plan("tc_dim", "").Run("tc_base", want_result=False)
         assert P.hash_drops("tc_dim", 0) == len(base[0]) - 10
+        want_sum = int(sum(int(v) for k, v in zip(*fact) if int(k) in kept))
         got = fact_plan("tc_q", "", "$(0.1)").Run("tc_fact", ["tc_dim"])
-        assert int(got.columns[0][0]) == int(sum(int(v) for k, v in zip(*fact) if int(k) in kept))
+        assert int(got.columns[0][0]) == want_sum
+        # the uploaded table itself as the dimension (its own kept rows, not a subset's)
+        got = fact_plan("tc_q2", "", "$(0.1)").Run("tc_fact", ["tc_base"])
+        assert int(got.columns[0][0]) == want_sum
         g = fact_plan("tc_g", "$(1.1)", "$(0.1)+$(1.0)").Run("tc_fact", ["tc_dim"])
         want = O.run_pipeline(fact_plan("tc_g", "$(1.1)", "$(0.1)+$(1.0)").plan("f", ["d"]),
                               O.OTable([I, I], [4, 4], fact, 16),
