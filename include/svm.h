@@ -101,7 +101,28 @@ int svm_result_col_type(const svm_result* r, int col, int32_t* type, int64_t* va
 const void* svm_result_col(const svm_result* r, int col);
 /* Timing of the last run of this result: build, scan and finish, in ms (GPU events). */
 int svm_result_timing(const svm_result* r, double* build_ms, double* scan_ms, double* finish_ms);
+/* Everything the accessors above return, in one call (no reference counterpart: it
+ * saves a binding one foreign call per field when many results are read). */
+typedef struct svm_result_desc {
+  int32_t kind;               /* as svm_result_kind */
+  int32_t ncols;              /* <= 24 (MaxColumnNum, com/cpp/define.hpp:25) */
+  int64_t rows;
+  int32_t type[24];
+  int64_t val_size[24];
+  const void* col[24];        /* host column data, rows * val_size bytes each */
+  double build_ms, scan_ms, finish_ms;
+} svm_result_desc;
+int svm_result_describe(const svm_result* r, svm_result_desc* d);
 void svm_result_free(svm_result* r);
+
+/* The same for n results at once (no reference counterpart; for bindings that read
+ * a whole batch): describe fills d[i] for each non-NULL rs[i] (kind -1 for NULL);
+ * copy writes every column of every non-NULL result into dst, results in order,
+ * columns in order, rows * val_size bytes each, each column starting at the next
+ * multiple of 8 bytes; free releases them all. */
+int svm_results_describe(int n, svm_result* const* rs, svm_result_desc* d);
+int svm_results_copy(int n, svm_result* const* rs, void* dst, uint64_t dst_bytes);
+void svm_results_free(int n, svm_result* const* rs);
 
 /* ---- batched execution: many star-join queries fused into one shared scan ----
  * plans: dimension and fact pipelines of all queries, in submission order.  Fact

@@ -29,6 +29,13 @@ class SvmConfig(ctypes.Structure):
                 ("verbose", ctypes.c_int)]
 
 
+class SvmResultDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("ncols", ctypes.c_int32), ("rows", ctypes.c_int64),
+                ("type", ctypes.c_int32 * 24), ("val_size", ctypes.c_int64 * 24),
+                ("col", ctypes.c_void_p * 24), ("build_ms", ctypes.c_double),
+                ("scan_ms", ctypes.c_double), ("finish_ms", ctypes.c_double)]
+
+
 class SvmGenParams(ctypes.Structure):
     _fields_ = [("n_cust", ctypes.c_uint64), ("n_supp", ctypes.c_uint64),
                 ("n_part", ctypes.c_uint64), ("zipf_s", ctypes.c_double)]
@@ -60,7 +67,11 @@ SIGNATURES = {
     "svm_result_col_type": (ctypes.c_int, [c_vp, ctypes.c_int, c_i32p, c_i64p]),
     "svm_result_col": (c_vp, [c_vp, ctypes.c_int]),
     "svm_result_timing": (ctypes.c_int, [c_vp, c_dblp, c_dblp, c_dblp]),
+    "svm_result_describe": (ctypes.c_int, [c_vp, ctypes.POINTER(SvmResultDesc)]),
     "svm_result_free": (None, [c_vp]),
+    "svm_results_describe": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_vp), ctypes.POINTER(SvmResultDesc)]),
+    "svm_results_copy": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_vp), c_vp, ctypes.c_uint64]),
+    "svm_results_free": (None, [ctypes.c_int, ctypes.POINTER(c_vp)]),
     "svm_batch_run": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_u8p), ctypes.POINTER(ctypes.c_size_t),
                                      ctypes.POINTER(c_vp)]),
     "svm_server_start": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]),

@@ -30,7 +30,6 @@
 #include <deque>
 #include <map>
 #include <mutex>
-#include <regex>
 #include <set>
 #include <thread>
 #include <tuple>
@@ -115,39 +114,113 @@ struct Atom {
   int n = 0;
 };
 
+// A cursor over the text of an atom, for the fixed grammar of the fusable forms
+// (white space as C isspace, digits 0-9).
+struct Scan {
+  const char* p;
+  const char* e;
+  explicit Scan(const std::string& s) : p(s.data()), e(s.data() + s.size()) {}
+  void ws() { while (p < e && isspace((unsigned char)*p)) ++p; }
+  bool lit(const char* t) {
+    const size_t n = strlen(t);
+    if ((size_t)(e - p) < n || memcmp(p, t, n) != 0) return false;
+    p += n;
+    return true;
+  }
+  bool digits(std::string& out) {
+    const char* b = p;
+    while (p < e && *p >= '0' && *p <= '9') ++p;
+    if (p == b) return false;
+    out.assign(b, p);
+    return true;
+  }
+  bool end() const { return p == e; }
+  // $( o . c ), with white space allowed inside; the object's digits are kept as text
+  bool operand(std::string& obj, int& col) {
+    std::string c;
+    if (!lit("$(")) return false;
+    ws();
+    if (!digits(obj)) return false;
+    ws();
+    if (!lit(".")) return false;
+    ws();
+    if (!digits(c)) return false;
+    ws();
+    if (!lit(")")) return false;
+    col = atoi(c.c_str());
+    return true;
+  }
+  // == <= >= < > (tried in that order)
+  bool cmp(std::string& op) {
+    for (const char* o : {"==", "<=", ">=", "<", ">"})
+      if (lit(o)) {
+        op = o;
+        return true;
+      }
+    return false;
+  }
+  // an optional minus, white space, digits
+  bool int_text(std::string& t) {
+    const char* b = p;
+    lit("-");
+    ws();
+    std::string d;
+    if (!digits(d)) return false;
+    t.assign(b, p);
+    return true;
+  }
+};
+
 // "$(o.c) OP int", "int OP $(o.c)", "StrCmp($(o.c),"lit",n)==0" / "!=0"
 bool parse_atom(const std::string& text, Atom& a) {
-  static const std::regex re_l(R"(^\$\(\s*(\d+)\s*\.\s*(\d+)\s*\)\s*(==|<=|>=|<|>)\s*(-?\s*\d+)$)");
-  static const std::regex re_r(R"(^(-?\s*\d+)\s*(==|<=|>=|<|>)\s*\$\(\s*(\d+)\s*\.\s*(\d+)\s*\)$)");
-  static const std::regex re_s(
-      R"RE(^StrCmp\s*\(\s*\$\(\s*(\d+)\s*\.\s*(\d+)\s*\)\s*,\s*"([^"\\]*)"\s*,\s*(\d+)\s*\)\s*(==|!=)\s*0$)RE");
-  std::string s = unwrap(text);
-  std::smatch m;
-  std::string op;
+  const std::string s = unwrap(text);
+  std::string op, num, obj;
   long long v = 0;
-  if (std::regex_match(s, m, re_l)) {
-    if (!plain_int(m[4].str(), v)) return false;
-    a.obj = atoi(m[1].str().c_str());
-    a.col = atoi(m[2].str().c_str());
-    op = m[3].str();
-  } else if (std::regex_match(s, m, re_r)) {
-    if (!plain_int(m[1].str(), v)) return false;
-    a.obj = atoi(m[3].str().c_str());
-    a.col = atoi(m[4].str().c_str());
-    op = m[2].str();  // mirror: v OP c  <=>  c OP' v
+  Scan l(s);
+  if (l.operand(obj, a.col) && (l.ws(), l.cmp(op)) && (l.ws(), l.int_text(num)) && l.end()) {
+    if (!plain_int(num, v)) return false;
+    a.obj = atoi(obj.c_str());
+  } else if (Scan r(s); r.int_text(num) && (r.ws(), r.cmp(op)) && (r.ws(), r.operand(obj, a.col)) && r.end()) {
+    if (!plain_int(num, v)) return false;
+    a.obj = atoi(obj.c_str());
+    // mirror: v OP c  <=>  c OP' v
     if (op == "<") op = ">";
     else if (op == ">") op = "<";
     else if (op == "<=") op = ">=";
     else if (op == ">=") op = "<=";
-  } else if (std::regex_match(s, m, re_s)) {
-    a.obj = atoi(m[1].str().c_str());
-    a.col = atoi(m[2].str().c_str());
-    a.kind = m[5].str() == "==" ? bk::DP_STREQ : bk::DP_STRNE;
-    a.lit = m[3].str();
-    a.n = atoi(m[4].str().c_str());
-    return a.lit.size() < 47;
   } else {
-    return false;
+    Scan t(s);
+    std::string n, lit;
+    if (!t.lit("StrCmp")) return false;
+    t.ws();
+    if (!t.lit("(")) return false;
+    t.ws();
+    if (!t.operand(obj, a.col)) return false;
+    t.ws();
+    if (!t.lit(",")) return false;
+    t.ws();
+    if (!t.lit("\"")) return false;
+    const char* b = t.p;
+    while (t.p < t.e && *t.p != '"' && *t.p != '\\') ++t.p;
+    lit.assign(b, t.p);
+    if (!t.lit("\"")) return false;
+    t.ws();
+    if (!t.lit(",")) return false;
+    t.ws();
+    if (!t.digits(n)) return false;
+    t.ws();
+    if (!t.lit(")")) return false;
+    t.ws();
+    if (!t.cmp(op) && !t.lit("!=")) return false;
+    const bool ne = op.empty();
+    if (!ne && op != "==") return false;
+    t.ws();
+    if (!t.lit("0") || !t.end()) return false;
+    a.obj = atoi(obj.c_str());
+    a.kind = ne ? bk::DP_STRNE : bk::DP_STREQ;
+    a.lit = lit;
+    a.n = atoi(n.c_str());
+    return a.lit.size() < 47;
   }
   if (op == "==") { a.lo = v; a.hi = v; }
   else if (op == "<") { a.hi = v - 1; }
@@ -171,30 +244,32 @@ bool parse_conj(const std::vector<std::string>& filters, std::vector<Atom>& atom
 }
 
 bool parse_operand(const std::string& t, int& obj, int& col) {
-  static const std::regex re(R"(^\$\(\s*(\d+)\s*\.\s*(\d+)\s*\)$)");
-  std::smatch m;
-  std::string s = unwrap(t);
-  if (!std::regex_match(s, m, re)) return false;
-  obj = atoi(m[1].str().c_str());
-  col = atoi(m[2].str().c_str());
+  const std::string s = unwrap(t);
+  Scan c(s);
+  std::string o;
+  if (!c.operand(o, col) || !c.end()) return false;
+  obj = atoi(o.c_str());
   return true;
 }
 
+// "$(0.a)" or "$(0.a) OP $(0.b)" with OP one of - + *
 bool parse_metric(const std::string& t, int& op, int& a, int& b) {
-  static const std::regex re(R"(^\$\(\s*0\s*\.\s*(\d+)\s*\)\s*([-+*])\s*\$\(\s*0\s*\.\s*(\d+)\s*\)$)");
   int obj = 0;
   if (parse_operand(t, obj, a)) {
     op = bk::MOP_COL;
     b = a;
     return obj == 0;
   }
-  std::smatch m;
-  std::string s = unwrap(t);
-  if (!std::regex_match(s, m, re)) return false;
-  a = atoi(m[1].str().c_str());
-  b = atoi(m[3].str().c_str());
-  char c = m[2].str()[0];
-  op = c == '-' ? bk::MOP_SUB : c == '+' ? bk::MOP_ADD : bk::MOP_MUL;
+  const std::string s = unwrap(t);
+  Scan c(s);
+  std::string o1, o2;
+  if (!c.operand(o1, a) || o1 != "0") return false;
+  c.ws();
+  if (c.p == c.e || (*c.p != '-' && *c.p != '+' && *c.p != '*')) return false;
+  const char x = *c.p++;
+  c.ws();
+  if (!c.operand(o2, b) || o2 != "0" || !c.end()) return false;
+  op = x == '-' ? bk::MOP_SUB : x == '+' ? bk::MOP_ADD : bk::MOP_MUL;
   return true;
 }
 
@@ -1124,12 +1199,14 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
     jobs[i].info = decode_pipeline_info(plans[i], lens[i]);
     jobs[i].plan = decode_plan(jobs[i].info);
   }
+  const double t_decoded = host_ms();
   // dimension candidates, by pipeline id
   std::map<std::string, DimLowered> dims;
   for (auto& j : jobs) {
     DimLowered d = lower_dim(j.info, j.plan, E);
     if (d.ok) dims[j.info.id] = d;
   }
+  const double t_dims = host_ms();
   std::vector<FactLowered> facts(n);
   std::vector<bool> fused(n, false);
   for (int i = 0; i < n; ++i) {
@@ -1197,8 +1274,9 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
   if (outs)
     for (int i = 0; i < n; ++i) outs[i] = res[i] ? res[i].release() : nullptr;
   if (E.cfg.verbose)
-    fprintf(stderr, "[svm] batch_run %d plans, %d fused: host lowering %.3f ms, plain plans %.3f ms, total %.3f ms\n",
-            n, nfused, t_lowered, t_plain - t_lowered, host_ms());
+    fprintf(stderr, "[svm] batch_run %d plans, %d fused: host lowering %.3f ms (decode %.3f, dimensions %.3f, facts "
+                    "%.3f), plain plans %.3f ms, total %.3f ms\n",
+            n, nfused, t_lowered, t_decoded, t_dims - t_decoded, t_lowered - t_dims, t_plain - t_lowered, host_ms());
   return nfused;
 }
 

@@ -230,7 +230,63 @@ int svm_result_timing(const svm_result* r, double* build_ms, double* scan_ms, do
   });
 }
 
+int svm_result_describe(const svm_result* r, svm_result_desc* d) {
+  return guard([&] {
+    if (!r || !d) fail(SVM_ERR_INVALID, "null result or descriptor");
+    if (r->types.size() > 24) fail(SVM_ERR_INVALID, "result has more than 24 columns");
+    memset(d, 0, sizeof(*d));
+    d->kind = r->kind;
+    d->ncols = (int32_t)r->types.size();
+    d->rows = r->rows;
+    for (size_t c = 0; c < r->types.size(); ++c) {
+      d->type[c] = r->types[c].first;
+      d->val_size[c] = r->types[c].second;
+      d->col[c] = c < r->cols.size() ? (const void*)r->cols[c].data() : nullptr;
+    }
+    d->build_ms = r->build_ms;
+    d->scan_ms = r->scan_ms;
+    d->finish_ms = r->finish_ms;
+  });
+}
+
 void svm_result_free(svm_result* r) { delete r; }
+
+int svm_results_describe(int n, svm_result* const* rs, svm_result_desc* d) {
+  return guard([&] {
+    if (n < 0 || (n && (!rs || !d))) fail(SVM_ERR_INVALID, "bad result arrays");
+    for (int i = 0; i < n; ++i) {
+      if (!rs[i]) {
+        memset(&d[i], 0, sizeof(d[i]));
+        d[i].kind = -1;
+        continue;
+      }
+      const int rc = svm_result_describe(rs[i], &d[i]);
+      if (rc != SVM_OK) fail(rc, svm_last_error());
+    }
+  });
+}
+
+int svm_results_copy(int n, svm_result* const* rs, void* dst, uint64_t dst_bytes) {
+  return guard([&] {
+    if (n < 0 || (n && !rs)) fail(SVM_ERR_INVALID, "bad result array");
+    uint64_t off = 0;
+    for (int i = 0; i < n; ++i) {
+      const svm_result* r = rs[i];
+      if (!r) continue;
+      for (size_t c = 0; c < r->types.size(); ++c) {
+        const uint64_t b = (uint64_t)r->rows * (uint64_t)r->types[c].second;
+        if (off + b > dst_bytes) fail(SVM_ERR_INVALID, "destination buffer too small");
+        if (b && c < r->cols.size()) memcpy((char*)dst + off, r->cols[c].data(), b);
+        off = (off + b + 7) & ~7ull;
+      }
+    }
+  });
+}
+
+void svm_results_free(int n, svm_result* const* rs) {
+  if (!rs) return;
+  for (int i = 0; i < n; ++i) delete rs[i];
+}
 
 int svm_batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result** outs) {
   int rc = SVM_OK;

@@ -40,6 +40,18 @@ struct Reader {
       default: fail(SVM_ERR_INVALID, "PipelineInfo: unsupported wire type");
     }
   }
+  // occurrences of length-delimited field f at this level (to size vectors once)
+  size_t count(int f) const {
+    Reader r{p, end};
+    size_t n = 0;
+    while (!r.done()) {
+      uint64_t tag = r.varint();
+      int g = (int)(tag >> 3), wt = (int)(tag & 7);
+      if (g == f && wt == 2) ++n;
+      r.skip(wt);
+    }
+    return n;
+  }
   void advance(size_t n) {
     if (n > (size_t)(end - p)) fail(SVM_ERR_INVALID, "PipelineInfo: truncated fixed field");
     p += n;
@@ -68,6 +80,7 @@ ColumnInfo read_column(Reader r) {
 
 ObjectInfo read_object(Reader r) {
   ObjectInfo o;
+  o.column_list.reserve(r.count(3));
   while (!r.done()) {
     uint64_t tag = r.varint();
     int f = (int)(tag >> 3), wt = (int)(tag & 7);
@@ -83,11 +96,15 @@ ObjectInfo read_object(Reader r) {
 
 OperatorInfo read_operator(Reader r) {
   OperatorInfo op;
+  bool typed = false;
   while (!r.done()) {
     uint64_t tag = r.varint();
     int f = (int)(tag >> 3), wt = (int)(tag & 7);
     if (f == 1 && wt == 2) op.id = r.str();
-    else if (f == 2 && wt == 0) op.type = (int)r.varint();
+    else if (f == 2 && wt == 0) {
+      op.type = (int)r.varint();
+      typed = true;
+    }
     else if (f == 4 && wt == 2) {  // map<string,string> entry {1: key, 2: value}
       Reader e = r.sub();
       std::string k, v;
@@ -100,8 +117,18 @@ OperatorInfo read_operator(Reader r) {
       }
       op.args[k] = v;  // later entries win, as protobuf map parsing does
     } else if (f == 5 && wt == 2) {
-      op.output = read_object(r.sub());
+      Reader o = r.sub();
       op.has_output = true;
+      // decoded when the operator's type (field 2, written first by proto3 encoders)
+      // makes the schema one the plan reads, or is not known yet
+      if (!typed || op.type == OpSource || op.type == OpLoader || op.type == OpSink) {
+        op.output = read_object(o);
+        op.output_decoded = true;
+        op.output_raw.clear();
+      } else {
+        op.output_raw.assign((const char*)o.p, (size_t)(o.end - o.p));
+        op.output_decoded = false;
+      }
     } else if (f == 6 && wt == 0) op.perf = (int)r.varint();
     else r.skip(wt);  // field 3 (DependencyInfo) is not used by the data plane
   }
@@ -152,6 +179,7 @@ PipelineInfo decode_pipeline_info(const uint8_t* data, size_t len) {
   if (!data && len) fail(SVM_ERR_INVALID, "PipelineInfo: null buffer");
   Reader r{data, data + len};
   PipelineInfo p;
+  p.workflow.reserve(r.count(2));
   while (!r.done()) {
     uint64_t tag = r.varint();
     int f = (int)(tag >> 3), wt = (int)(tag & 7);
@@ -179,7 +207,7 @@ std::string encode_pipeline_info(const PipelineInfo& p) {
       put_bytes(e, 2, kv.second);
       put_bytes(o, 4, e);
     }
-    if (op.has_output) put_bytes(o, 5, enc_object(op.output));
+    if (op.has_output) put_bytes(o, 5, op.output_decoded ? enc_object(op.output) : op.output_raw);
     put_uint(o, 6, (uint64_t)op.perf);
     put_bytes(s, 2, o);
   }

@@ -48,8 +48,11 @@ struct OperatorInfo {
   std::string id;
   int type = 0;
   std::map<std::string, std::string> args;
+  // The output schema is decoded for Source, Loader and Sink operators, the only ones
+  // whose schema the data plane reads; the others keep its bytes (output_raw).
   ObjectInfo output;
-  bool has_output = false;
+  std::string output_raw;
+  bool has_output = false, output_decoded = false;
   int perf = 0;
 };
 struct PipelineInfo {

@@ -88,15 +88,42 @@ def table_view(id, base_id, row0, rows):
 def batch_run(plans, want_result=True):
     """svm_batch_run over plan dicts; returns (results list, number fused).
     Absorbed dimension pipelines give None."""
-    datas = [_bytes(p) for p in plans]
-    n = len(datas)
-    bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
-    arr = (ctypes.POINTER(ctypes.c_uint8) * n)(*[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
-    lens = (ctypes.c_size_t * n)(*[len(d) for d in datas])
+    arr, lens, keep = _plan_array(plans)
+    n = len(keep[2])
     outs = (ctypes.c_void_p * n)()
     fused = _lib.check(_lib.lib().svm_batch_run(n, arr, lens, outs if want_result else None))
-    res = [(_take_result(outs[i]) if outs[i] else None) for i in range(n)] if want_result else None
-    return res, fused
+    return (_take_results(outs, n) if want_result else None), fused
+
+
+def _take_results(outs, n):
+    """Results of a batch as numpy arrays (None where outs[i] is NULL), read with one
+    describe, one copy and one free call for the whole batch."""
+    L = _lib.lib()
+    try:
+        descs = (_lib.SvmResultDesc * n)()
+        _lib.check(L.svm_results_describe(n, outs, descs))
+        a = np.frombuffer(descs, dtype=np.dtype(_lib.SvmResultDesc))
+        idx = np.flatnonzero(a["kind"] >= 0)
+        d = a[idx]
+        mc = int(d["ncols"].max()) if len(idx) else 0
+        # bytes of each column (0 past a result's last column); each starts at a
+        # multiple of 8 in the copy, results in order, columns in order
+        nb = np.where(np.arange(mc)[None, :] < d["ncols"][:, None], d["rows"][:, None] * d["val_size"][:, :mc], 0)
+        padded = (nb + 7) & ~7
+        offs = (np.cumsum(padded.ravel()) - padded.ravel()).reshape(padded.shape).tolist()
+        total = int(padded.sum())
+        buf = np.empty(total + 8, np.uint8)
+        _lib.check(L.svm_results_copy(n, outs, buf.ctypes.data, total + 8))
+        kinds, ncols, rows = d["kind"].tolist(), d["ncols"].tolist(), d["rows"].tolist()
+        types, sizes, nb = d["type"][:, :mc].tolist(), d["val_size"][:, :mc].tolist(), nb.tolist()
+        bms, sms, fms = d["build_ms"].tolist(), d["scan_ms"].tolist(), d["finish_ms"].tolist()
+        res = [None] * n
+        for j, i in enumerate(idx.tolist()):
+            cols = [_view(buf[offs[j][c]:offs[j][c] + nb[j][c]], types[j][c], sizes[j][c]) for c in range(ncols[j])]
+            res[i] = Result(_KINDS[kinds[j]], rows[j], cols, bms[j], sms[j], fms[j])
+        return res
+    finally:
+        L.svm_results_free(n, outs)
 
 
 def _bytes(plan):
@@ -105,12 +132,18 @@ def _bytes(plan):
 
 
 def _plan_array(plans):
+    """(pointer array, length array, objects to keep alive) for a list of plans: the
+    bytes go into one buffer, and the pointers are computed with numpy."""
     datas = [_bytes(p) for p in plans]
-    bufs = [(ctypes.c_uint8 * len(d)).from_buffer_copy(d) for d in datas]
-    arr = (ctypes.POINTER(ctypes.c_uint8) * len(datas))(
-        *[ctypes.cast(b, ctypes.POINTER(ctypes.c_uint8)) for b in bufs])
-    lens = (ctypes.c_size_t * len(datas))(*[len(d) for d in datas])
-    return arr, lens, bufs
+    n = len(datas)
+    blob = np.frombuffer(b"".join(datas) + b"\0", np.uint8)
+    lens = np.fromiter((len(d) for d in datas), np.uint64, n)
+    addrs = np.zeros(n, np.uint64)
+    if n > 1:
+        np.cumsum(lens[:-1], out=addrs[1:])
+    addrs += np.uint64(blob.ctypes.data)
+    arr = addrs.ctypes.data_as(ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)))
+    return arr, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), (blob, addrs, datas, lens)
 
 
 def server_start(workers=2, budget=0, max_batch=64, mode=0):
@@ -212,26 +245,25 @@ class Result:
     finish_ms: float
 
 
+_KINDS = {0: "select", 1: "group", 2: "scalar"}
+
+
 def _take_result(ptr) -> Result:
+    """Copy a result to numpy arrays and free it (svm_result_describe: one call)."""
     L = _lib.lib()
     try:
-        rows = int(L.svm_result_rows(ptr))
-        ncols = int(L.svm_result_cols(ptr))
-        cols, types = [], []
-        for c in range(ncols):
-            t, s = ctypes.c_int32(), ctypes.c_int64()
-            _lib.check(L.svm_result_col_type(ptr, c, ctypes.byref(t), ctypes.byref(s)))
-            nbytes = rows * s.value
-            addr = L.svm_result_col(ptr, c)
-            buf = np.zeros(nbytes, np.uint8)
-            if nbytes and addr:
-                ctypes.memmove(buf.ctypes.data, addr, nbytes)
-            cols.append(_view(buf, t.value, s.value))
-            types.append(t.value)
-        b, sc, f = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        _lib.check(L.svm_result_timing(ptr, ctypes.byref(b), ctypes.byref(sc), ctypes.byref(f)))
-        kind = {0: "select", 1: "group", 2: "scalar"}[int(L.svm_result_kind(ptr))]
-        return Result(kind, rows, cols, b.value, sc.value, f.value)
+        d = _lib.SvmResultDesc()
+        _lib.check(L.svm_result_describe(ptr, ctypes.byref(d)))
+        rows = int(d.rows)
+        cols = []
+        for c in range(d.ncols):
+            t, s = int(d.type[c]), int(d.val_size[c])
+            nbytes = rows * s
+            buf = np.empty(nbytes, np.uint8)
+            if nbytes and d.col[c]:
+                ctypes.memmove(buf.ctypes.data, d.col[c], nbytes)
+            cols.append(_view(buf, t, s))
+        return Result(_KINDS[int(d.kind)], rows, cols, d.build_ms, d.scan_ms, d.finish_ms)
     finally:
         L.svm_result_free(ptr)
 
