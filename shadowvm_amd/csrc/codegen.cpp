@@ -57,6 +57,33 @@ __device__ __forceinline__ unsigned svm_map_get(const unsigned* m, unsigned off)
   return (m[off >> S] >> ((off & (unsigned)(E - 1)) * (unsigned)W)) & ((1u << W) - 1u);
 }
 
+/* Probe<T> (reference com/cpp/cpu-runtime.hpp:125-138): walk at most 10 slots from
+   Hash(key bytes) % slots; -1 at an empty slot, the row whose key equals, -1 after 10. */
+template <typename T>
+__device__ __forceinline__ int svm_probe(const T* key, int key_size, const T* tab, const int* ht, unsigned slots) {
+  unsigned idx = Hash((const char*)key, key_size) % slots;
+  for (int t = 0; t < 10; ++t) {
+    const int pos = ht[idx];
+    if (pos < 0) return -1;
+    if (tab[pos] == *key) return pos;
+    idx = idx + 1u == slots ? 0u : idx + 1u;
+  }
+  return -1;
+}
+/* Probe<char> (:140-152): StrHash home, StrCmp (prefix) equality, and row 0 (not -1)
+   after 10 slots. */
+__device__ __forceinline__ int svm_probe_str(const char* key, int key_size, const char* tab, const int* ht,
+                                             unsigned slots) {
+  unsigned idx = StrHash(key, key_size) % slots;
+  for (int t = 0; t < 10; ++t) {
+    const int pos = ht[idx];
+    if (pos < 0) return -1;
+    if (StrCmp(key, tab + (long long)pos * key_size, key_size) == 0) return pos;
+    idx = idx + 1u == slots ? 0u : idx + 1u;
+  }
+  return 0;
+}
+
 #define PI(k) (a.pi[k])
 #define PS(off) ((const char*)(a.sblob + (off)))
 )SVMRT";
@@ -79,6 +106,31 @@ void emit_colptrs(std::ostringstream& o, const KernelShape& s) {
   for (auto& kv : s.dcol_slot)
     o << "  const char* DCOL" << kv.first.first << "_" << kv.first.second
       << " = (const char*)a.ptr[" << kv.second << "];\n";
+}
+
+const char* key_ctype(int t) { return t == 1 ? "short" : t == 3 ? "long long" : t == 0 ? "char" : "int"; }
+
+// Statements computing `int <var>` = the dimension row a probe join finds for source
+// row i (Probe<T> / Probe<char> on the reference's hash table in a.ptr[map slot],
+// a.iv[4j+1] slots), as GenJoinBlock emits it (codegen/codegen.cpp:137-141): the key
+// is read at the dimension key's width.
+std::string probe_stmt(const JoinShape& J, size_t j, const std::string& var) {
+  std::ostringstream o;
+  const std::string ht = "(const int*)a.ptr[" + I(SVM_SLOT_MAP + (int)j) + "]";
+  const std::string slots = "(unsigned)a.iv[" + I(4 * (int)j + 1) + "]";
+  const std::string tab = "DCOL" + I(J.robj) + "_" + I(J.rcol);
+  if (J.kind == JOIN_PROBE_STR) {
+    o << "const int " << var << " = svm_probe_str((const char*)(FCOL" << J.lcol << " + i * " << J.ksize << "ll), "
+      << J.ksize << ", " << tab << ", " << ht << ", " << slots << ");";
+  } else if (J.ktype == 2) {  // Int32 keys are held in registers
+    o << "const int " << var << "k = f" << J.lcol << "; const int " << var << " = svm_probe<int>(&" << var
+      << "k, 4, (const int*)" << tab << ", " << ht << ", " << slots << ");";
+  } else {
+    const char* ct = key_ctype(J.ktype);
+    o << "const int " << var << " = svm_probe<" << ct << ">((const " << ct << "*)(FCOL" << J.lcol << " + i * "
+      << J.ksize << "ll), " << J.ksize << ", (const " << ct << "*)" << tab << ", " << ht << ", " << slots << ");";
+  }
+  return o.str();
 }
 
 void emit_map_ptrs(std::ostringstream& o, const KernelShape& s) {
@@ -109,6 +161,17 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
   o << ind << "if (cond) cond = (" << s.filter << ");\n";
   for (size_t j = 0; j < s.joins.size(); ++j) {
     const JoinShape& J = s.joins[j];
+    if (J.kind != JOIN_MAP) {
+      // p = row + 1, so 0 still means "no row"
+      o << ind << "unsigned p" << j << " = 0u;\n"
+        << ind << "if (cond) {\n"
+        << ind << "  " << probe_stmt(J, j, "r" + I((int)j)) << "\n"
+        << ind << "  p" << j << " = (unsigned)(r" << j << " + 1);\n"
+        << ind << "  cond = r" << j << " >= 0;\n"
+        << ind << "}\n"
+        << ind << "const long long JI" << J.robj << " = (long long)p" << j << " - 1;\n";
+      continue;
+    }
     o << ind << "unsigned p" << j << " = 0u;\n"
       << ind << "if (cond) {\n"
       << ind << "  const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
@@ -237,8 +300,11 @@ std::string gen_scan_source(const KernelShape& s) {
           << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
     }
   }
+  // the flat form holds dense-map lookups only; probe joins use the per-row form
+  bool flat = s.flat;
+  for (auto& J : s.joins) flat = flat && J.kind == JOIN_MAP;
   for (int u = 0; u < U; ++u) {
-    if (!s.flat) {
+    if (!flat) {
     o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
       << "      const long long i = v" << u << "_ * 4 + e;\n";
     for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
@@ -372,6 +438,11 @@ std::string gen_select_source(const KernelShape& s) {
       for (size_t j = 0; j < s.joins.size(); ++j) {
         const JoinShape& J = s.joins[j];
         if (J.payload != PAY_ROW) continue;
+        if (J.kind != JOIN_MAP) {
+          o << "      " << probe_stmt(J, j, "rj" + I((int)j)) << "\n"
+            << "      const long long JI" << J.robj << " = rj" << j << ";\n";
+          continue;
+        }
         o << "      const unsigned offj" << j << " = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
           << "      const long long JI" << J.robj << " = (long long)svm_map_get<" << J.wbits << ">(m" << j
           << ", offj" << j << ") - 1;\n";

@@ -14,12 +14,23 @@ enum PayloadKind {
   PAY_ROW = 2     // row+1 of the first dimension row with this key
 };
 
+// How a join finds its dimension row.
+enum JoinKind {
+  JOIN_MAP = 0,    // Int32 keys: dense map over the key range
+  JOIN_PROBE = 1,  // the reference's hash table, Probe<T> over the key's bytes
+  JOIN_PROBE_STR = 2  // the reference's hash table, Probe<char> (StrHash, StrCmp)
+};
+
 struct JoinShape {
   int robj = 1;      // ref object (D4: join_index is addressed by this object)
-  int lcol = 0;      // source (fact) column holding the foreign key (Int32)
+  int lcol = 0;      // source (fact) column holding the foreign key
   int payload = PAY_EXIST;
   int wbits = 1;     // 1,2,4,8,16 or 32 bits per key slot
   bool lds = false;  // map staged in LDS at kernel start
+  int kind = JOIN_MAP;
+  int rcol = 0;      // probes: the dimension's key column (read as DCOL<robj>_<rcol>)
+  int ktype = 2;     // probes: key type (ValType) and width in bytes
+  int ksize = 4;
 };
 
 enum KeySource { KEY_DIMCODE = 0, KEY_DIMROW = 1, KEY_FACT = 2 };

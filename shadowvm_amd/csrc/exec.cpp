@@ -68,6 +68,8 @@ struct JoinBuild {
   DevBuf packed;
   uint32_t lds_off = 0;
   uint64_t dropped = 0;  // inserts the reference's hash table drops
+  const int32_t* table = nullptr;  // probe joins: the reference's hash table (slots entries)
+  uint64_t slots = 0;
 };
 
 struct KeyBuild {
@@ -155,24 +157,39 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     if (!seen.insert(R.obj).second) fail(SVM_ERR_UNSUPPORTED, "two JoinWith on the same ref object");
     if (L.col >= (int)c.schemas[0].size() || R.col >= (int)c.schemas[R.obj].size())
       fail(SVM_ERR_INVALID, "JoinWith key column out of range");
-    if (c.schemas[0][L.col].first != Int32 || c.schemas[R.obj][R.col].first != Int32)
-      fail(SVM_ERR_UNSUPPORTED, "join keys must be Int32 columns");
+    const auto lt = c.schemas[0][L.col], rt = c.schemas[R.obj][R.col];
+    // BuildHashTable is instantiated for these key types (core/pipeline.cu:97-129;
+    // Byte keys get no table there)
+    if (rt.first != Int16 && rt.first != Int32 && rt.first != Int64 && rt.first != String)
+      fail(SVM_ERR_UNSUPPORTED, "join keys must be Int16, Int32, Int64 or String columns");
+    // Probe reads the lkey at the rkey's type and width (codegen/codegen.cpp:137-141)
+    if (lt != rt) fail(SVM_ERR_UNSUPPORTED, "JoinWith lkey and rkey must have the same type and width");
     const Table& D = *c.refs[R.obj - 1];
+    const TablePtr& Dp = c.refs[R.obj - 1];
     // only bulk 0 of the dimension is joined (core/pipeline.cu:91,294-295)
     const uint64_t n0 = D.bulk0_rows();
     JoinBuild jb;
     jb.rcol = R.col;
     jb.shape.robj = R.obj;
     jb.shape.lcol = L.col;
+    // Int32 keys within the dense-map range join through a dense map; other keys probe
+    // the reference's hash table itself
+    auto root = column_root(Dp, R.col);
+    KeyEntry ke;
+    uint64_t range = 0;
+    if (rt.first == Int32 && n0) {
+      ke = table_key(root.first, root.second, c.s);
+      range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
+    }
+    const bool probe = rt.first != Int32 || range > kMaxDenseRange;
     // payload selection
     std::vector<int> comps;
     for (size_t q = 0; q < keys.size(); ++q)
       if (keys[q].shape.src != KEY_FACT && keys[q].shape.obj == R.obj) comps.push_back((int)q);
     bool refs = false;
     for (auto& o : expr_refs) refs = refs || o.first == R.obj;
-    int payload = refs || comps.size() > 1 ? PAY_ROW : comps.size() == 1 ? PAY_CODE : PAY_EXIST;
+    int payload = refs || comps.size() > 1 || probe ? PAY_ROW : comps.size() == 1 ? PAY_CODE : PAY_EXIST;
     // key dictionaries of this dimension
-    const TablePtr& Dp = c.refs[R.obj - 1];
     DevBuf codes_for_pack;
     for (int q : comps) {
       KeyBuild& kb = keys[q];
@@ -183,17 +200,23 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     jb.shape.payload = payload;
     uint64_t maxval = payload == PAY_EXIST ? 1 : payload == PAY_CODE ? keys[comps[0]].n : D.rows;
     jb.shape.wbits = payload == PAY_ROW ? 32 : width_for(maxval);
-    // dense map over the key range of the column's origin (a superset of this
-    // dimension object's keys, cached per resident table: no host round trip)
     const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
-    auto root = column_root(Dp, R.col);
-    KeyEntry ke;
+    if (probe) {
+      jb.shape.kind = rt.first == String ? JOIN_PROBE_STR : JOIN_PROBE;
+      jb.shape.rcol = R.col;
+      jb.shape.ktype = rt.first;
+      jb.shape.ksize = (int)rt.second;
+      jb.table = join_table(Dp, R.col, &jb.slots, &jb.dropped, c.s);
+      if (jb.dropped && c.E.cfg.verbose)
+        fprintf(stderr, "[svm] %s: the hash table on '%s' drops %llu of %llu inserts\n", c.info.id.c_str(),
+                D.id.c_str(), (unsigned long long)jb.dropped, (unsigned long long)n0);
+      joins.push_back(std::move(jb));
+      continue;
+    }
+    // dense map over the key range of the column's origin (a superset of this
+    // dimension object's keys, cached per resident table: no host round trip)
     if (n0) {
-      ke = table_key(root.first, root.second, c.s);
-      uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
-      if (range > kMaxDenseRange)
-        fail(SVM_ERR_UNSUPPORTED, "join key range " + std::to_string(range) + " of '" + D.id + "' exceeds the dense map limit");
       jb.kmin = ke.kmin;
       jb.range = (uint32_t)range;
     }
@@ -250,6 +273,11 @@ void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
   // source columns by their base address; the kernels scan rows [row0, row0 + rows)
   for (size_t q = 0; q < c.src->cols.size() && q < 24; ++q) a.ptr[q] = c.src->cols[q].dev;
   for (size_t j = 0; j < joins.size(); ++j) {
+    if (joins[j].table) {  // probe: the hash table and its size
+      a.ptr[SVM_SLOT_MAP + j] = joins[j].table;
+      a.iv[4 * j + 1] = (int)(uint32_t)joins[j].slots;
+      continue;
+    }
     a.ptr[SVM_SLOT_MAP + j] = joins[j].packed.p;
     a.iv[4 * j + 0] = joins[j].kmin;
     a.iv[4 * j + 1] = (int)joins[j].range;
@@ -290,7 +318,10 @@ void classify_fact_columns(Ctx& c, const std::set<std::pair<int, int>>& ops, Ker
     if (c.schemas[0][o.second].first == Int32) i32.insert(o.second);
     else other.insert(o.second);
   }
-  for (auto& jb : ks.joins) i32.insert(jb.lcol);
+  for (auto& jb : ks.joins) {
+    if (jb.kind == JOIN_MAP || jb.ktype == Int32) i32.insert(jb.lcol);
+    else other.insert(jb.lcol);
+  }
   for (auto& k : ks.keys)
     if (k.src == KEY_FACT) i32.insert(k.col);
   ks.fact_i32.assign(i32.begin(), i32.end());
@@ -426,6 +457,9 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     if (!joined) fail(SVM_ERR_INVALID, "metric reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
   }
   std::vector<JoinBuild> joins = build_joins(c, metric_ops, keys);
+  // probe joins compare against the dimension's key column
+  for (auto& jb : joins)
+    if (jb.table) metric_ops.insert({jb.shape.robj, jb.shape.rcol});
   const double t_build_host = host_ms(t_host0);
   for (auto& jb : joins) ks.joins.push_back(jb.shape);
   // source-column keys: code = value - min over the column
@@ -673,6 +707,9 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   }
   std::vector<KeyBuild> nokeys;
   std::vector<JoinBuild> joins = build_joins(c, proj_ops, nokeys);
+  // probe joins compare against the dimension's key column
+  for (auto& jb : joins)
+    if (jb.table) proj_ops.insert({jb.shape.robj, jb.shape.rcol});
   for (auto& jb : joins) ks.joins.push_back(jb.shape);
   ks.dcol_slot = assign_dcols(proj_ops);
   classify_fact_columns(c, ops, ks);

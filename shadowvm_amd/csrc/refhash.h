@@ -37,6 +37,14 @@ uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* ke
 bool ht_subset_may_drop(const HtClusters& hc, const int32_t* keys, uint64_t n, int32_t kmin, uint64_t range,
                         hipStream_t s);
 
+// The reference's hash table itself (BuildHashTable, com/cpp/com.hpp:168-174) over n
+// keys of `size` bytes each (row r at col + r*size), hashed with Hash over all bytes or,
+// for strings, StrHash up to the first NUL (InsertHashTable<char>,
+// com/cpp/cpu-runtime.hpp:92-103): table[slot] = row, -1 for an empty slot.  Returns
+// the number of dropped inserts (synchronises).
+uint64_t ht_build(const char* col, int64_t size, bool str, uint64_t n, uint64_t slots, int32_t* table,
+                  hipStream_t s);
+
 // Widest cluster (in slots) replayed in registers by the subset kernels.
 constexpr int kReplaySpan = 256;
 

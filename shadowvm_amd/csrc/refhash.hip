@@ -64,6 +64,22 @@ __global__ void k_home(const int32_t* __restrict__ keys, uint32_t n, uint32_t sl
   }
 }
 
+// Home slots of keys of `size` bytes each (row r at col + r*size): Hash over all
+// bytes, or for strings StrHash up to the first NUL (com/cpp/cpu-runtime.hpp:47-66).
+__global__ void k_home_bytes(const char* __restrict__ col, int64_t size, int str, uint32_t n, uint32_t slots,
+                             uint32_t* __restrict__ home, uint32_t* __restrict__ row) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const char* p = col + (uint64_t)r * (uint64_t)size;
+    uint32_t h = 0;
+    for (int64_t k = 0; k < size; ++k) {
+      if (str && p[k] == '\0') break;
+      h = h * 31u + (uint32_t)(int32_t)(signed char)p[k];
+    }
+    home[r] = (h & 0x7FFFFFFFu) % slots;
+    row[r] = r;
+  }
+}
+
 // head[i] = 1 where a new cluster starts in home order
 __global__ void k_heads(const uint32_t* __restrict__ hs, uint32_t n, uint32_t* __restrict__ head) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -88,7 +104,8 @@ __global__ void k_cluster_of(const uint32_t* __restrict__ hs, const uint32_t* __
 // with `risky`, it also marks every entry of a cluster that drops.
 __global__ void k_replay(const uint32_t* __restrict__ cl, const uint32_t* __restrict__ rows,
                          const uint32_t* __restrict__ home, uint32_t n, uint32_t slots, uint32_t* occ,
-                         uint8_t* __restrict__ kept, uint8_t* __restrict__ risky, unsigned long long* dropped) {
+                         uint8_t* __restrict__ kept, uint8_t* __restrict__ risky, int32_t* __restrict__ table,
+                         unsigned long long* dropped) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t c = cl[i];
     if (i > 0 && cl[i - 1] == c) continue;
@@ -102,6 +119,7 @@ __global__ void k_replay(const uint32_t* __restrict__ cl, const uint32_t* __rest
         const uint32_t bit = 1u << (s & 31u);
         if (!(atomicOr(&occ[s >> 5], bit) & bit)) {
           placed = true;
+          if (table) table[s] = (int32_t)r;
           break;
         }
         s = s + 1u == slots ? 0u : s + 1u;
@@ -216,17 +234,14 @@ int bits_of(uint64_t v) {
 
 }  // namespace
 
-uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, HtClusters* out, hipStream_t s) {
-  if (out) *out = HtClusters();
-  if (!n) return 0;
-  if (slots == 0 || slots > 0xFFFFFFFFull) fail(SVM_ERR_INVALID, "hash table size out of range");
-  if (n > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "more than 2^31 rows in one bulk");
+namespace {
+
+// The replay of inserts whose home slots are home[0..n) (row ids 0..n-1 in `row`).
+uint64_t replay_homes(const int32_t* keys, DevBuf& home, DevBuf& row, uint64_t n, uint64_t slots, uint8_t* kept,
+                      HtClusters* out, int32_t* table, hipStream_t s) {
   const uint32_t n32 = (uint32_t)n, sl = (uint32_t)slots;
-  DevBuf home(n * 4 + 4, s), row(n * 4 + 4, s), hs(n * 4 + 4, s), rs(n * 4 + 4, s), incl(n * 4 + 4, s),
-      cl(n * 4 + 4, s), cl2(n * 4 + 4, s), rows2(n * 4 + 4, s), occ(((uint64_t)sl + 31) / 32 * 4 + 4, s),
-      drop(8, s), risky(out ? n + 16 : 16, s);
-  k_home<<<grid_for(n), kBlock, 0, s>>>(keys, n32, sl, home.as<uint32_t>(), row.as<uint32_t>());
-  hip_check(hipGetLastError(), "ht_analyse home");
+  DevBuf hs(n * 4 + 4, s), rs(n * 4 + 4, s), incl(n * 4 + 4, s), cl(n * 4 + 4, s), cl2(n * 4 + 4, s),
+      rows2(n * 4 + 4, s), occ(((uint64_t)sl + 31) / 32 * 4 + 4, s), drop(8, s), risky(out ? n + 16 : 16, s);
   // sort by home slot; rocPRIM's radix sort is stable, so equal homes stay in row order
   const int hbits = bits_of(slots - 1);
   const int cbits = bits_of(n);
@@ -257,8 +272,9 @@ uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* ke
   fill_u32(occ.p, ((uint64_t)sl + 31) / 32, 0u, s);
   fill_u32(drop.p, 2, 0u, s);
   if (out) fill_u32(risky.p, (n + 3) / 4, 0u, s);
+  if (table) fill_u32(table, sl, 0xFFFFFFFFu, s);
   k_replay<<<grid_for(n), kBlock, 0, s>>>(cl2.as<uint32_t>(), rows2.as<uint32_t>(), home.as<uint32_t>(), n32, sl,
-                                          occ.as<uint32_t>(), kept, out ? risky.as<uint8_t>() : nullptr,
+                                          occ.as<uint32_t>(), kept, out ? risky.as<uint8_t>() : nullptr, table,
                                           drop.as<unsigned long long>());
   hip_check(hipGetLastError(), "ht_analyse replay");
   unsigned long long d = 0;
@@ -311,6 +327,37 @@ uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* ke
   out->max_rows = st[0];
   out->max_span = st[1];
   return d;
+}
+
+void check_sizes(uint64_t n, uint64_t slots) {
+  if (slots == 0 || slots > 0xFFFFFFFFull) fail(SVM_ERR_INVALID, "hash table size out of range");
+  if (n > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "more than 2^31 rows in one bulk");
+}
+
+}  // namespace
+
+uint64_t ht_analyse(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, HtClusters* out, hipStream_t s) {
+  if (out) *out = HtClusters();
+  if (!n) return 0;
+  check_sizes(n, slots);
+  DevBuf home(n * 4 + 4, s), row(n * 4 + 4, s);
+  k_home<<<grid_for(n), kBlock, 0, s>>>(keys, (uint32_t)n, (uint32_t)slots, home.as<uint32_t>(), row.as<uint32_t>());
+  hip_check(hipGetLastError(), "ht_analyse home");
+  return replay_homes(keys, home, row, n, slots, kept, out, nullptr, s);
+}
+
+uint64_t ht_build(const char* col, int64_t size, bool str, uint64_t n, uint64_t slots, int32_t* table,
+                  hipStream_t s) {
+  check_sizes(n, slots);
+  if (!n) {
+    fill_u32(table, slots, 0xFFFFFFFFu, s);
+    return 0;
+  }
+  DevBuf home(n * 4 + 4, s), row(n * 4 + 4, s);
+  k_home_bytes<<<grid_for(n), kBlock, 0, s>>>(col, size, str ? 1 : 0, (uint32_t)n, (uint32_t)slots,
+                                              home.as<uint32_t>(), row.as<uint32_t>());
+  hip_check(hipGetLastError(), "ht_build home");
+  return replay_homes(nullptr, home, row, n, slots, nullptr, nullptr, table, s);
 }
 
 bool ht_subset_may_drop(const HtClusters& hc, const int32_t* keys, uint64_t n, int32_t kmin, uint64_t range,

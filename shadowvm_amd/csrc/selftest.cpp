@@ -81,6 +81,39 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.fact_other = {2};
     out.push_back({"scan_row_flat", gen_scan_source(s)});
   }
+  {  // probe joins on the reference's hash table: an Int64 key and an Int32 key, grouped
+    KernelShape s;
+    s.fact_i32 = {1, 3};
+    s.fact_other = {0};
+    JoinShape a; a.robj = 1; a.lcol = 0; a.payload = PAY_ROW; a.wbits = 32;
+    a.kind = JOIN_PROBE; a.rcol = 0; a.ktype = Int64; a.ksize = 8;
+    JoinShape b; b.robj = 2; b.lcol = 1; b.payload = PAY_ROW; b.wbits = 32;
+    b.kind = JOIN_PROBE; b.rcol = 0; b.ktype = Int32; b.ksize = 4;
+    s.joins = {a, b};
+    s.dcol_slot[{1, 0}] = 32;
+    s.dcol_slot[{2, 0}] = 33;
+    KeyShape k0; k0.src = KEY_DIMROW; k0.join = 1; k0.obj = 2; k0.col = 1;
+    s.keys = {k0};
+    s.metrics = {"(f3)"};
+    s.metric_ctype = {"int"};
+    s.agg = AGG_HASH;
+    s.flat = true;  // falls back to the per-row form
+    out.push_back({"scan_probe", gen_scan_source(s)});
+  }
+  {  // select with a string-key probe join (Probe<char>) projecting dimension strings
+    KernelShape s;
+    s.fact_other = {2, 4};
+    s.filter = "StrCmp(((const char*)(FCOL4+(long long)i*10)),PS(0),PI(0))==PI(1)";
+    JoinShape a; a.robj = 1; a.lcol = 2; a.payload = PAY_ROW; a.wbits = 32;
+    a.kind = JOIN_PROBE_STR; a.rcol = 0; a.ktype = String; a.ksize = 40;
+    s.joins = {a};
+    s.dcol_slot[{1, 0}] = 32;
+    s.dcol_slot[{1, 1}] = 33;
+    s.proj = {"((const char*)(DCOL1_1+(long long)JI1*40))", "((const char*)(DCOL1_0+(long long)JI1*40))",
+              "((const char*)(FCOL2+(long long)i*40))"};
+    s.proj_type = {{String, 40}, {String, 40}, {String, 40}};
+    out.push_back({"select_probe_str", gen_select_source(s)});
+  }
   {  // select with a string filter and projections, plus a row-id join
     KernelShape s;
     s.fact_i32 = {0, 2};

@@ -22,6 +22,22 @@ struct KeptEntry {
 // (table, column, rows inserted, slots)
 std::map<std::tuple<const Table*, int, uint64_t, uint64_t>, KeptEntry> g_kept;
 
+struct TableEntry {
+  std::weak_ptr<Table> owner;
+  uint64_t dropped = 0;
+  DevBuf table;  // slots int32
+};
+std::map<std::tuple<const Table*, int, uint64_t, uint64_t>, TableEntry> g_tables;
+
+// Entries of tables that no longer exist (a served query's dimension objects come
+// and go) are removed once a cache grows past a few hundred entries.
+template <typename M>
+void prune_dead(M& m) {
+  if (m.size() < 256) return;
+  for (auto it = m.begin(); it != m.end();)
+    it = it->second.owner.expired() ? m.erase(it) : std::next(it);
+}
+
 struct ClusterEntry {
   std::weak_ptr<Table> owner;
   std::shared_ptr<const k::HtClusters> clusters;  // null: nothing dropped
@@ -134,6 +150,7 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto it = g_kept.find(key);
   if (it == g_kept.end() || it->second.owner.lock() != t) {
+    prune_dead(g_kept);
     g_kept[key] = std::move(e);
     it = g_kept.find(key);
   }
@@ -154,8 +171,43 @@ std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int 
   auto hc = std::make_shared<k::HtClusters>();
   k::ht_analyse((const int32_t*)t->col_ptr(col), t->rows, slots, nullptr, hc.get(), s);
   if (hc->dropped) e.clusters = hc;
+  prune_dead(g_clusters);
   g_clusters[key] = e;
   return e.clusters;
+}
+
+const int32_t* join_table(const TablePtr& t, int col, uint64_t* slots, uint64_t* dropped, hipStream_t s) {
+  const Column& c = t->cols[col];
+  if (!t->col_ptr(col)) fail(SVM_ERR_INVALID, "join key column of '" + t->id + "' is not materialised");
+  if (c.type != Int16 && c.type != Int32 && c.type != Int64 && c.type != String)
+    fail(SVM_ERR_UNSUPPORTED, "join keys must be Int16, Int32, Int64 or String columns");
+  const uint64_t n = t->bulk0_rows();
+  // the reference passes HashRate * max_tuple_count_ as an unsigned int
+  // (com/cpp/com.hpp:169, core/pipeline.cu:105-124)
+  *slots = (uint64_t)(uint32_t)(kHashRate * (t->max_tuple_count ? t->max_tuple_count : n));
+  if (!*slots) fail(SVM_ERR_UNSUPPORTED, "join hash table of '" + t->id + "' has 0 slots (HashRate x bulk capacity wraps)");
+  const auto key = std::make_tuple(t.get(), col, n, *slots);
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto it = g_tables.find(key);
+    if (it != g_tables.end() && it->second.owner.lock() == t) {
+      *dropped = it->second.dropped;
+      return it->second.table.as<int32_t>();
+    }
+  }
+  TableEntry e;
+  e.owner = t;
+  e.table = DevBuf(*slots * 4 + 16);  // cached past this run: not from the stream's arena
+  e.dropped = k::ht_build(t->col_ptr(col), c.size, c.type == String, n, *slots, e.table.as<int32_t>(), s);
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  auto it = g_tables.find(key);
+  if (it == g_tables.end() || it->second.owner.lock() != t) {
+    prune_dead(g_tables);
+    g_tables[key] = std::move(e);
+    it = g_tables.find(key);
+  }
+  *dropped = it->second.dropped;
+  return it->second.table.as<int32_t>();
 }
 
 uint64_t pipeline_ht_slots(const Engine& E) { return (uint64_t)(uint32_t)(kHashRate * E.cfg.tuple_per_bulk); }

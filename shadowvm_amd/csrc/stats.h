@@ -43,6 +43,12 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
 std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int col, uint64_t slots,
                                                          hipStream_t s);
 
+// The reference's hash table over bulk 0 of column `col` of `t` (Int16/Int32/Int64
+// keys hashed over their bytes, String keys with StrHash), HashRate x max_tuple_count
+// slots: table[slot] = row or -1, valid while `t` lives.  *slots and *dropped receive
+// its size and the inserts it dropped.  Cached per table.
+const int32_t* join_table(const TablePtr& t, int col, uint64_t* slots, uint64_t* dropped, hipStream_t s);
+
 // Slots of the hash table on a dimension object made by a pipeline (a bulk of
 // tuple_per_bulk rows): HashRate x tuple_per_bulk, as the reference's unsigned int.
 uint64_t pipeline_ht_slots(const Engine& E);
