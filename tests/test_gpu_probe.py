@@ -185,3 +185,36 @@ def test_string_probe_quirks(engine):
         assert nothing not in joined
         assert got.rows == want.rows
         assert_same_columns(got.columns, want.columns)
+
+
+def test_yahoo_from_files(engine, tmp_path):
+    """The whole of test/test-yahoo.cu: both tables loaded from '|'-separated files by
+    Loader pipelines (object/object.cpp:122-169), then the filtered string-key join and
+    projection, against the oracle's loader and query."""
+    from oracle import loader as OL
+    from .test_gpu_loader import _loader
+    ad, ev = yahoo_tables(6, n_ad=800, n_ev=60_000)
+
+    def text(cols):
+        rows = zip(*[[bytes(r).rstrip(b"\0").decode() for r in c] for c in cols])
+        return "".join("|".join(r) + "|\n" for r in rows)
+
+    (tmp_path / "ad-camp-map").mkdir()
+    (tmp_path / "ad-camp-map" / "part0").write_text(text(ad))
+    (tmp_path / "events").mkdir()
+    body = text(ev)
+    cut = body.index("\n", len(body) // 2) + 1  # two files, split at a row boundary
+    (tmp_path / "events" / "part0").write_text(body[:cut])
+    (tmp_path / "events" / "part1").write_text(body[cut:])
+    ad_t, ad_s = [S, S, S], [40, 40, 40]
+    ev_t, ev_s = [S] * 7, [40, 40, 40, 10, 10, 10, 10]
+    _loader("yf_ad", tmp_path / "ad-camp-map", ad_t, ad_s).Run(None, want_result=False)
+    _loader("yf_ev", tmp_path / "events", ev_t, ev_s).Run(None, want_result=False)
+    _, _, q = yahoo_plans("yf")
+    got = q.Run("yf_ev", ["yf_ad"])
+    o_ad = OL.load(str(tmp_path / "ad-camp-map"), ad_t, ad_s)
+    o_ev = OL.load(str(tmp_path / "events"), ev_t, ev_s)
+    want = O.run_pipeline(q.plan("yf_ev", ["yf_ad"]), O.OTable(ev_t, ev_s, o_ev), [O.OTable(ad_t, ad_s, o_ad)])
+    print(f"[probe] yahoo from files: {got.rows} rows", flush=True)
+    assert got.rows == want.rows > 5_000
+    assert_same_columns(got.columns, want.columns)
