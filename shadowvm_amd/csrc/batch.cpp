@@ -923,6 +923,37 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     for (auto& fp : by_col[c]) A->pred[np++] = fp;
   }
   A->pred0[bk::kMaxCols] = np;
+  // value segments of each column's predicates (one search per row instead of one
+  // test per predicate when a tile has many active predicates on the column)
+  {
+    int nb = 0;
+    for (int c = 0; c < bk::kMaxCols; ++c) {
+      A->seg0[c] = nb;
+      std::vector<long long> bounds;
+      unsigned long long cq = 0;
+      for (auto& fp : by_col[c]) {
+        cq |= 1ull << fp.q;
+        if (fp.lo > INT_MIN) bounds.push_back(fp.lo);
+        if (fp.hi < INT_MAX) bounds.push_back((long long)fp.hi + 1);
+      }
+      std::sort(bounds.begin(), bounds.end());
+      bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+      for (size_t j = 0; j <= bounds.size(); ++j) {
+        // every value of segment j (from its lower bound on) meets the same predicates
+        const long long v = j == 0 ? (long long)INT_MIN : bounds[j - 1];
+        unsigned long long m = ~0ull;
+        for (auto& fp : by_col[c])
+          if (v < fp.lo || v > fp.hi) m &= ~(1ull << fp.q);
+        A->segm[nb + c + (int)j] = m;
+      }
+      for (long long b : bounds) A->segb[nb++] = (int)b;
+      A->colq[c] = cq;
+      int steps = 0;
+      while ((1u << steps) <= bounds.size()) ++steps;
+      A->segsteps[c] = steps;
+    }
+    A->seg0[bk::kMaxCols] = nb;
+  }
   int nt = 0;
   for (size_t k = 0; k < P.slots.size(); ++k) {
     bk::MaskSlot& ms = M->slots[k];

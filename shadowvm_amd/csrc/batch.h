@@ -33,6 +33,7 @@ constexpr int kMaxAttr = 6;          // dimension columns one slot's profiles en
 constexpr int kMaxProfiles = 4096;   // profiles of all slots of one scan (staged in LDS)
 constexpr int kMaxMetric = 4;        // distinct metric expressions of one scan
 constexpr int kMaxFactPred = 128;    // (query, column, range) entries of one scan
+constexpr int kMaxSegBounds = 2 * kMaxFactPred;  // interval bounds of all fact predicates
 constexpr int kMaxTerms = kMaxBatch * kMaxJoin;
 constexpr unsigned kMaxProfilesPerSlot = 0xFFFEu;  // profile ids and the no-row id P fit 16 bits
 
@@ -142,6 +143,16 @@ struct ScanArgs {
   long long row_hi;
   unsigned ntiles;
   unsigned* counter;
+  // The fact predicates as value segments per column: the predicates' interval
+  // bounds of column c, ascending and distinct, are segb[seg0[c] .. seg0[c+1]); a value
+  // lies in segment j = the number of those bounds <= it, whose query mask (the
+  // queries all of whose predicates on c hold there) is segm[seg0[c] + c + j].  colq[c]:
+  // queries with a predicate on c; segsteps[c]: steps of the search over its bounds.
+  int seg0[kMaxCols + 1];
+  int segb[kMaxSegBounds];
+  unsigned long long segm[kMaxSegBounds + kMaxCols];
+  unsigned long long colq[kMaxCols];
+  int segsteps[kMaxCols];
   const DropCheck* chk;         // slots to check for dropped inserts
   int nchk;
   unsigned long long* dropq;    // bit q: query q's dimension object drops an insert

@@ -231,6 +231,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   unsigned* hkey = (unsigned*)(pmask + nprof);
   unsigned* praw = hkey + kSlots;
   QDesc* qd = (QDesc*)(praw + ((nprof + 3u) & ~3u));
+  unsigned long long* lsegm = (unsigned long long*)(qd + kMaxBatch);
+  int* lsegb = (int*)(lsegm + kMaxSegBounds + kMaxCols);
   unsigned long long* const gsum = A->sums;
   unsigned* const gflag = A->flags;
 
@@ -246,6 +248,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   }
   for (unsigned i = threadIdx.x; i < (unsigned)nq * (unsigned)(sizeof(QDesc) / 4); i += BLOCK)
     ((unsigned*)qd)[i] = ((const unsigned*)A->qd)[i];
+  {
+    const unsigned nb = (unsigned)A->seg0[kMaxCols];
+    for (unsigned i = threadIdx.x; i < nb; i += BLOCK) lsegb[i] = A->segb[i];
+    for (unsigned i = threadIdx.x; i < nb + (unsigned)kMaxCols; i += BLOCK) lsegm[i] = A->segm[i];
+  }
   for (unsigned i = threadIdx.x; i < (unsigned)kSlots; i += BLOCK) {
     hkey[i] = 0u;
     hsum[i] = 0ull;
@@ -327,18 +334,37 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
           }
           pass[e] = m;
         }
-        // fact predicates of the tile's queries, grouped by column
+        // fact predicates of the tile's queries, grouped by column: tested one by one
+        // when the tile has few on the column, else through the column's value
+        // segments (a search over its sorted bounds, then that segment's query mask)
 #pragma unroll
         for (int c = 0; c < kMaxCols; ++c) {
-          const int p1 = A->pred0[c + 1];
-          for (int p = A->pred0[c]; p < p1; ++p) {
-            const int q = A->pred[p].q;
-            if (!((active >> q) & (M)1)) continue;
-            const int plo = A->pred[p].lo, phi = A->pred[p].hi;
-            const M keep = ~((M)1 << q);
+          const M cq = (M)A->colq[c] & active;
+          if (!cq) continue;
+          const int steps = A->segsteps[c];
+          if (__popcll((unsigned long long)cq) <= steps + 1) {
+            const int p1 = A->pred0[c + 1];
+            for (int p = A->pred0[c]; p < p1; ++p) {
+              const int q = A->pred[p].q;
+              if (!((active >> q) & (M)1)) continue;
+              const int plo = A->pred[p].lo, phi = A->pred[p].hi;
+              const M keep = ~((M)1 << q);
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-              if (x[c][e] < plo || x[c][e] > phi) pass[e] &= keep;
+              for (int e = 0; e < 4; ++e)
+                if (x[c][e] < plo || x[c][e] > phi) pass[e] &= keep;
+            }
+          } else {
+            const int b0 = A->seg0[c], nb = A->seg0[c + 1] - b0;
+            const int* bounds = lsegb + b0;
+            const unsigned long long* masks = lsegm + b0 + c;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int v = x[c][e];
+              int pos = 0;  // number of bounds <= v
+              for (int step = steps > 0 ? 1 << (steps - 1) : 0; step > 0; step >>= 1)
+                if (pos + step <= nb && bounds[pos + step - 1] <= v) pos += step;
+              pass[e] &= (M)masks[pos];
+            }
           }
         }
         // joins: one profile lookup per slot for the rows some joining query still
@@ -696,7 +722,8 @@ bool scan_variant_prefetch(int v) { return kVariants[v].pf; }
 bool scan_variant_mask32(int v) { return kVariants[v].mask32; }
 
 size_t shared_scan_lds(int variant, unsigned nprof_cap) {
-  return ((size_t)1 << kVariants[variant].hbits) * 12 + (size_t)nprof_cap * 12 + (size_t)kMaxBatch * sizeof(QDesc);
+  return ((size_t)1 << kVariants[variant].hbits) * 12 + (size_t)nprof_cap * 12 + (size_t)kMaxBatch * sizeof(QDesc) +
+         (size_t)(kMaxSegBounds + kMaxCols) * 8 + (size_t)kMaxSegBounds * 4;
 }
 
 void prepare_shared_scan() {
