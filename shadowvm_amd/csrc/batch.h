@@ -19,6 +19,8 @@
 
 #include <cstdint>
 
+#include "refhash.h"
+
 namespace svm {
 namespace bk {
 
@@ -121,7 +123,7 @@ struct DropCheck {
   const unsigned long long* mask;  // query mask of each profile < P
   unsigned long long joined;       // queries joining through this slot
 };
-constexpr int kCheckSpan = 256;  // widest cluster (in slots) a check replays
+constexpr int kCheckSpan = k::kReplaySpan;  // widest cluster (in slots) a check replays
 
 struct ScanArgs {
   const void* cols[kMaxCols];
