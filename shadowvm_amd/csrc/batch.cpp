@@ -1226,10 +1226,11 @@ int batch_run(int n, const uint8_t* const* plans, const size_t* lens, svm_result
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   };
   std::vector<Job> jobs(n);
-  for (int i = 0; i < n; ++i) {
+  // plans decode independently: on the host workers
+  parallel_for((size_t)n, [&](size_t i) {
     jobs[i].info = decode_pipeline_info(plans[i], lens[i]);
     jobs[i].plan = decode_plan(jobs[i].info);
-  }
+  });
   const double t_decoded = host_ms();
   // dimension candidates, by pipeline id
   std::map<std::string, DimLowered> dims;

@@ -4,6 +4,10 @@
 #include "kernels.h"
 #include "svm_internal.h"
 
+#include <condition_variable>
+#include <exception>
+#include <thread>
+
 namespace svm {
 
 void fail(int code, const std::string& msg) { throw Error(code, msg); }
@@ -214,6 +218,87 @@ void Store::clear() {
 
 static std::unique_ptr<Engine> g_engine;
 static std::mutex g_engine_mu;
+
+// ----------------------------------------------------------------- host workers
+namespace {
+
+class Workers {
+ public:
+  static constexpr int kThreads = 3;  // plus the caller
+  static constexpr size_t kInline = 32;
+
+  void run(size_t n, const std::function<void(size_t)>& fn) {
+    std::lock_guard<std::mutex> one(call_mu_);  // one parallel_for at a time
+    start();
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      n_ = n;
+      next_.store(0);
+      busy_ = kThreads;
+      err_ = nullptr;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return busy_ == 0; });
+    fn_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
+  }
+
+ private:
+  void start() {
+    if (!threads_.empty()) return;
+    for (int t = 0; t < kThreads; ++t)
+      threads_.emplace_back([this] {
+        uint64_t seen = 0;
+        for (;;) {
+          {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+          }
+          work();
+          std::lock_guard<std::mutex> g(mu_);
+          if (--busy_ == 0) done_.notify_all();
+        }
+      });
+    for (auto& t : threads_) t.detach();  // they live as long as the process
+  }
+  void work() {
+    for (;;) {
+      const size_t i = next_.fetch_add(1);
+      if (i >= n_) return;
+      try {
+        (*fn_)(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (!err_) err_ = std::current_exception();
+      }
+    }
+  }
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void(size_t)>* fn_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  int busy_ = 0;
+  uint64_t gen_ = 0;
+  std::exception_ptr err_;
+};
+
+}  // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t)>& fn) {
+  if (n < Workers::kInline) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  static Workers* w = new Workers();  // never destroyed: its threads are detached
+  w->run(n, fn);
+}
 
 Engine& engine() {
   if (!g_engine) fail(SVM_ERR_STATE, "svm_init has not been called");

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <functional>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -192,5 +193,10 @@ struct Engine {
 };
 Engine& engine();  // throws SVM_ERR_STATE if not initialised
 bool engine_ready();
+
+// Runs fn(0..n-1) on the calling thread and a few persistent host workers; returns
+// when all are done and rethrows the first exception any of them raised.  Small n
+// runs inline.
+void parallel_for(size_t n, const std::function<void(size_t)>& fn);
 
 }  // namespace svm
