@@ -4,6 +4,8 @@
 #include "kernels.h"
 #include "svm_internal.h"
 
+static_assert(svm::kColumnPad == (uint64_t)svm::k::kPadBytes, "pad kernel clears the column padding");
+
 #include <condition_variable>
 #include <exception>
 #include <thread>
@@ -117,11 +119,33 @@ void arena_reset(hipStream_t s) {
   a.off = 0;
 }
 
+namespace {
+std::map<hipStream_t, HostSlots> g_host_slots;  // guarded by g_arena_mu
+}  // namespace
+
+HostSlots host_slots(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  auto it = g_host_slots.find(s);
+  if (it != g_host_slots.end()) return it->second;
+  void* h = nullptr;
+  void* d = nullptr;
+  SVM_HIP(hipHostMalloc(&h, kHostSlots * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent));
+  SVM_HIP(hipHostGetDevicePointer(&d, h, 0));
+  memset(h, 0, kHostSlots * sizeof(unsigned long long));
+  HostSlots hs;
+  hs.host = (volatile unsigned long long*)h;
+  hs.dev = (unsigned long long*)d;
+  g_host_slots[s] = hs;
+  return hs;
+}
+
 void arena_release_all() {
   std::lock_guard<std::mutex> g(g_arena_mu);
   for (auto& kv : g_arenas)
     for (auto& c : kv.second.chunks) (void)hipFree(c.first);
   g_arenas.clear();
+  for (auto& kv : g_host_slots) (void)hipHostFree((void*)kv.second.host);
+  g_host_slots.clear();
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {
@@ -165,6 +189,7 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
   t->rows = rows;
   t->info.id = id;
   t->info.storage = 2;  // DeviceMem (com/svm.proto:49-54)
+  std::vector<char*> pads;
   for (size_t c = 0; c < schema.size(); ++c) {
     Column col;
     col.type = schema[c].first;
@@ -175,7 +200,7 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
       col.bytes = rows * (uint64_t)col.size + kColumnPad;
       col.dev = dev_alloc(col.bytes);
       if (fill_stream)
-        SVM_HIP(hipMemsetAsync((char*)col.dev + rows * (uint64_t)col.size, 0, kColumnPad, fill_stream));
+        pads.push_back((char*)col.dev + rows * (uint64_t)col.size);
       else
         SVM_HIP(hipMemset(col.dev, 0, col.bytes));
     }
@@ -185,6 +210,8 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
     ci.val_size = col.size;
     t->info.column_list.push_back(ci);
   }
+  // the padding of every column behind its last row, cleared in one launch
+  if (!pads.empty()) k::zero_pads(pads.data(), (int)pads.size(), fill_stream);
   return t;
 }
 

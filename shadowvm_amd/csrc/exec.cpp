@@ -729,15 +729,16 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   SVM_HIP(hipEventRecord(ev1.e, c.s));
   uint64_t total = 0;
   std::string src;
+  const HostSlots slots = host_slots(c.s);
   if (tiles) {
     src = gen_select_source(ks);
     hipFunction_t fcount = jit_function(src, "svm_sel_count");
     if (tiles > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "select source too large");
     launch(fcount, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
-                          offs.as<unsigned long long>() + tiles, c.s);
-    SVM_HIP(hipMemcpyAsync(&total, offs.as<unsigned long long>() + tiles, 8, hipMemcpyDeviceToHost, c.s));
+                          offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
     SVM_HIP(hipStreamSynchronize(c.s));
+    total = slots.host[0];
   }
   // Sink bulk i holds what source bulk i produced (core/pipeline.cu:161-162,294-296), so
   // the output's bulk 0 -- all a later join reads of it -- is the passing rows of the
@@ -756,9 +757,9 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
       launch(jit_function(src, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s,
              a0);
       k::exclusive_scan_u32(counts0.as<uint32_t>(), tiles0, offs0.as<unsigned long long>(),
-                            offs0.as<unsigned long long>() + tiles0, c.s);
-      SVM_HIP(hipMemcpyAsync(&out_bulk0, offs0.as<unsigned long long>() + tiles0, 8, hipMemcpyDeviceToHost, c.s));
+                            offs0.as<unsigned long long>() + tiles0, c.s, slots.dev + 1);
       SVM_HIP(hipStreamSynchronize(c.s));
+      out_bulk0 = slots.host[1];
     }
   }
   // output object: the sink schema; only projected columns are materialised (the

@@ -36,9 +36,10 @@ void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* 
               uint32_t* packed, hipStream_t s);
 // True when two rows of keys[0..n) share a key (keys within [kmin, kmin+range)); synchronises.
 bool has_duplicates(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, hipStream_t s);
-// out[i] = sum(in[0..i)), *total = sum(in) (single workgroup).
+// out[i] = sum(in[0..i)), *total = sum(in) (single workgroup); also writes the
+// total to host_total (a pinned host word) when given.
 void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,
-                        unsigned long long* total, hipStream_t s);
+                        unsigned long long* total, hipStream_t s, unsigned long long* host_total = nullptr);
 
 struct EmitArgs {
   const uint32_t* flags;      // bit per group
@@ -56,6 +57,15 @@ struct EmitArgs {
 uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scratch_blocks,
                      unsigned long long* scratch_offsets, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
+
+// Zeroes the 256 padding bytes behind each of `n` column ends, in one launch per 32.
+constexpr int kMaxPads = 32;
+constexpr int kPadBytes = 256;
+struct PadList {
+  char* p[kMaxPads];
+  int n;
+};
+void zero_pads(char* const* ptrs, int n, hipStream_t s);
 
 // Streaming 16-byte copy of `bytes` (a multiple of 16) — counter calibration.
 void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);

@@ -128,7 +128,7 @@ __global__ void k_pack(const int32_t* __restrict__ first, uint32_t range, int pa
 // Single-workgroup exclusive scan with a running carry (inputs up to a few million).
 template <typename T>
 __global__ void k_exscan(const T* __restrict__ in, uint64_t n, unsigned long long* out,
-                         unsigned long long* total) {
+                         unsigned long long* total, unsigned long long* host_total) {
   __shared__ unsigned long long buf[1024];
   unsigned long long carry = 0;
   for (uint64_t base = 0; base < n; base += 1024) {
@@ -147,7 +147,10 @@ __global__ void k_exscan(const T* __restrict__ in, uint64_t n, unsigned long lon
     __syncthreads();
     carry += blk;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (host_total) *host_total = carry;
+  }
 }
 
 __global__ void k_group_popc(const uint32_t* __restrict__ flags, uint64_t nwords,
@@ -165,32 +168,39 @@ __global__ void k_group_popc(const uint32_t* __restrict__ flags, uint64_t nwords
   }
 }
 
+// One thread per group id: block b covers groups [256 b, 256 b + 256), i.e. flag
+// words [8 b, 8 b + 8).  `offsets` holds the exclusive prefix of set flags per
+// k_group_popc block (kBlock words); the rows of earlier words inside that block
+// are counted here, then a wave ballot orders the block's own groups.
 __global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ offsets) {
-  __shared__ unsigned pre[kBlock];
-  const uint64_t nwords = (a.G + 31) / 32;
-  uint64_t w = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  uint32_t bits = w < nwords ? a.flags[w] : 0u;
-  unsigned c = (unsigned)__popc(bits);
-  pre[threadIdx.x] = c;
+  static_assert(kBlock == 256, "one block covers 8 flag words");
+  __shared__ unsigned red[kBlock / 64];
+  __shared__ unsigned cnt[kBlock / 64];
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t w0 = (uint64_t)blockIdx.x * (kBlock / 32);
+  const uint64_t wa = w0 / kBlock * kBlock;  // first word of the k_group_popc block
+  unsigned c = wa + threadIdx.x < w0 ? (unsigned)__popc(a.flags[wa + threadIdx.x]) : 0u;
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+  if (lane == 0) red[wave] = c;
+  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool set = g < a.G && ((a.flags[g >> 5] >> (g & 31)) & 1u);
+  const unsigned long long m = __ballot(set);
+  if (lane == 0) cnt[wave] = (unsigned)__popcll(m);
   __syncthreads();
-  for (int d = 1; d < kBlock; d <<= 1) {
-    unsigned t = threadIdx.x >= (unsigned)d ? pre[threadIdx.x - d] : 0u;
-    __syncthreads();
-    pre[threadIdx.x] += t;
-    __syncthreads();
+  if (!set) return;
+  unsigned long long row = offsets[w0 / kBlock] + red[0] + red[1] + red[2] + red[3];
+  for (unsigned q = 0; q < wave; ++q) row += cnt[q];
+  row += (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+  for (int q = 0; q < a.K; ++q) {
+    uint32_t code = (uint32_t)((g / a.stride[q]) % a.n[q]);
+    a.keys_out[q][row] = a.dict[q] ? a.dict[q][code] : a.fmin[q] + (int32_t)code;
   }
-  unsigned long long row = offsets[blockIdx.x] + pre[threadIdx.x] - c;
-  while (bits) {
-    int b = __ffs(bits) - 1;
-    bits &= bits - 1;
-    uint64_t g = w * 32 + (uint64_t)b;
-    for (int q = 0; q < a.K; ++q) {
-      uint32_t code = (uint32_t)((g / a.stride[q]) % a.n[q]);
-      a.keys_out[q][row] = a.dict[q] ? a.dict[q][code] : a.fmin[q] + (int32_t)code;
-    }
-    for (int m = 0; m < a.M; ++m) a.sums_out[m][row] = (int64_t)a.sums[g * (uint64_t)a.M + m];
-    row++;
-  }
+  for (int m2 = 0; m2 < a.M; ++m2) a.sums_out[m2][row] = (int64_t)a.sums[g * (uint64_t)a.M + m2];
+}
+
+// Zeroes kColumnPad bytes at each pointer (one block per pointer).
+__global__ void k_zero_pads(PadList l) {
+  if (blockIdx.x < (unsigned)l.n && threadIdx.x < kPadBytes) l.p[blockIdx.x][threadIdx.x] = 0;
 }
 
 __global__ void svm_calib_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
@@ -286,8 +296,8 @@ bool has_duplicates(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t rang
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint64_t n, unsigned long long* out,
-                        unsigned long long* total, hipStream_t s) {
-  k_exscan<uint32_t><<<1, 1024, 0, s>>>(in, n, out, total);
+                        unsigned long long* total, hipStream_t s, unsigned long long* host_total) {
+  k_exscan<uint32_t><<<1, 1024, 0, s>>>(in, n, out, total, host_total);
   LAUNCH_CHECK("exclusive_scan_u32");
 }
 
@@ -297,20 +307,29 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* bloc
   uint64_t nb = (nwords + kBlock - 1) / kBlock;
   if (!nb) return 0;
   k_group_popc<<<(unsigned)nb, kBlock, 0, s>>>(flags, nwords, blocks);
-  k_exscan<unsigned long long><<<1, 1024, 0, s>>>(blocks, nb, offsets, offsets + nb);
+  const HostSlots hs = host_slots(s);
+  k_exscan<unsigned long long><<<1, 1024, 0, s>>>(blocks, nb, offsets, offsets + nb, hs.dev + 2);
   LAUNCH_CHECK("group_count");
-  unsigned long long total = 0;
-  SVM_HIP(hipMemcpyAsync(&total, offsets + nb, sizeof(total), hipMemcpyDeviceToHost, s));
   SVM_HIP(hipStreamSynchronize(s));
-  return total;
+  return hs.host[2];
 }
 
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s) {
-  uint64_t nwords = (a.G + 31) / 32;
-  uint64_t nb = (nwords + kBlock - 1) / kBlock;
+  const uint64_t nb = (a.G + kBlock - 1) / kBlock;
   if (!nb) return;
+  if (nb > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "group_emit: group space too large");
   k_group_emit<<<(unsigned)nb, kBlock, 0, s>>>(a, offsets);
   LAUNCH_CHECK("group_emit");
+}
+
+void zero_pads(char* const* ptrs, int n, hipStream_t s) {
+  for (int i = 0; i < n; i += kMaxPads) {
+    PadList l;
+    l.n = n - i < kMaxPads ? n - i : kMaxPads;
+    for (int q = 0; q < l.n; ++q) l.p[q] = ptrs[i + q];
+    k_zero_pads<<<(unsigned)l.n, kPadBytes, 0, s>>>(l);
+    LAUNCH_CHECK("zero_pads");
+  }
 }
 
 void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

@@ -144,6 +144,16 @@ void dev_free_async(void* p, hipStream_t s);
 void* arena_alloc(hipStream_t s, uint64_t bytes);
 void arena_reset(hipStream_t s);
 void arena_release_all();
+// Pinned, coherent host words that kernels write directly: counts and totals the
+// host reads after synchronising the stream, with no copy dispatch.  Each stream has
+// kHostSlots of them; a run owns its stream's slots until it has synchronised.
+// Slot owners: 0 = select output rows, 1 = select bulk-0 rows, 2 = group count.
+constexpr int kHostSlots = 16;
+struct HostSlots {
+  volatile unsigned long long* host = nullptr;
+  unsigned long long* dev = nullptr;
+};
+HostSlots host_slots(hipStream_t s);
 // Temporary device buffer.  With a stream it lives in that stream's arena until
 // the next arena_reset; without one it is a plain (cached) device allocation.
 struct DevBuf {
