@@ -390,17 +390,45 @@ std::string gen_select_source(const KernelShape& s) {
   std::ostringstream o;
   emit_prologue(o, s);
   o << "#define SVM_RPT 4\n";
-  const char* names[2] = {"svm_sel_count", "svm_sel_write"};
-  for (int pass = 0; pass < 2; ++pass) {
+  // A workgroup compacts one tile of SVM_BLOCK * SVM_RPT rows in row order:
+  //   svm_sel_count / svm_sel_write   two launches around a scan of the tile counts;
+  //   svm_sel_single                  one launch: tiles are taken from a ticket and
+  //                                   each finds the rows of the earlier tiles by a
+  //                                   decoupled look-back over per-tile status words.
+  const char* names[3] = {"svm_sel_count", "svm_sel_write", "svm_sel_single"};
+  for (int pass = 0; pass < 3; ++pass) {
     o << "extern \"C\" __global__ void __launch_bounds__(SVM_BLOCK) " << names[pass] << "(SvmArgs a) {\n";
     o << "  extern __shared__ svm_u64 svm_lds64[];\n  unsigned* svm_lds = (unsigned*)svm_lds64;\n";
     emit_colptrs(o, s);
     emit_stage_maps(o, s);
     emit_map_ptrs(o, s);
     o << "  __shared__ unsigned wcnt[SVM_RPT][SVM_BLOCK / 64];\n";
-    o << "  __syncthreads();\n";
+    if (pass == 2) {
+      // status word: epoch << 32 | inclusive << 31 | rows (< 2^31); a word of another
+      // epoch is not yet published.  The ticket counts the tiles taken; the workgroup
+      // that takes the last one puts it back to 0 for the next launch on the stream.
+      o << "  __shared__ unsigned long long s_tile, s_pos;\n"
+        << "  __shared__ int s_err;\n"
+        << "  const unsigned long long epoch = (unsigned long long)(unsigned)a.iv[46];\n"
+        << "  unsigned long long* ticket = (unsigned long long*)a.out[7];\n"
+        << "  unsigned long long* status = ticket + 1;\n"
+        << "  const unsigned long long ntiles = ((unsigned long long)a.rows + SVM_BLOCK * SVM_RPT - 1) / (SVM_BLOCK * SVM_RPT);\n"
+        << "  if (threadIdx.x == 0) {\n"
+        << "    const unsigned long long t = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "    if (t + 1 == ntiles) __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "    s_tile = t;\n"
+        << "    s_err = t >= ntiles;\n"
+        << "    if (s_err) *(volatile unsigned long long*)((unsigned long long*)a.out[5] + 3) = 1ull;\n"
+        << "  }\n"
+        << "  __syncthreads();\n"
+        << "  if (s_err) return;\n"
+        << "  const unsigned long long tile = s_tile;\n";
+    } else {
+      o << "  __syncthreads();\n"
+        << "  const unsigned long long tile = blockIdx.x;\n";
+    }
     o << "  const long long r_hi = a.row0 + a.rows;\n"
-      << "  const long long t0 = a.row0 + (long long)blockIdx.x * (SVM_BLOCK * SVM_RPT);\n"
+      << "  const long long t0 = a.row0 + (long long)tile * (SVM_BLOCK * SVM_RPT);\n"
       << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
       << "  unsigned passbits = 0u;\n";
     // first sweep: conditions and per-wave counts
@@ -420,49 +448,111 @@ std::string gen_select_source(const KernelShape& s) {
         << "    unsigned t = 0;\n"
         << "    for (int k = 0; k < SVM_RPT; ++k) for (int w = 0; w < SVM_BLOCK / 64; ++w) t += wcnt[k][w];\n"
         << "    ((unsigned*)a.out[7])[blockIdx.x] = t;\n"
-        << "  }\n";
+        << "  }\n"
+        << "}\n";
+      continue;
+    }
+    if (pass == 1) {
+      o << "  unsigned long long pos = ((const unsigned long long*)a.out[6])[blockIdx.x];\n";
     } else {
-      o << "  unsigned long long pos = ((const unsigned long long*)a.out[6])[blockIdx.x];\n"
-        << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
-        << "    unsigned before = 0;\n"
-        << "    for (int w = 0; w < wave; ++w) before += wcnt[k][w];\n"
-        << "    unsigned round = 0;\n"
-        << "    for (int w = 0; w < SVM_BLOCK / 64; ++w) round += wcnt[k][w];\n"
-        << "    const bool cond = (passbits >> k) & 1u;\n"
-        << "    const unsigned long long mask = __ballot(cond);\n"
-        << "    if (cond) {\n"
-        << "      const long long i = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
-        << "      const unsigned long long at = pos + before + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));\n";
-      for (int c : s.fact_i32) o << "      const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
-      // recompute join payloads for the projection (row ids)
-      for (size_t j = 0; j < s.joins.size(); ++j) {
-        const JoinShape& J = s.joins[j];
-        if (J.payload != PAY_ROW) continue;
-        if (J.kind != JOIN_MAP) {
-          o << "      " << probe_stmt(J, j, "rj" + I((int)j)) << "\n"
-            << "      const long long JI" << J.robj << " = rj" << j << ";\n";
-          continue;
-        }
-        o << "      const unsigned offj" << j << " = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
-          << "      const long long JI" << J.robj << " = (long long)svm_map_get<" << J.wbits << ">(m" << j
-          << ", offj" << j << ") - 1;\n";
+      // publish this tile's count, then wave 0 adds the counts of earlier tiles, 64
+      // statuses per step, up to the nearest one that holds its inclusive prefix
+      // (positions before tile 0 read as an inclusive 0)
+      o << "  if (wave == 0) {\n"
+        << "    unsigned long long c = 0;\n"
+        << "    for (int k = 0; k < SVM_RPT; ++k) for (int w = 0; w < SVM_BLOCK / 64; ++w) c += wcnt[k][w];\n"
+        << "    if (lane == 0 && tile > 0)\n"
+        << "      __hip_atomic_store(status + tile, (epoch << 32) | c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "    unsigned long long pre = 0;\n"
+        << "    int err = 0;\n"
+        << "    if (tile > 0) {\n"
+        << "      long long top = (long long)tile - 1;\n"
+        << "      unsigned spins = 0;\n"
+        << "      for (;;) {\n"
+        << "        const long long p = top - lane;\n"
+        << "        const unsigned long long v = p >= 0 ? __hip_atomic_load(status + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)\n"
+        << "                                            : ((epoch << 32) | 0x80000000ull);\n"
+        << "        const bool ready = (v >> 32) == epoch;\n"
+        << "        const unsigned long long rm = __ballot(ready), im = __ballot(ready && (v & 0x80000000ull));\n"
+        << "        // lanes 0..k up to the nearest inclusive status k (all 64 when there is none)\n"
+        << "        const unsigned long long need = im ? ((im & (~im + 1ull)) << 1) - 1ull : ~0ull;\n"
+        << "        if ((rm & need) == need) {\n"
+        << "          unsigned long long x = ((need >> lane) & 1ull) ? (v & 0x7FFFFFFFull) : 0ull;\n"
+        << "          for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d);\n"
+        << "          pre += x;\n"
+        << "          if (im) break;\n"
+        << "          top -= 64;\n"
+        << "          continue;\n"
+        << "        }\n"
+        << "        // bounded: an earlier tile that never publishes ends the launch with an error\n"
+        << "        if (++spins > (1u << 22)) { err = 1; break; }\n"
+        << "        __builtin_amdgcn_s_sleep(1);\n"
+        << "      }\n"
+        << "    }\n"
+        << "    if (lane == 0) {\n"
+        << "      if (err) {\n"
+        << "        *(volatile unsigned long long*)((unsigned long long*)a.out[5] + 3) = 1ull;\n"
+        << "        __hip_atomic_store(status + tile, (epoch << 32) | 0x80000000ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "      } else {\n"
+        << "        __hip_atomic_store(status + tile, (epoch << 32) | 0x80000000ull | (pre + c), __ATOMIC_RELEASE,\n"
+        << "                           __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "        if (tile + 1 == ntiles) *(volatile unsigned long long*)a.out[5] = pre + c;\n"
+        << "      }\n"
+        << "      s_err = err;\n"
+        << "      s_pos = pre;\n"
+        << "    }\n"
+        << "  }\n"
+        << "  __syncthreads();\n"
+        << "  if (s_err) return;  // no rows are written from an inconsistent prefix\n"
+        << "  unsigned long long pos = s_pos;\n";
+    }
+    o << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
+      << "    unsigned before = 0;\n"
+      << "    for (int w = 0; w < wave; ++w) before += wcnt[k][w];\n"
+      << "    unsigned round = 0;\n"
+      << "    for (int w = 0; w < SVM_BLOCK / 64; ++w) round += wcnt[k][w];\n"
+      << "    const bool cond = (passbits >> k) & 1u;\n"
+      << "    const unsigned long long mask = __ballot(cond);\n"
+      << "    if (cond) {\n"
+      << "      const long long i = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
+      << "      const unsigned long long at = pos + before + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));\n";
+    for (int c : s.fact_i32) o << "      const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
+    // recompute join payloads for the projection (row ids)
+    for (size_t j = 0; j < s.joins.size(); ++j) {
+      const JoinShape& J = s.joins[j];
+      if (J.payload != PAY_ROW) continue;
+      if (J.kind != JOIN_MAP) {
+        o << "      " << probe_stmt(J, j, "rj" + I((int)j)) << "\n"
+          << "      const long long JI" << J.robj << " = rj" << j << ";\n";
+        continue;
       }
-      for (size_t q = 0; q < s.proj.size(); ++q) {
-        int t = s.proj_type[q].first;
-        long long size = s.proj_type[q].second;
-        std::string dst = "((char*)a.ptr[" + I(SVM_SLOT_OUT + (int)q) + "])";
-        if (t == 7) {
-          o << "      StrCopy(" << dst << " + at * " << size << "ull, (const char*)(" << s.proj[q] << "), "
-            << size << ");\n";
-        } else {
-          const char* ct = t == 0 ? "char" : t == 1 ? "short" : t == 3 ? "long long" : "int";
-          o << "      *(" << ct << "*)(" << dst << " + at * " << size << "ull) = (" << ct << ")(" << s.proj[q]
-            << ");\n";
-        }
+      o << "      const unsigned offj" << j << " = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+        << "      const long long JI" << J.robj << " = (long long)svm_map_get<" << J.wbits << ">(m" << j
+        << ", offj" << j << ") - 1;\n";
+    }
+    for (size_t q = 0; q < s.proj.size(); ++q) {
+      int t = s.proj_type[q].first;
+      long long size = s.proj_type[q].second;
+      std::string dst = "((char*)a.ptr[" + I(SVM_SLOT_OUT + (int)q) + "])";
+      if (t == 7) {
+        o << "      StrCopy(" << dst << " + at * " << size << "ull, (const char*)(" << s.proj[q] << "), "
+          << size << ");\n";
+      } else {
+        const char* ct = t == 0 ? "char" : t == 1 ? "short" : t == 3 ? "long long" : "int";
+        o << "      *(" << ct << "*)(" << dst << " + at * " << size << "ull) = (" << ct << ")(" << s.proj[q]
+          << ");\n";
       }
-      o << "    }\n"
-        << "    pos += round;\n"
-        << "  }\n";
+    }
+    o << "    }\n"
+      << "    pos += round;\n"
+      << "  }\n";
+    if (pass == 2) {
+      // the last tile ends the output: the column padding behind the last row
+      o << "  if (tile + 1 == ntiles && threadIdx.x < 256) {\n";
+      for (size_t q = 0; q < s.proj.size(); ++q)
+        o << "    ((char*)a.ptr[" << SVM_SLOT_OUT + (int)q << "])[pos * " << s.proj_type[q].second
+          << "ull + threadIdx.x] = 0;\n";
+      o << "  }\n";
     }
     o << "}\n";
   }

@@ -121,7 +121,50 @@ void arena_reset(hipStream_t s) {
 
 namespace {
 std::map<hipStream_t, HostSlots> g_host_slots;  // guarded by g_arena_mu
+struct ScanBuf {
+  unsigned long long* buf = nullptr;
+  uint64_t cap = 0;
+  bool dirty = true;  // cleared by a kernel on its stream before the next use
+};
+std::map<hipStream_t, ScanBuf> g_scan_bufs;  // guarded by g_arena_mu
+uint32_t g_scan_epoch = 0;                   // guarded by g_arena_mu; unique across buffers
 }  // namespace
+
+ScanState scan_state(hipStream_t s, uint64_t tiles) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  ScanBuf& b = g_scan_bufs[s];
+  if (b.cap < tiles) {
+    // runs on a stream are sequential and synchronised, so the old buffer is idle
+    if (b.buf) (void)hipFree(b.buf);
+    uint64_t cap = 4096;
+    while (cap < tiles) cap <<= 1;
+    b.buf = (unsigned long long*)raw_alloc((cap + 1) * sizeof(unsigned long long));
+    b.cap = cap;
+    b.dirty = true;
+  }
+  if (++g_scan_epoch == 0) {
+    // after 2^32 launches the epochs repeat: every buffer starts over from zeros
+    for (auto& kv : g_scan_bufs) kv.second.dirty = true;
+    g_scan_epoch = 1;
+  }
+  if (b.dirty) {
+    // a kernel on the same stream, so the clearing is ordered before the launch that
+    // follows (memory from the allocator may hold another buffer's old statuses;
+    // their epochs are older than any epoch handed out from now on)
+    k::fill_u32(b.buf, (b.cap + 1) * 2, 0u, s);
+    b.dirty = false;
+  }
+  ScanState st;
+  st.buf = b.buf;
+  st.epoch = g_scan_epoch;
+  return st;
+}
+
+void scan_state_reset(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  auto it = g_scan_bufs.find(s);
+  if (it != g_scan_bufs.end()) it->second.dirty = true;
+}
 
 HostSlots host_slots(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_arena_mu);
@@ -146,6 +189,8 @@ void arena_release_all() {
   g_arenas.clear();
   for (auto& kv : g_host_slots) (void)hipHostFree((void*)kv.second.host);
   g_host_slots.clear();
+  for (auto& kv : g_scan_bufs) (void)hipFree(kv.second.buf);
+  g_scan_bufs.clear();
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {
@@ -183,7 +228,8 @@ TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t
 }
 
 TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,
-                       uint64_t rows, const std::vector<bool>* materialise, hipStream_t fill_stream) {
+                       uint64_t rows, const std::vector<bool>* materialise, hipStream_t fill_stream,
+                       bool zero_pad) {
   auto t = std::make_shared<Table>();
   t->id = id;
   t->rows = rows;
@@ -199,10 +245,11 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
     if (mat) {
       col.bytes = rows * (uint64_t)col.size + kColumnPad;
       col.dev = dev_alloc(col.bytes);
-      if (fill_stream)
-        pads.push_back((char*)col.dev + rows * (uint64_t)col.size);
+      if (fill_stream) {
+        if (zero_pad) pads.push_back((char*)col.dev + rows * (uint64_t)col.size);
+      }
       else
-        SVM_HIP(hipMemset(col.dev, 0, col.bytes));
+        SVM_HIP(hipMemsetAsync(col.dev, 0, col.bytes, nullptr));
     }
     t->cols.push_back(col);
     ColumnInfo ci;
@@ -212,6 +259,9 @@ TablePtr Store::create(const std::string& id, const std::vector<std::pair<int, i
   }
   // the padding of every column behind its last row, cleared in one launch
   if (!pads.empty()) k::zero_pads(pads.data(), (int)pads.size(), fill_stream);
+  // a table zeroed on the null stream is complete before any stream (none of which
+  // waits on the null stream) can fill or read it
+  if (!fill_stream) SVM_HIP(hipStreamSynchronize(nullptr));
   return t;
 }
 

@@ -32,6 +32,14 @@ constexpr uint64_t kMaxFactKeyRange = 1ull << 24;
 constexpr uint64_t kDenseAggBytes = 16 * 1024;  // LDS for a dense per-workgroup table
 constexpr int kHashBits = 10;                    // LDS hash table slots = 2^kHashBits
 constexpr uint64_t kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup
+// Largest output (source rows x projected bytes) a select allocates for every source
+// row so that it can compact in one launch.
+constexpr uint64_t kSingleSelectBytes = 256ull << 20;
+// ... and at most this many 1024-row tiles: beyond a few hundred, the tiles' polling
+// of their predecessors' statuses costs more than the second pass over the source.
+constexpr uint64_t kSingleSelectTiles = 256;
+// Rows above which a dense map with sub-byte slots is built by key, not by row.
+constexpr uint64_t kDirectNarrowRows = 1ull << 16;
 
 const char* ctype(int t) {
   switch (t) {
@@ -90,7 +98,7 @@ int width_for(uint64_t maxval) {
 // Group-key component of a dimension: the dictionary of the column's origin (the
 // base table it was projected from, cached across queries, so no host round trip)
 // and, by binary search, the code of every row of this dimension object.
-void build_key_dict(const TablePtr& D, int col, KeyBuild& kb, DevBuf* codes_out, hipStream_t s) {
+void key_dict(const TablePtr& D, int col, KeyBuild& kb, hipStream_t s) {
   const Column& c = D->cols[col];
   if (!c.dev) fail(SVM_ERR_INVALID, "group key column is not materialised");
   if (c.type != Int32 && c.type != String && c.type != Int16 && c.type != Byte)
@@ -98,6 +106,11 @@ void build_key_dict(const TablePtr& D, int col, KeyBuild& kb, DevBuf* codes_out,
   auto root = column_root(D, col);
   kb.dict = table_dict(root.first, root.second, s);
   kb.n = kb.dict->n;
+}
+
+void build_key_dict(const TablePtr& D, int col, KeyBuild& kb, DevBuf* codes_out, hipStream_t s) {
+  key_dict(D, col, kb, s);
+  const Column& c = D->cols[col];
   uint64_t n = D->rows;
   DevBuf vals(n * 4 + 4, s);
   k::values_of(D->col_ptr(col), c.type, c.size, n, vals.as<int32_t>(), s);
@@ -189,17 +202,46 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     bool refs = false;
     for (auto& o : expr_refs) refs = refs || o.first == R.obj;
     int payload = refs || comps.size() > 1 || probe ? PAY_ROW : comps.size() == 1 ? PAY_CODE : PAY_EXIST;
-    // key dictionaries of this dimension
+    // A dimension object drawn (in order) from a root table with unique keys has
+    // unique keys itself: its dense map is built in one pass over its rows (each row
+    // writes its own slot), and the inserts the reference's table would drop are
+    // replayed inside the root's dropping clusters afterwards.
+    const bool derived = root.first != Dp || root.second != R.col;
+    const uint64_t slots = (uint64_t)(uint32_t)(kHashRate * (D.max_tuple_count ? D.max_tuple_count : n0));
+    std::shared_ptr<const k::HtClusters> hc;
+    bool direct = false;
+    if (!probe && range && derived && ke.unique && slots) {
+      hc = table_drop_clusters(root.first, root.second, slots, c.s);
+      direct = !hc || hc->max_span <= (uint32_t)k::kReplaySpan;
+    }
+    // key dictionaries of this dimension (a code payload of the direct build is
+    // looked up inside it)
     DevBuf codes_for_pack;
+    const KeyBuild* direct_code = nullptr;
     for (int q : comps) {
       KeyBuild& kb = keys[q];
       kb.shape.join = (int)j;
       kb.shape.src = payload == PAY_CODE ? KEY_DIMCODE : KEY_DIMROW;
-      build_key_dict(Dp, kb.shape.col, kb, payload == PAY_CODE ? &codes_for_pack : &kb.codes, c.s);
+      if (direct && payload == PAY_CODE) {
+        key_dict(Dp, kb.shape.col, kb, c.s);
+        direct_code = &kb;
+      } else {
+        build_key_dict(Dp, kb.shape.col, kb, payload == PAY_CODE ? &codes_for_pack : &kb.codes, c.s);
+      }
     }
     jb.shape.payload = payload;
     uint64_t maxval = payload == PAY_EXIST ? 1 : payload == PAY_CODE ? keys[comps[0]].n : D.rows;
     jb.shape.wbits = payload == PAY_ROW ? 32 : width_for(maxval);
+    // slots narrower than a byte share words: many rows with neighbouring keys would
+    // serialise on their atomicOr, so large objects build those maps by key instead
+    if (direct && n0 > kDirectNarrowRows && jb.shape.wbits < 8) {
+      direct = false;
+      if (direct_code) {
+        KeyBuild& kb = keys[comps[0]];
+        build_key_dict(Dp, kb.shape.col, kb, &codes_for_pack, c.s);
+        direct_code = nullptr;
+      }
+    }
     const void* kcol = D.col_ptr(R.col);
     if (!kcol) fail(SVM_ERR_INVALID, "join key column of '" + D.id + "' is not materialised");
     if (probe) {
@@ -230,15 +272,19 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
       // only inside the root's dropping clusters: its map is built, then those
       // clusters are replayed on the device.  Otherwise the kept rows are computed
       // (and cached) for the object itself.
+      if (direct) {
+        const Column* ac = direct_code ? &D.cols[direct_code->shape.col] : nullptr;
+        k::pack_direct((const int32_t*)kcol, n0, jb.kmin, jb.range, payload,
+                       ac ? D.col_ptr(direct_code->shape.col) : nullptr, ac ? ac->type : Int32, ac ? ac->size : 4,
+                       direct_code ? direct_code->dict->dict.as<int32_t>() : nullptr, direct_code ? direct_code->n : 0,
+                       jb.shape.wbits, jb.packed.as<uint32_t>(), c.s);
+        if (hc && n0 > 10)
+          k::ht_clear_dropped_packed(*hc, jb.kmin, jb.range, jb.packed.as<uint32_t>(), jb.shape.wbits, c.s);
+        joins.push_back(std::move(jb));
+        continue;
+      }
       DevBuf first((uint64_t)jb.range * 4, c.s);
-      const bool derived = root.first != Dp || root.second != R.col;
-      const uint64_t slots = (uint64_t)(uint32_t)(kHashRate * (D.max_tuple_count ? D.max_tuple_count : n0));
-      std::shared_ptr<const k::HtClusters> hc;
-      if (derived && ke.unique && slots) hc = table_drop_clusters(root.first, root.second, slots, c.s);
-      if (derived && ke.unique && slots && (!hc || hc->max_span <= (uint32_t)k::kReplaySpan)) {
-        k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s);
-        if (hc && n0 > 10) k::ht_clear_dropped(*hc, jb.kmin, jb.range, first.as<int32_t>(), c.s);
-      } else {
+      {
         const uint8_t* kept = join_kept(Dp, R.col, &jb.dropped, c.s);
         k::first_rows((const int32_t*)kcol, n0, jb.kmin, jb.range, first.as<int32_t>(), c.s, kept);
         if (jb.dropped && c.E.cfg.verbose)
@@ -720,77 +766,110 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   const uint64_t rows = c.src->rows;
   const uint64_t tile = (uint64_t)ks.block * 4;
   const uint64_t tiles = (rows + tile - 1) / tile;
-  DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
-  SvmArgs a;
-  DevBuf blob;
-  fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
-  a.out[6] = offs.p;
-  a.out[7] = counts.p;
-  SVM_HIP(hipEventRecord(ev1.e, c.s));
-  uint64_t total = 0;
-  std::string src;
-  const HostSlots slots = host_slots(c.s);
-  if (tiles) {
-    src = gen_select_source(ks);
-    hipFunction_t fcount = jit_function(src, "svm_sel_count");
-    if (tiles > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "select source too large");
-    launch(fcount, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
-    k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
-                          offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
-    SVM_HIP(hipStreamSynchronize(c.s));
-    total = slots.host[0];
-  }
-  // Sink bulk i holds what source bulk i produced (core/pipeline.cu:161-162,294-296), so
-  // the output's bulk 0 -- all a later join reads of it -- is the passing rows of the
-  // source's bulk 0.
-  uint64_t out_bulk0 = total;
-  const uint64_t src_bulk0 = c.src->bulk0_rows();
-  if (tiles && src_bulk0 < rows) {
-    const uint64_t tiles0 = (src_bulk0 + tile - 1) / tile;
-    out_bulk0 = 0;
-    if (tiles0) {
-      DevBuf counts0(tiles0 * 4 + 4, c.s), offs0(tiles0 * 8 + 16, c.s);
-      SvmArgs a0 = a;
-      a0.rows = (long long)src_bulk0;
-      a0.out[6] = offs0.p;
-      a0.out[7] = counts0.p;
-      launch(jit_function(src, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s,
-             a0);
-      k::exclusive_scan_u32(counts0.as<uint32_t>(), tiles0, offs0.as<unsigned long long>(),
-                            offs0.as<unsigned long long>() + tiles0, c.s, slots.dev + 1);
-      SVM_HIP(hipStreamSynchronize(c.s));
-      out_bulk0 = slots.host[1];
-    }
-  }
+  if (tiles > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "select source too large");
   // output object: the sink schema; only projected columns are materialised (the
   // reference reserves the trailing column for the join hash table, core/pipeline.cu:94)
   std::vector<std::pair<int, int64_t>> oschema;
   std::vector<bool> mat;
+  uint64_t row_bytes = 0;
   for (size_t q = 0; q < outcols.size(); ++q) {
     int64_t vs = outcols[q].val_size > 0 ? outcols[q].val_size : 4;
     oschema.push_back({outcols[q].type, vs});
     mat.push_back((int)q < P);
+    if ((int)q < P) row_bytes += (uint64_t)vs;
   }
-  TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat, c.s);
-  out->max_tuple_count = c.E.cfg.tuple_per_bulk;
-  out->bulk0 = out_bulk0;
-  for (int q = 0; q < P; ++q) {
-    a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
-    // a bare source column projected unchanged keeps a subset of that column's values
-    int obj = -1, col = -1;
-    std::string e = p.select_exp[q];
-    size_t b = e.find_first_not_of(" \t"), t = e.find_last_not_of(" \t");
-    if (b != std::string::npos && sscanf(e.substr(b, t - b + 1).c_str(), "$(%d.%d)", &obj, &col) == 2 &&
-        e.substr(b, t - b + 1) == "$(" + std::to_string(obj) + "." + std::to_string(col) + ")" && obj == 0 &&
-        col < (int)c.schemas[0].size() && c.schemas[0][col].first == oschema[q].first &&
-        c.schemas[0][col].second == oschema[q].second) {
-      out->cols[q].origin = c.src;
-      out->cols[q].origin_col = col;
+  // A source of one bulk is compacted in one launch into an output sized for every
+  // row (trimmed to the passing rows once counted) when that capacity is modest;
+  // otherwise the tile counts are scanned first and the output is sized exactly.
+  const uint64_t src_bulk0 = c.src->bulk0_rows();
+  const bool single = tiles && tiles <= kSingleSelectTiles && src_bulk0 >= rows && rows * row_bytes <= kSingleSelectBytes;
+  SvmArgs a;
+  DevBuf blob;
+  fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
+  SVM_HIP(hipEventRecord(ev1.e, c.s));
+  uint64_t total = 0;
+  std::string src;
+  const HostSlots slots = host_slots(c.s);
+  TablePtr out;
+  // projected columns into the kernel arguments; a bare source column projected
+  // unchanged keeps a subset of that column's values
+  auto bind_outputs = [&]() {
+    out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+    for (int q = 0; q < P; ++q) {
+      a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
+      int obj = -1, col = -1;
+      std::string e = p.select_exp[q];
+      size_t b = e.find_first_not_of(" \t"), t = e.find_last_not_of(" \t");
+      if (b != std::string::npos && sscanf(e.substr(b, t - b + 1).c_str(), "$(%d.%d)", &obj, &col) == 2 &&
+          e.substr(b, t - b + 1) == "$(" + std::to_string(obj) + "." + std::to_string(col) + ")" && obj == 0 &&
+          col < (int)c.schemas[0].size() && c.schemas[0][col].first == oschema[q].first &&
+          c.schemas[0][col].second == oschema[q].second) {
+        out->cols[q].origin = c.src;
+        out->cols[q].origin_col = col;
+      }
     }
-  }
-  if (tiles && total) {
-    hipFunction_t fwrite = jit_function(src, "svm_sel_write");
-    launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+  };
+  if (single) {
+    src = gen_select_source(ks);
+    hipFunction_t fsingle = jit_function(src, "svm_sel_single");
+    out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
+    bind_outputs();
+    const ScanState st = scan_state(c.s, tiles);
+    a.out[5] = slots.dev + 0;
+    a.out[7] = st.buf;
+    a.iv[46] = (int)st.epoch;
+    slots.host[3] = 0;
+    launch(fsingle, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    SVM_HIP(hipStreamSynchronize(c.s));
+    if (slots.host[3]) {
+      scan_state_reset(c.s);
+      fail(SVM_ERR_DEVICE, "select '" + c.info.id + "': the ordered compaction lost a tile status");
+    }
+    total = slots.host[0];
+    // the passing rows (and bulk 0: the source is one bulk)
+    out->rows = total;
+    out->bulk0 = total;
+  } else {
+    DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
+    a.out[6] = offs.p;
+    a.out[7] = counts.p;
+    if (tiles) {
+      src = gen_select_source(ks);
+      hipFunction_t fcount = jit_function(src, "svm_sel_count");
+      launch(fcount, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+      k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
+                            offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
+      SVM_HIP(hipStreamSynchronize(c.s));
+      total = slots.host[0];
+    }
+    // Sink bulk i holds what source bulk i produced (core/pipeline.cu:161-162,294-296), so
+    // the output's bulk 0 -- all a later join reads of it -- is the passing rows of the
+    // source's bulk 0.
+    uint64_t out_bulk0 = total;
+    if (tiles && src_bulk0 < rows) {
+      const uint64_t tiles0 = (src_bulk0 + tile - 1) / tile;
+      out_bulk0 = 0;
+      if (tiles0) {
+        DevBuf counts0(tiles0 * 4 + 4, c.s), offs0(tiles0 * 8 + 16, c.s);
+        SvmArgs a0 = a;
+        a0.rows = (long long)src_bulk0;
+        a0.out[6] = offs0.p;
+        a0.out[7] = counts0.p;
+        launch(jit_function(src, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4),
+               c.s, a0);
+        k::exclusive_scan_u32(counts0.as<uint32_t>(), tiles0, offs0.as<unsigned long long>(),
+                              offs0.as<unsigned long long>() + tiles0, c.s, slots.dev + 1);
+        SVM_HIP(hipStreamSynchronize(c.s));
+        out_bulk0 = slots.host[1];
+      }
+    }
+    out = c.E.store.create(c.info.id, oschema, total, &mat, c.s);
+    out->bulk0 = out_bulk0;
+    bind_outputs();
+    if (tiles && total) {
+      hipFunction_t fwrite = jit_function(src, "svm_sel_write");
+      launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    }
   }
   SVM_HIP(hipEventRecord(ev2.e, c.s));
   SVM_HIP(hipEventRecord(ev3.e, c.s));

@@ -32,6 +32,13 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
 // inserts (synchronises).
 uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, hipStream_t s);
 // Packed map: slot = 0 if first == INT32_MAX else 1 / codes[row]+1 / row+1 by payload.
+// The packed map of a dimension object with unique keys, built in one pass over its
+// rows (zeroes the map first).  payload 0: existence, 1: code + 1 of the row's value
+// of attribute column `attr` (type atype, width asize; strings as Hash31) in the
+// sorted dictionary dict[0..nd), 2: row + 1.
+void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int payload, const void* attr,
+                 int atype, int64_t asize, const int32_t* dict, uint64_t nd, int wbits, uint32_t* packed,
+                 hipStream_t s);
 void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
               uint32_t* packed, hipStream_t s);
 // True when two rows of keys[0..n) share a key (keys within [kmin, kmin+range)); synchronises.

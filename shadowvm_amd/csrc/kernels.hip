@@ -125,6 +125,42 @@ __global__ void k_pack(const int32_t* __restrict__ first, uint32_t range, int pa
   }
 }
 
+// The packed map of a dimension object whose keys are unique: every row is its key's
+// first row, so each row writes its own slot (payload 0: existence, 1: code + 1 of
+// the row's attribute value in the sorted dictionary, 2: row + 1).  The map must be
+// zeroed; slots narrower than 32 bits share words and are set with atomicOr.
+__global__ void k_pack_direct(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
+                              int payload, const char* __restrict__ attr, int atype, int64_t asize,
+                              const int32_t* __restrict__ dict, uint64_t nd, int wbits, uint32_t* packed) {
+  const unsigned per = 32u / (unsigned)wbits;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t off = (uint32_t)(keys[r] - kmin);
+    if (off >= range) continue;
+    uint32_t val = 1u;
+    if (payload == 2) {
+      val = (uint32_t)r + 1u;
+    } else if (payload == 1) {
+      const char* p = attr + r * (uint64_t)asize;
+      int32_t v;
+      switch (atype) {
+        case Byte: v = (int32_t)*(const signed char*)p; break;
+        case Int16: v = (int32_t)*(const short*)p; break;
+        case String: v = (int32_t)hash31(p, (int)asize); break;
+        default: v = *(const int32_t*)p; break;
+      }
+      uint64_t lo = 0, hi = nd;
+      while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (dict[mid] < v) lo = mid + 1; else hi = mid;
+      }
+      val = (uint32_t)lo + 1u;
+    }
+    if (wbits == 32) packed[off] = val;
+    else atomicOr(&packed[off / per], val << ((off % per) * (unsigned)wbits));
+  }
+}
+
 // Single-workgroup exclusive scan with a running carry (inputs up to a few million).
 template <typename T>
 __global__ void k_exscan(const T* __restrict__ in, uint64_t n, unsigned long long* out,
@@ -271,6 +307,16 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
   fill_u32(first, range, (uint32_t)INT_MAX, s);
   if (n) k_first_rows<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first, kept);
   LAUNCH_CHECK("first_rows");
+}
+
+void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int payload, const void* attr,
+                 int atype, int64_t asize, const int32_t* dict, uint64_t nd, int wbits, uint32_t* packed,
+                 hipStream_t s) {
+  const uint64_t per = 32 / (uint64_t)wbits;
+  fill_u32(packed, ((uint64_t)range + per - 1) / per, 0u, s);
+  if (n) k_pack_direct<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, payload, (const char*)attr, atype,
+                                                     asize, dict, nd, wbits, packed);
+  LAUNCH_CHECK("pack_direct");
 }
 
 void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,

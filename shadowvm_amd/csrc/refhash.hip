@@ -190,27 +190,52 @@ __global__ void k_cluster_hits(const uint32_t* __restrict__ start, uint32_t nc, 
   }
 }
 
+// Membership and clearing of a dimension object's entries: the first-row array
+// (INT_MAX = no row) or the packed map (0 = no row).
+struct FirstRows {
+  int32_t* first;
+  __device__ bool has(uint32_t off) const { return first[off] != INT_MAX; }
+  __device__ void clear(uint32_t off) const { first[off] = INT_MAX; }
+};
+struct PackedSlots {
+  uint32_t* words;
+  unsigned wbits;
+  __device__ bool has(uint32_t off) const {
+    if (wbits == 32u) return words[off] != 0u;
+    const unsigned per = 32u / wbits;
+    return ((words[off / per] >> ((off % per) * wbits)) & ((1u << wbits) - 1u)) != 0u;
+  }
+  __device__ void clear(uint32_t off) const {
+    if (wbits == 32u) { words[off] = 0u; return; }
+    const unsigned per = 32u / wbits;
+    atomicAnd(&words[off / per], ~(((1u << wbits) - 1u) << ((off % per) * wbits)));
+  }
+};
+
 // Replays, in each dropping cluster where the object holds kProbes + 1 or more rows,
-// the object's inserts (entries whose key has a row in `first`) and clears the entry of
-// every key whose insert drops.
+// the object's inserts (entries whose key has an entry in `map`) and clears the entry
+// of every key whose insert drops.
+template <typename Slots>
 __global__ void k_clear_dropped(const uint32_t* __restrict__ start, uint32_t nc, const int32_t* __restrict__ key,
-                                const uint32_t* __restrict__ home, int32_t kmin, uint32_t range, int32_t* first) {
+                                const uint32_t* __restrict__ home, int32_t kmin, uint32_t range, Slots map) {
   constexpr int kWords = kReplaySpan / 64;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     const uint32_t e0 = start[c], e1 = start[c + 1];
     uint32_t members = 0, base = 0xFFFFFFFFu;
     for (uint32_t e = e0; e < e1; ++e) {
       const uint32_t off = (uint32_t)(key[e] - kmin);
-      if (off < range && first[off] != INT_MAX) {
+      if (off < range && map.has(off)) {
         ++members;
         base = min(base, home[e]);
       }
     }
     if (members <= (uint32_t)kProbes) continue;
+    // each key occurs once in the root (unique keys), so clearing one member's entry
+    // leaves the membership of the others unchanged
     unsigned long long occ[kWords] = {};
     for (uint32_t e = e0; e < e1; ++e) {
       const uint32_t off = (uint32_t)(key[e] - kmin);
-      if (off >= range || first[off] == INT_MAX) continue;
+      if (off >= range || !map.has(off)) continue;
       uint32_t pos = home[e] - base;  // < kReplaySpan - kProbes + 1: the cluster spans <= kReplaySpan slots
       bool placed = false;
       for (int t = 0; t < kProbes; ++t, ++pos) {
@@ -221,7 +246,7 @@ __global__ void k_clear_dropped(const uint32_t* __restrict__ start, uint32_t nc,
           break;
         }
       }
-      if (!placed) first[off] = INT_MAX;
+      if (!placed) map.clear(off);
     }
   }
 }
@@ -383,9 +408,20 @@ bool ht_subset_may_drop(const HtClusters& hc, const int32_t* keys, uint64_t n, i
 void ht_clear_dropped(const HtClusters& hc, int32_t kmin, uint32_t range, int32_t* first, hipStream_t s) {
   if (!hc.dropped || !hc.clusters) return;
   if (hc.max_span > (uint32_t)kReplaySpan) fail(SVM_ERR_INVALID, "ht_clear_dropped: cluster wider than the replay window");
-  k_clear_dropped<<<grid_for(hc.clusters), kBlock, 0, s>>>(hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(),
-                                                           hc.home.as<uint32_t>(), kmin, range, first);
+  k_clear_dropped<FirstRows><<<grid_for(hc.clusters), kBlock, 0, s>>>(
+      hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(), hc.home.as<uint32_t>(), kmin, range,
+      FirstRows{first});
   hip_check(hipGetLastError(), "ht_clear_dropped");
+}
+
+void ht_clear_dropped_packed(const HtClusters& hc, int32_t kmin, uint32_t range, uint32_t* packed, int wbits,
+                             hipStream_t s) {
+  if (!hc.dropped || !hc.clusters) return;
+  if (hc.max_span > (uint32_t)kReplaySpan) fail(SVM_ERR_INVALID, "ht_clear_dropped: cluster wider than the replay window");
+  k_clear_dropped<PackedSlots><<<grid_for(hc.clusters), kBlock, 0, s>>>(
+      hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(), hc.home.as<uint32_t>(), kmin, range,
+      PackedSlots{packed, (unsigned)wbits});
+  hip_check(hipGetLastError(), "ht_clear_dropped_packed");
 }
 
 uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, hipStream_t s) {

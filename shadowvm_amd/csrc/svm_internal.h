@@ -117,9 +117,11 @@ class Store {
   // Allocates the columns.  Without a stream every byte is zeroed synchronously
   // (tables filled by the caller); with a stream only the padding after the last
   // row is zeroed, asynchronously, for outputs whose rows a kernel writes in full.
+  // zero_pad false: the caller's kernel writes the padding itself (outputs sized to a
+  // capacity whose final row count a kernel decides).
   TablePtr create(const std::string& id, const std::vector<std::pair<int, int64_t>>& schema,
                   uint64_t rows, const std::vector<bool>* materialise = nullptr,
-                  hipStream_t fill_stream = nullptr);
+                  hipStream_t fill_stream = nullptr, bool zero_pad = true);
   void put(const std::string& id, TablePtr t);
   TablePtr get(const std::string& id);  // throws SVM_ERR_NOT_FOUND
   TablePtr find(const std::string& id);
@@ -147,13 +149,27 @@ void arena_release_all();
 // Pinned, coherent host words that kernels write directly: counts and totals the
 // host reads after synchronising the stream, with no copy dispatch.  Each stream has
 // kHostSlots of them; a run owns its stream's slots until it has synchronised.
-// Slot owners: 0 = select output rows, 1 = select bulk-0 rows, 2 = group count.
+// Slot owners: 0 = select output rows, 1 = select bulk-0 rows, 2 = group count,
+// 3 = one-launch select failure (a tile found no consistent status in time).
 constexpr int kHostSlots = 16;
 struct HostSlots {
   volatile unsigned long long* host = nullptr;
   unsigned long long* dev = nullptr;
 };
 HostSlots host_slots(hipStream_t s);
+// Per-stream state of the one-launch ordered compaction (a tile ticket, which each
+// launch leaves at 0, and one status word per tile, tagged with a launch epoch so
+// the statuses are never cleared between launches): buf[0] is the ticket,
+// buf[1..tiles] the statuses.  Each call returns
+// an epoch no earlier call returned (for any stream); the buffer holds at least
+// `tiles` statuses and is cleared in stream order when new.
+struct ScanState {
+  unsigned long long* buf = nullptr;
+  uint32_t epoch = 0;
+};
+ScanState scan_state(hipStream_t s, uint64_t tiles);
+// Clears the stream's state before its next use (after a launch that failed).
+void scan_state_reset(hipStream_t s);
 // Temporary device buffer.  With a stream it lives in that stream's arena until
 // the next arena_reset; without one it is a plain (cached) device allocation.
 struct DevBuf {
