@@ -238,7 +238,7 @@ void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
 
 }  // namespace
 
-std::string gen_scan_source(const KernelShape& s) {
+std::string gen_scan_source(const KernelShape& s, const char* entry) {
   std::ostringstream o;
   emit_prologue(o, s);
   const int M = (int)s.metrics.size();
@@ -248,7 +248,7 @@ std::string gen_scan_source(const KernelShape& s) {
     o << "#define SVM_HSLOTS " << (1 << s.hash_bits) << "\n";
     o << "#define SVM_HBITS " << s.hash_bits << "\n";
   }
-  o << "extern \"C\" __global__ void __launch_bounds__(SVM_BLOCK) svm_scan(SvmArgs a) {\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(SVM_BLOCK) " << entry << "(SvmArgs a) {\n";
   o << "  extern __shared__ svm_u64 svm_lds64[];\n  unsigned* svm_lds = (unsigned*)svm_lds64;\n";
   emit_colptrs(o, s);
   emit_stage_maps(o, s);

@@ -65,7 +65,9 @@ struct KernelShape {
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs
 };
 
-std::string gen_scan_source(const KernelShape& s);
+// entry: the kernel's symbol (the autotuner's trial launches use "svm_scan_tune",
+// so that profiles tell them from the query's own scans)
+std::string gen_scan_source(const KernelShape& s, const char* entry = "svm_scan");
 std::string gen_select_source(const KernelShape& s);
 const char* device_runtime_text();
 

@@ -426,7 +426,9 @@ const TuneChoice kTuneCandidates[] = {
     {1024, 1, 0, true, false, true},  {512, 1, 0, false, false, true},
     // two workgroups per CU: a plain read of several columns streams fastest with
     // few waves, a scan with dependent map lookups needs more to hide them
-    {512, 1, 2, true, false, false},  {512, 2, 2, true, false, false}};
+    {512, 1, 2, true, false, false},  {512, 2, 2, true, false, false},
+    // the flat form with the next groups' loads in flight during this group's lookups
+    {1024, 1, 0, true, true, true},   {512, 1, 0, true, true, true},    {1024, 2, 0, true, false, true}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -601,7 +603,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         KernelShape k2;
         uint64_t w2 = 0;
         shape_for(cand, k2, w2);
-        hipFunction_t f2 = jit_function(gen_scan_source(k2), "svm_scan");
+        hipFunction_t f2 = jit_function(gen_scan_source(k2, "svm_scan_tune"), "svm_scan_tune");
         unsigned g2 = scan_grid(c.E, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
         clear_outputs();
         Event ta, tb;
