@@ -36,8 +36,26 @@ __device__ __forceinline__ unsigned StrHash(const char* p, int n) {
   for (int k = 0; k < n && p[k] != '\0'; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
   return h & 0x7FFFFFFFu;
 }
-/* Prefix equality: 0 once either side reaches NUL (com/cpp/cuda-runtime.cu:34-54). */
+/* Prefix equality: 0 once either side reaches NUL, else the first byte difference
+   (com/cpp/cuda-runtime.cu:34-54).  Up to 64 bytes (the widths of fixed string columns;
+   every operand is a row of a padded column or a zero-tailed literal) all bytes are
+   read first and the answer is selected without branches, so the loads issue together;
+   the result is the same as the early-exit loop's. */
 __device__ __forceinline__ int StrCmp(const char* a, const char* b, int n) {
+  if (n <= 64) {
+    int r = 0;
+    bool done = false;
+    #pragma unroll
+    for (int k = 0; k < 64; ++k) {
+      if (k >= n) break;
+      const char x = a[k], y = b[k];
+      const int d = (int)(signed char)x - (int)(signed char)y;
+      const bool stop = (x == '\0') | (y == '\0');
+      r = (!done && !stop && d != 0) ? d : r;
+      done = done | stop | (d != 0);
+    }
+    return r;
+  }
   for (int k = 0; k < n; ++k) {
     char x = a[k], y = b[k];
     if (x == '\0' || y == '\0') return 0;
