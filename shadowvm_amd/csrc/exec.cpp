@@ -51,10 +51,30 @@ const char* ctype(int t) {
   }
 }
 
+// Timing events are reused across runs (a run synchronises its stream before it
+// returns, so an event handed back is idle; re-recording one is allowed anyway).
+std::mutex g_event_mu;
+std::vector<hipEvent_t> g_event_pool;
 struct Event {
   hipEvent_t e = nullptr;
-  Event() { SVM_HIP(hipEventCreate(&e)); }
-  ~Event() { if (e) (void)hipEventDestroy(e); }
+  Event() {
+    {
+      std::lock_guard<std::mutex> g(g_event_mu);
+      if (!g_event_pool.empty()) {
+        e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return;
+      }
+    }
+    SVM_HIP(hipEventCreate(&e));
+  }
+  ~Event() {
+    if (!e) return;
+    std::lock_guard<std::mutex> g(g_event_mu);
+    g_event_pool.push_back(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
 };
 
 double host_ms(std::chrono::steady_clock::time_point t0) {
@@ -548,11 +568,12 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   // outputs
   const uint64_t nflag = (G + 31) / 32;
   DevBuf gsum(K ? G * M * 8 + 8 : 8, c.s), gflag(nflag * 4 + 4, c.s), ssum(8 * 8, c.s);
-  if (K) {
-    k::fill_u32(gsum.p, G * M * 2, 0, c.s);
-    k::fill_u32(gflag.p, nflag, 0, c.s);
-  }
-  k::fill_u32(ssum.p, 16, 0, c.s);
+  auto clear_outputs = [&] {
+    void* ptrs[3] = {ssum.p, gsum.p, gflag.p};
+    const uint64_t words[3] = {16, K ? G * M * 2 : 0, K ? nflag : 0};
+    k::zero_ranges(ptrs, words, 3, c.s);
+  };
+  clear_outputs();
   SvmArgs a;
   DevBuf blob;
   fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
@@ -570,13 +591,6 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   const uint64_t rows = c.src->rows;
   const bool run_scan = rows > 0 && (K == 0 || G > 0);
   const uint64_t vec = (rows + 3) / 4;
-  auto clear_outputs = [&] {
-    if (K) {
-      k::fill_u32(gsum.p, G * M * 2, 0, c.s);
-      k::fill_u32(gflag.p, nflag, 0, c.s);
-    }
-    k::fill_u32(ssum.p, 16, 0, c.s);
-  };
   TuneChoice choice;
   choice.block = ks.block;
   choice.unroll = ks.unroll;

@@ -65,6 +65,9 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scra
                      unsigned long long* scratch_offsets, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
 
+// Zeroes words[r] 32-bit words at ptrs[r] for r < n (n <= 4), in one launch.
+void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s);
+
 // Zeroes the 256 padding bytes behind each of `n` column ends, in one launch per 32.
 constexpr int kMaxPads = 32;
 constexpr int kPadBytes = 256;

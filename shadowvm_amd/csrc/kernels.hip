@@ -234,6 +234,17 @@ __global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ 
   for (int m2 = 0; m2 < a.M; ++m2) a.sums_out[m2][row] = (int64_t)a.sums[g * (uint64_t)a.M + m2];
 }
 
+struct ZeroRanges {
+  uint32_t* p[4];
+  uint64_t n[4];
+};
+__global__ void k_zero_ranges(ZeroRanges z) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int r = 0; r < 4; ++r)
+    for (uint64_t i = t; i < z.n[r]; i += stride) z.p[r][i] = 0u;
+}
+
 // Zeroes kColumnPad bytes at each pointer (one block per pointer).
 __global__ void k_zero_pads(PadList l) {
   if (blockIdx.x < (unsigned)l.n && threadIdx.x < kPadBytes) l.p[blockIdx.x][threadIdx.x] = 0;
@@ -366,6 +377,20 @@ void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_
   if (nb > 0x7FFFFFFFull) fail(SVM_ERR_UNSUPPORTED, "group_emit: group space too large");
   k_group_emit<<<(unsigned)nb, kBlock, 0, s>>>(a, offsets);
   LAUNCH_CHECK("group_emit");
+}
+
+void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s) {
+  if (n < 0 || n > 4) fail(SVM_ERR_INVALID, "zero_ranges: at most 4 ranges");
+  ZeroRanges z;
+  uint64_t most = 0;
+  for (int r = 0; r < 4; ++r) {
+    z.p[r] = r < n ? (uint32_t*)ptrs[r] : nullptr;
+    z.n[r] = r < n && ptrs[r] ? words[r] : 0;
+    most = z.n[r] > most ? z.n[r] : most;
+  }
+  if (!most) return;
+  k_zero_ranges<<<grid_for(most), kBlock, 0, s>>>(z);
+  LAUNCH_CHECK("zero_ranges");
 }
 
 void zero_pads(char* const* ptrs, int n, hipStream_t s) {
