@@ -453,11 +453,6 @@ const TuneChoice kTuneCandidates[] = {
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
 
-uint64_t text_hash(const std::string& s) {
-  uint64_t h = 1469598103934665603ull;
-  for (unsigned char ch : s) { h ^= ch; h *= 1099511628211ull; }
-  return h;
-}
 
 bool autotune_enabled() {
   const char* e = getenv("SVM_AUTOTUNE");
@@ -607,7 +602,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     words = t.stage ? lds_words : agg_words;
   };
   if (run_scan && autotune_enabled() && rows >= kTuneMinRows && opt.block == 256 && opt.unroll == 1) {
-    const uint64_t tkey = text_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
+    const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
     std::unique_lock<std::mutex> lk(g_tune_mu);
     TuneState& st = g_tune[tkey];
     if (!st.done) {

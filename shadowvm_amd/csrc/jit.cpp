@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -31,14 +32,32 @@ std::mutex g_mu;
 std::map<uint64_t, Module> g_mods;
 std::atomic<uint64_t> g_compiles{0};
 
-uint64_t fnv1a(const std::string& s) {
-  uint64_t h = 1469598103934665603ull;
-  for (unsigned char c : s) {
-    h ^= c;
-    h *= 1099511628211ull;
+
+}  // namespace
+
+// 64-bit multiply-xorshift hash, eight bytes per step (a kernel source is hashed on
+// every run of a pipeline, so this stays a few microseconds for 10 KB).
+uint64_t source_hash(const std::string& s) {
+  const uint64_t k = 0x9E3779B97F4A7C15ull;
+  const size_t n = s.size();
+  uint64_t h = 0xCBF29CE484222325ull ^ ((uint64_t)n * k);
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, s.data() + i, 8);
+    h = (h ^ w) * k;
+    h ^= h >> 32;
   }
+  uint64_t w = 0;
+  memcpy(&w, s.data() + i, n - i);
+  h = (h ^ w) * k;
+  h ^= h >> 29;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 32;
   return h;
 }
+
+namespace {
 
 std::string cache_dir() {
   const char* d = getenv("SVM_JIT_CACHE");
@@ -89,7 +108,7 @@ std::string compile(const std::string& src, uint64_t h) {
 }  // namespace
 
 std::string jit_code_object(const std::string& src) {
-  uint64_t h = fnv1a(src);
+  uint64_t h = source_hash(src);
   // SVM_JIT_DUMP=<dir>: keep every generated source (for offline inspection)
   const char* dump = getenv("SVM_JIT_DUMP");
   if (dump && *dump) {
@@ -104,7 +123,7 @@ std::string jit_code_object(const std::string& src) {
 }
 
 hipFunction_t jit_function(const std::string& src, const char* name) {
-  uint64_t h = fnv1a(src);
+  uint64_t h = source_hash(src);
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_mods.find(h);
   if (it == g_mods.end()) {

@@ -14,6 +14,8 @@ hipFunction_t jit_function(const std::string& src, const char* name);
 // Code object for `src` (compiled if not cached); used to warm the cache.
 std::string jit_code_object(const std::string& src);
 uint64_t jit_compile_count();
+// The hash a kernel source is cached under.
+uint64_t source_hash(const std::string& s);
 void launch(hipFunction_t fn, unsigned grid, unsigned block, unsigned lds, hipStream_t s,
             const SvmArgs& args);
 
