@@ -160,6 +160,29 @@ ScanState scan_state(hipStream_t s, uint64_t tiles) {
   return st;
 }
 
+namespace {
+struct Staging {
+  void* p = nullptr;
+  uint64_t cap = 0;
+};
+std::map<hipStream_t, Staging> g_staging;  // guarded by g_arena_mu
+}  // namespace
+
+void* host_staging(hipStream_t s, uint64_t bytes) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  Staging& st = g_staging[s];
+  if (st.cap < bytes) {
+    if (st.p) (void)hipHostFree(st.p);
+    uint64_t cap = 1 << 20;
+    while (cap < bytes) cap <<= 1;
+    st.p = nullptr;
+    st.cap = 0;
+    SVM_HIP(hipHostMalloc(&st.p, cap, hipHostMallocDefault));
+    st.cap = cap;
+  }
+  return st.p;
+}
+
 void scan_state_reset(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   auto it = g_scan_bufs.find(s);
@@ -191,6 +214,8 @@ void arena_release_all() {
   g_host_slots.clear();
   for (auto& kv : g_scan_bufs) (void)hipFree(kv.second.buf);
   g_scan_bufs.clear();
+  for (auto& kv : g_staging) (void)hipHostFree(kv.second.p);
+  g_staging.clear();
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {

@@ -77,6 +77,41 @@ struct Event {
   Event& operator=(const Event&) = delete;
 };
 
+// The first `ncols` columns of an output table copied to host vectors once the work
+// on stream `s` is done (this synchronises `s`): through the stream's pinned staging
+// buffer with one synchronisation when they fit in kStagedCopyBytes, else one
+// synchronous copy per column.
+constexpr uint64_t kStagedCopyBytes = 64ull << 20;
+void copy_out(const Table& out, const std::vector<std::pair<int, int64_t>>& schema, int ncols, uint64_t rows,
+              hipStream_t s, std::vector<std::vector<uint8_t>>& cols) {
+  uint64_t total = 0;
+  for (int q = 0; q < ncols; ++q) total += rows * (uint64_t)schema[q].second;
+  cols.clear();
+  if (total && total <= kStagedCopyBytes) {
+    char* st = (char*)host_staging(s, total);
+    uint64_t off = 0;
+    for (int q = 0; q < ncols; ++q) {
+      const uint64_t b = rows * (uint64_t)schema[q].second;
+      if (b) SVM_HIP(hipMemcpyAsync(st + off, out.cols[q].dev, b, hipMemcpyDeviceToHost, s));
+      off += b;
+    }
+    SVM_HIP(hipStreamSynchronize(s));
+    off = 0;
+    for (int q = 0; q < ncols; ++q) {
+      const uint64_t b = rows * (uint64_t)schema[q].second;
+      cols.emplace_back(st + off, st + off + b);
+      off += b;
+    }
+    return;
+  }
+  SVM_HIP(hipStreamSynchronize(s));  // the copies below are not ordered after the stream's work
+  for (int q = 0; q < ncols; ++q) {
+    std::vector<uint8_t> h(rows * (uint64_t)schema[q].second);
+    if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out.cols[q].dev, h.size(), hipMemcpyDeviceToHost));
+    cols.push_back(std::move(h));
+  }
+}
+
 double host_ms(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -694,17 +729,13 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   }
   out->max_tuple_count = c.E.cfg.tuple_per_bulk;
   SVM_HIP(hipEventRecord(ev3.e, c.s));
-  SVM_HIP(hipStreamSynchronize(c.s));
   res->kind = K == 0 ? 2 : 1;
   res->rows = (int64_t)out_rows;
   res->types = oschema;
-  if (opt.want_host) {
-    for (size_t q = 0; q < oschema.size(); ++q) {
-      std::vector<uint8_t> h(out_rows * (uint64_t)oschema[q].second);
-      if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out->cols[q].dev, h.size(), hipMemcpyDeviceToHost));
-      res->cols.push_back(std::move(h));
-    }
-  }
+  if (opt.want_host)
+    copy_out(*out, oschema, (int)oschema.size(), out_rows, c.s, res->cols);
+  else
+    SVM_HIP(hipStreamSynchronize(c.s));
   res->build_ms = elapsed(ev0, ev1);
   res->scan_ms = elapsed(ev1, ev2);
   res->finish_ms = elapsed(ev2, ev3);
@@ -884,17 +915,13 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   }
   SVM_HIP(hipEventRecord(ev2.e, c.s));
   SVM_HIP(hipEventRecord(ev3.e, c.s));
-  SVM_HIP(hipStreamSynchronize(c.s));
   res->kind = 0;
   res->rows = (int64_t)total;
   for (int q = 0; q < P; ++q) res->types.push_back(oschema[q]);
-  if (opt.want_host) {
-    for (int q = 0; q < P; ++q) {
-      std::vector<uint8_t> h(total * (uint64_t)oschema[q].second);
-      if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out->cols[q].dev, h.size(), hipMemcpyDeviceToHost));
-      res->cols.push_back(std::move(h));
-    }
-  }
+  if (opt.want_host)
+    copy_out(*out, oschema, P, total, c.s, res->cols);
+  else
+    SVM_HIP(hipStreamSynchronize(c.s));
   res->build_ms = elapsed(ev0, ev1);
   res->scan_ms = elapsed(ev1, ev2);
   res->finish_ms = elapsed(ev2, ev3);
