@@ -151,8 +151,7 @@ int width_for(uint64_t maxval) {
 }
 
 // Group-key component of a dimension: the dictionary of the column's origin (the
-// base table it was projected from, cached across queries, so no host round trip)
-// and, by binary search, the code of every row of this dimension object.
+// base table it was projected from, cached across queries, so no host round trip).
 void key_dict(const TablePtr& D, int col, KeyBuild& kb, hipStream_t s) {
   const Column& c = D->cols[col];
   if (!c.dev) fail(SVM_ERR_INVALID, "group key column is not materialised");
@@ -163,6 +162,7 @@ void key_dict(const TablePtr& D, int col, KeyBuild& kb, hipStream_t s) {
   kb.n = kb.dict->n;
 }
 
+// ... and, by binary search, the code of every row of this dimension object.
 void build_key_dict(const TablePtr& D, int col, KeyBuild& kb, DevBuf* codes_out, hipStream_t s) {
   key_dict(D, col, kb, s);
   const Column& c = D->cols[col];
