@@ -118,3 +118,18 @@ def test_generated_kernels_compile_for_gfx950(tmp_path, monkeypatch):
     assert rc == 0, report
     assert report.count(": ok") == 10
     assert len(os.listdir(tmp_path)) == 10  # code objects cached per source
+
+
+def test_ssb_query_plans_decode_and_lower():
+    """Every plan of the thirteen SSB query tests (dimension and fact pipelines) is
+    decoded and its expressions lowered without a GPU."""
+    from .ssb_queries import NAMES, queries
+    ids = {k: f"ssbq_{k}" for k in ("lineorder", "date", "customer", "supplier", "part")}
+    q = queries("ssbq_", ids)
+    for name in NAMES:
+        dims, fact = q[name]
+        for pt, src in dims:
+            rc, js = describe(pt.plan(src))
+            assert rc == 0, (name, pt.pipe_info["id"], js)
+        rc, js = describe(fact.plan(ids["lineorder"], [pt.pipe_info["id"] for pt, _ in dims]))
+        assert rc == 0, (name, js)
