@@ -217,7 +217,16 @@ def queries(prefix, t):
         f = _fact(f"{prefix}{name}", None, [("$(0.4)", "$(2.0)"), ("$(0.5)", "$(1.0)"), ("$(0.2)", "$(3.0)"),
                                             ("$(0.3)", "$(4.0)")], "$(1.1), $(4.1))", "($(0.12)-$(0.13))/100", 12)
         q[name] = ([(d, t["date"]), (p, t["part"]), (s, t["supplier"]), (c, t["customer"])], f)
+    # test-query.cu:30-95 (test_query_SSB_Q2_1): a fact filter with a string predicate on
+    # lo_shipmode; the test passes only the date object, so its second join names a ref
+    # that does not exist -- here the part object of the same file is passed as that ref
+    d = _dim(f"{prefix}QRY_date", DATE, "$(0.4)>=1992 && $(0.4)<=1997", "$(0.0), $(0.4)", [I, I, S], [4, 4, 40])
+    p = _dim(f"{prefix}QRY_part", PART, 'StrCmp($(0.3),"MFGR#2221",8)==0 ', "$(0.0), $(0.2)", [I, S, S], [4, 10, 40])
+    f = _fact(f"{prefix}QRY", "$(0.5)>=19930000 && $(0.5)<19940000 && $(0.8)<25 && $(0.11)>=1 && $(0.11)<=3 && "
+              'StrCmp($(0.16),"AIR", 10)==0', [("$(0.5)", "$(1.0)"), ("$(0.3)", "$(2.0)")], "$(1.1), $(2.1)",
+              "$(0.12)/100", 9)
+    q["QRY"] = ([(d, t["date"]), (p, t["part"])], f)
     return q
 
 
-NAMES = ["Q11", "Q12", "Q13", "Q21", "Q22", "Q23", "Q31", "Q32", "Q33", "Q34", "Q41", "Q42", "Q43"]
+NAMES = ["Q11", "Q12", "Q13", "Q21", "Q22", "Q23", "Q31", "Q32", "Q33", "Q34", "Q41", "Q42", "Q43", "QRY"]
