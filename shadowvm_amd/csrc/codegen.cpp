@@ -190,6 +190,17 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
         << ind << "const long long JI" << J.robj << " = (long long)p" << j << " - 1;\n";
       continue;
     }
+    if (J.defer) {
+      // existence bit first; the payload only for keys that exist
+      o << ind << "unsigned p" << j << " = 0u;\n"
+        << ind << "if (cond) {\n"
+        << ind << "  const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+        << ind << "  if (off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<1>(m" << j << " + a.iv["
+        << 4 * j + 2 << "], off)) p" << j << " = svm_map_get<" << J.wbits << ">(m" << j << ", off);\n"
+        << ind << "  cond = p" << j << " != 0u;\n"
+        << ind << "}\n";
+      continue;
+    }
     o << ind << "unsigned p" << j << " = 0u;\n"
       << ind << "if (cond) {\n"
       << ind << "  const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
@@ -344,6 +355,16 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       << "      cond = cond && (" << s.filter << ");\n";
     for (size_t j = 0; j < s.joins.size(); ++j) {
       const JoinShape& J = s.joins[j];
+      if (J.defer) {
+        // the existence bit decides; the slot offset is kept and the payload read
+        // only for rows that pass every join
+        o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+          << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
+          << "        const unsigned b = svm_map_get<1>(m" << j << " + a.iv[" << 4 * j + 2 << "], in ? off : 0u);\n"
+          << "        cond = in && b != 0u;\n"
+          << "        pp" << j << "_" << u << "[e] = off; }\n";
+        continue;
+      }
       o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
         << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
         << "        unsigned p = svm_map_get<" << J.wbits << ">(m" << j << ", in ? off : 0u);\n"
@@ -357,7 +378,11 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
     for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
     for (size_t j = 0; j < s.joins.size(); ++j) {
-      o << "      const unsigned p" << j << " = pp" << j << "_" << u << "[e];\n      (void)p" << j << ";\n";
+      if (s.joins[j].defer)
+        o << "      const unsigned p" << j << " = svm_map_get<" << s.joins[j].wbits << ">(m" << j << ", pp" << j
+          << "_" << u << "[e]);\n      (void)p" << j << ";\n";
+      else
+        o << "      const unsigned p" << j << " = pp" << j << "_" << u << "[e];\n      (void)p" << j << ";\n";
       if (s.joins[j].payload == PAY_ROW)
         o << "      const long long JI" << s.joins[j].robj << " = (long long)p" << j << " - 1;\n";
     }

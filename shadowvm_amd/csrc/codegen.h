@@ -27,6 +27,10 @@ struct JoinShape {
   int payload = PAY_EXIST;
   int wbits = 1;     // 1,2,4,8,16 or 32 bits per key slot
   bool lds = false;  // map staged in LDS at kernel start
+  // The scan tests a 1-bit existence map first (stored after the packed words, at
+  // word a.iv[4j+2]) and reads the packed payload only for rows that pass every
+  // join: a large code map then stays out of the probe chain.
+  bool defer = false;
   int kind = JOIN_MAP;
   int rcol = 0;      // probes: the dimension's key column (read as DCOL<robj>_<rcol>)
   int ktype = 2;     // probes: key type (ValType) and width in bytes

@@ -40,6 +40,10 @@ constexpr uint64_t kSingleSelectBytes = 256ull << 20;
 constexpr uint64_t kSingleSelectTiles = 256;
 // Rows above which a dense map with sub-byte slots is built by key, not by row.
 constexpr uint64_t kDirectNarrowRows = 1ull << 16;
+// Code maps larger than this are probed through an existence bitmap first (the scan
+// reads the code only for rows that pass every join), so the probe chain touches
+// 1 bit per key instead of the code width.
+constexpr uint64_t kDeferMapBytes = 128 * 1024;
 
 const char* ctype(int t) {
   switch (t) {
@@ -214,7 +218,7 @@ void resolve(Ctx& c) {
 // Joins: validation and dense-map builds.  `exprs_refs` are (obj,col) operands read
 // by metric / projection expressions (those dimensions need row-id payloads).
 std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& expr_refs,
-                                   std::vector<KeyBuild>& keys) {
+                                   std::vector<KeyBuild>& keys, bool allow_defer) {
   std::vector<JoinBuild> joins;
   std::set<int> seen;
   for (size_t j = 0; j < c.plan.join_rkey.size(); ++j) {
@@ -319,7 +323,14 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     }
     uint64_t per = 32 / (uint64_t)jb.shape.wbits;
     jb.words = ((uint64_t)jb.range + per - 1) / per;
-    jb.packed = DevBuf(jb.words * 4 + 16, c.s);
+    // a large code map is probed through its existence bits (stored after it)
+    jb.shape.defer = allow_defer && payload == PAY_CODE && jb.words * 4 > kDeferMapBytes;
+    const uint64_t bitwords = jb.shape.defer ? ((uint64_t)jb.range + 31) / 32 : 0;
+    jb.packed = DevBuf((jb.words + bitwords) * 4 + 16, c.s);
+    auto finish_map = [&](JoinBuild& b) {
+      if (b.shape.defer)
+        k::exist_bits(b.packed.as<uint32_t>(), b.shape.wbits, b.range, b.packed.as<uint32_t>() + b.words, c.s);
+    };
     if (jb.range) {
       // The reference's table drops inserts that find no free slot in 10 probes; a
       // dropped row is never found, so the map takes the first kept row of each key.
@@ -335,6 +346,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
                        jb.shape.wbits, jb.packed.as<uint32_t>(), c.s);
         if (hc && n0 > 10)
           k::ht_clear_dropped_packed(*hc, jb.kmin, jb.range, jb.packed.as<uint32_t>(), jb.shape.wbits, c.s);
+        finish_map(jb);
         joins.push_back(std::move(jb));
         continue;
       }
@@ -349,6 +361,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
       k::pack_map(first.as<int32_t>(), jb.range, payload, codes_for_pack.as<int32_t>(), jb.shape.wbits,
                   jb.packed.as<uint32_t>(), c.s);
       // `first` and the codes are released in stream order after pack_map
+      finish_map(jb);
     }
     joins.push_back(std::move(jb));
   }
@@ -358,7 +371,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
 void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t& lds_words) {
   uint64_t used = (agg_bytes + 7) / 8 * 2;  // agg table first, 8-byte aligned, in words
   for (auto& jb : joins) {
-    if (jb.words && (used + jb.words) * 4 <= kLdsBudget) {
+    if (jb.words && !jb.shape.defer && (used + jb.words) * 4 <= kLdsBudget) {
       jb.shape.lds = true;
       jb.lds_off = (uint32_t)used;
       used += jb.words;
@@ -554,7 +567,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     for (auto& r : p.join_rkey) joined = joined || r.obj == o.first;
     if (!joined) fail(SVM_ERR_INVALID, "metric reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
   }
-  std::vector<JoinBuild> joins = build_joins(c, metric_ops, keys);
+  std::vector<JoinBuild> joins = build_joins(c, metric_ops, keys, /*allow_defer=*/true);
   // probe joins compare against the dimension's key column
   for (auto& jb : joins)
     if (jb.table) metric_ops.insert({jb.shape.robj, jb.shape.rcol});
@@ -794,7 +807,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     if (!joined) fail(SVM_ERR_INVALID, "projection reads ref object " + std::to_string(o.first) + " which no JoinWith probes");
   }
   std::vector<KeyBuild> nokeys;
-  std::vector<JoinBuild> joins = build_joins(c, proj_ops, nokeys);
+  std::vector<JoinBuild> joins = build_joins(c, proj_ops, nokeys, /*allow_defer=*/false);
   // probe joins compare against the dimension's key column
   for (auto& jb : joins)
     if (jb.table) proj_ops.insert({jb.shape.robj, jb.shape.rcol});

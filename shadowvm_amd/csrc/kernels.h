@@ -32,6 +32,10 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
 // inserts (synchronises).
 uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept, hipStream_t s);
 // Packed map: slot = 0 if first == INT32_MAX else 1 / codes[row]+1 / row+1 by payload.
+// bits[0 .. ceil(range/32)): bit s set when slot s of the packed map (wbits per slot)
+// holds a payload.
+void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bits, hipStream_t s);
+
 // The packed map of a dimension object with unique keys, built in one pass over its
 // rows (zeroes the map first).  payload 0: existence, 1: code + 1 of the row's value
 // of attribute column `attr` (type atype, width asize; strings as Hash31) in the

@@ -161,6 +161,24 @@ __global__ void k_pack_direct(const int32_t* __restrict__ keys, uint64_t n, int3
   }
 }
 
+// Existence bits of a packed map: bit s of bits[] is set when slot s holds a payload.
+__global__ void k_exist_bits(const uint32_t* __restrict__ packed, int wbits, uint32_t range, uint32_t* bits,
+                             uint64_t nbw) {
+  const unsigned per = 32u / (unsigned)wbits;
+  const uint32_t mask = wbits == 32 ? 0xFFFFFFFFu : (1u << wbits) - 1u;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nbw;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t out = 0;
+    for (unsigned e = 0; e < 32u; ++e) {
+      const uint64_t slot = w * 32u + e;
+      if (slot >= range) break;
+      const uint32_t v = wbits == 32 ? packed[slot] : (packed[slot / per] >> ((slot % per) * (unsigned)wbits)) & mask;
+      out |= (v != 0u ? 1u : 0u) << e;
+    }
+    bits[w] = out;
+  }
+}
+
 // Single-workgroup exclusive scan with a running carry (inputs up to a few million).
 template <typename T>
 __global__ void k_exscan(const T* __restrict__ in, uint64_t n, unsigned long long* out,
@@ -318,6 +336,13 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
   fill_u32(first, range, (uint32_t)INT_MAX, s);
   if (n) k_first_rows<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first, kept);
   LAUNCH_CHECK("first_rows");
+}
+
+void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bits, hipStream_t s) {
+  const uint64_t nbw = ((uint64_t)range + 31) / 32;
+  if (!nbw) return;
+  k_exist_bits<<<grid_for(nbw), kBlock, 0, s>>>(packed, wbits, range, bits, nbw);
+  LAUNCH_CHECK("exist_bits");
 }
 
 void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int payload, const void* attr,

@@ -62,6 +62,13 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.unroll = 2;
     s.flat = true;
     out.push_back({"scan_hash_flat", gen_scan_source(s)});
+    // the large code maps probed through their existence bits
+    s.joins[1].defer = true;
+    s.joins[2].defer = true;
+    out.push_back({"scan_hash_flat_deferred", gen_scan_source(s)});
+    s.flat = false;
+    s.unroll = 1;
+    out.push_back({"scan_hash_deferred", gen_scan_source(s)});
   }
   {  // row-id payload, dimension attribute in the metric, source-column key
     KernelShape s;
