@@ -340,13 +340,13 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
       // (and cached) for the object itself.
       if (direct) {
         const Column* ac = direct_code ? &D.cols[direct_code->shape.col] : nullptr;
+        uint32_t* bits = jb.shape.defer ? jb.packed.as<uint32_t>() + jb.words : nullptr;
         k::pack_direct((const int32_t*)kcol, n0, jb.kmin, jb.range, payload,
                        ac ? D.col_ptr(direct_code->shape.col) : nullptr, ac ? ac->type : Int32, ac ? ac->size : 4,
                        direct_code ? direct_code->dict->dict.as<int32_t>() : nullptr, direct_code ? direct_code->n : 0,
-                       jb.shape.wbits, jb.packed.as<uint32_t>(), c.s);
+                       jb.shape.wbits, jb.packed.as<uint32_t>(), bits, c.s);
         if (hc && n0 > 10)
-          k::ht_clear_dropped_packed(*hc, jb.kmin, jb.range, jb.packed.as<uint32_t>(), jb.shape.wbits, c.s);
-        finish_map(jb);
+          k::ht_clear_dropped_packed(*hc, jb.kmin, jb.range, jb.packed.as<uint32_t>(), jb.shape.wbits, bits, c.s);
         joins.push_back(std::move(jb));
         continue;
       }

@@ -40,9 +40,10 @@ void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bit
 // rows (zeroes the map first).  payload 0: existence, 1: code + 1 of the row's value
 // of attribute column `attr` (type atype, width asize; strings as Hash31) in the
 // sorted dictionary dict[0..nd), 2: row + 1.
+// bits (optional, = packed + its word count): the map's existence bits, set as well.
 void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int payload, const void* attr,
                  int atype, int64_t asize, const int32_t* dict, uint64_t nd, int wbits, uint32_t* packed,
-                 hipStream_t s);
+                 uint32_t* bits, hipStream_t s);
 void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* codes, int wbits,
               uint32_t* packed, hipStream_t s);
 // True when two rows of keys[0..n) share a key (keys within [kmin, kmin+range)); synchronises.

@@ -131,7 +131,8 @@ __global__ void k_pack(const int32_t* __restrict__ first, uint32_t range, int pa
 // zeroed; slots narrower than 32 bits share words and are set with atomicOr.
 __global__ void k_pack_direct(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
                               int payload, const char* __restrict__ attr, int atype, int64_t asize,
-                              const int32_t* __restrict__ dict, uint64_t nd, int wbits, uint32_t* packed) {
+                              const int32_t* __restrict__ dict, uint64_t nd, int wbits, uint32_t* packed,
+                              uint32_t* bits) {
   const unsigned per = 32u / (unsigned)wbits;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (uint64_t)gridDim.x * blockDim.x) {
@@ -158,6 +159,7 @@ __global__ void k_pack_direct(const int32_t* __restrict__ keys, uint64_t n, int3
     }
     if (wbits == 32) packed[off] = val;
     else atomicOr(&packed[off / per], val << ((off % per) * (unsigned)wbits));
+    if (bits) atomicOr(&bits[off >> 5], 1u << (off & 31u));
   }
 }
 
@@ -347,11 +349,13 @@ void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bit
 
 void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, int payload, const void* attr,
                  int atype, int64_t asize, const int32_t* dict, uint64_t nd, int wbits, uint32_t* packed,
-                 hipStream_t s) {
+                 uint32_t* bits, hipStream_t s) {
   const uint64_t per = 32 / (uint64_t)wbits;
-  fill_u32(packed, ((uint64_t)range + per - 1) / per, 0u, s);
+  const uint64_t words = ((uint64_t)range + per - 1) / per;
+  // the existence bits, when kept, follow the packed words: both cleared at once
+  fill_u32(packed, words + (bits ? ((uint64_t)range + 31) / 32 : 0), 0u, s);
   if (n) k_pack_direct<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, payload, (const char*)attr, atype,
-                                                     asize, dict, nd, wbits, packed);
+                                                     asize, dict, nd, wbits, packed, bits);
   LAUNCH_CHECK("pack_direct");
 }
 

@@ -55,9 +55,10 @@ constexpr int kReplaySpan = 256;
 // the object holds at least 11 rows; needs hc.max_span <= kReplaySpan.  No
 // synchronisation.
 void ht_clear_dropped(const HtClusters& hc, int32_t kmin, uint32_t range, int32_t* first, hipStream_t s);
-// The same over a packed map (an entry is cleared to 0).
+// The same over a packed map (an entry is cleared to 0, and its existence bit when
+// `bits` is given).
 void ht_clear_dropped_packed(const HtClusters& hc, int32_t kmin, uint32_t range, uint32_t* packed, int wbits,
-                             hipStream_t s);
+                             uint32_t* bits, hipStream_t s);
 
 }  // namespace k
 }  // namespace svm

@@ -200,12 +200,14 @@ struct FirstRows {
 struct PackedSlots {
   uint32_t* words;
   unsigned wbits;
+  uint32_t* bits;  // existence bits kept beside the map, or null
   __device__ bool has(uint32_t off) const {
     if (wbits == 32u) return words[off] != 0u;
     const unsigned per = 32u / wbits;
     return ((words[off / per] >> ((off % per) * wbits)) & ((1u << wbits) - 1u)) != 0u;
   }
   __device__ void clear(uint32_t off) const {
+    if (bits) atomicAnd(&bits[off >> 5], ~(1u << (off & 31u)));
     if (wbits == 32u) { words[off] = 0u; return; }
     const unsigned per = 32u / wbits;
     atomicAnd(&words[off / per], ~(((1u << wbits) - 1u) << ((off % per) * wbits)));
@@ -415,12 +417,12 @@ void ht_clear_dropped(const HtClusters& hc, int32_t kmin, uint32_t range, int32_
 }
 
 void ht_clear_dropped_packed(const HtClusters& hc, int32_t kmin, uint32_t range, uint32_t* packed, int wbits,
-                             hipStream_t s) {
+                             uint32_t* bits, hipStream_t s) {
   if (!hc.dropped || !hc.clusters) return;
   if (hc.max_span > (uint32_t)kReplaySpan) fail(SVM_ERR_INVALID, "ht_clear_dropped: cluster wider than the replay window");
   k_clear_dropped<PackedSlots><<<grid_for(hc.clusters), kBlock, 0, s>>>(
       hc.start.as<uint32_t>(), hc.clusters, hc.key.as<int32_t>(), hc.home.as<uint32_t>(), kmin, range,
-      PackedSlots{packed, (unsigned)wbits});
+      PackedSlots{packed, (unsigned)wbits, bits});
   hip_check(hipGetLastError(), "ht_clear_dropped_packed");
 }
 
