@@ -96,7 +96,7 @@ void copy_out(const Table& out, const std::vector<std::pair<int, int64_t>>& sche
     uint64_t off = 0;
     for (int q = 0; q < ncols; ++q) {
       const uint64_t b = rows * (uint64_t)schema[q].second;
-      if (b) SVM_HIP(hipMemcpyAsync(st + off, out.cols[q].dev, b, hipMemcpyDeviceToHost, s));
+      if (b) SVM_HIP(hipMemcpyAsync(st + off, out.col_ptr(q), b, hipMemcpyDeviceToHost, s));
       off += b;
     }
     SVM_HIP(hipStreamSynchronize(s));
@@ -111,9 +111,24 @@ void copy_out(const Table& out, const std::vector<std::pair<int, int64_t>>& sche
   SVM_HIP(hipStreamSynchronize(s));  // the copies below are not ordered after the stream's work
   for (int q = 0; q < ncols; ++q) {
     std::vector<uint8_t> h(rows * (uint64_t)schema[q].second);
-    if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out.cols[q].dev, h.size(), hipMemcpyDeviceToHost));
+    if (!h.empty()) SVM_HIP(hipMemcpy(h.data(), out.col_ptr(q), h.size(), hipMemcpyDeviceToHost));
     cols.push_back(std::move(h));
   }
+}
+
+// The source column an expression names when it is exactly "$(0.c)" and the output
+// column has that column's type and width (the projection then copies it unchanged);
+// -1 otherwise.
+int bare_source_col(const std::string& e, const Schemas& schemas, int type, int64_t size) {
+  int obj = -1, col = -1;
+  const size_t b = e.find_first_not_of(" \t"), t = e.find_last_not_of(" \t");
+  if (b == std::string::npos) return -1;
+  const std::string x = e.substr(b, t - b + 1);
+  if (sscanf(x.c_str(), "$(%d.%d)", &obj, &col) != 2 || obj != 0) return -1;
+  if (x != "$(0." + std::to_string(col) + ")") return -1;
+  if (col < 0 || col >= (int)schemas[0].size()) return -1;
+  if (schemas[0][col].first != type || schemas[0][col].second != size) return -1;
+  return col;
 }
 
 double host_ms(std::chrono::steady_clock::time_point t0) {
@@ -833,6 +848,55 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     mat.push_back((int)q < P);
     if ((int)q < P) row_bytes += (uint64_t)vs;
   }
+  // A Map of bare source columns without Filter or JoinWith keeps every row of its
+  // source unchanged (in every bulk): the output object shares the source's columns
+  // (no copy; it keeps its source alive), holding the bytes the reference's copy holds.
+  if (p.filter_exp.empty() && p.join_rkey.empty()) {
+    std::vector<int> src_col(P, -1);
+    bool bare = P > 0;
+    for (int q = 0; q < P && bare; ++q) {
+      src_col[q] = bare_source_col(p.select_exp[q], c.schemas, oschema[q].first, oschema[q].second);
+      bare = src_col[q] >= 0;
+    }
+    if (bare) {
+      auto out = std::make_shared<Table>();
+      out->id = c.info.id;
+      out->rows = c.src->rows;
+      out->row0 = c.src->row0;
+      out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+      out->bulk0 = c.src->bulk0_rows();
+      out->info.id = c.info.id;
+      out->info.storage = 2;  // DeviceMem (com/svm.proto:49-54)
+      out->base = c.src->base ? c.src->base : c.src;
+      for (size_t q = 0; q < oschema.size(); ++q) {
+        Column col;
+        col.type = oschema[q].first;
+        col.size = oschema[q].second;
+        col.owns = false;
+        if ((int)q < P) {
+          const Column& sc = c.src->cols[src_col[q]];
+          col.dev = sc.dev;
+          col.bytes = sc.bytes;
+          col.origin = c.src;
+          col.origin_col = src_col[q];
+        }
+        out->cols.push_back(col);
+        ColumnInfo ci;
+        ci.type = col.type;
+        ci.val_size = col.size;
+        out->info.column_list.push_back(ci);
+      }
+      res->kind = 0;
+      res->rows = (int64_t)out->rows;
+      for (int q = 0; q < P; ++q) res->types.push_back(oschema[q]);
+      if (opt.want_host) copy_out(*out, oschema, P, out->rows, c.s, res->cols);
+      c.E.store.put(c.info.id, out);
+      if (c.E.cfg.verbose)
+        fprintf(stderr, "[svm] %s: select host %.3f ms; every row of '%s' kept, columns shared; rows %llu\n",
+                c.info.id.c_str(), host_ms(t_host0), c.src->id.c_str(), (unsigned long long)out->rows);
+      return res;
+    }
+  }
   // A source of one bulk is compacted in one launch into an output sized for every
   // row (trimmed to the passing rows once counted) when that capacity is modest;
   // otherwise the tile counts are scanned first and the output is sized exactly.
@@ -852,13 +916,8 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     out->max_tuple_count = c.E.cfg.tuple_per_bulk;
     for (int q = 0; q < P; ++q) {
       a.ptr[SVM_SLOT_OUT + q] = out->cols[q].dev;
-      int obj = -1, col = -1;
-      std::string e = p.select_exp[q];
-      size_t b = e.find_first_not_of(" \t"), t = e.find_last_not_of(" \t");
-      if (b != std::string::npos && sscanf(e.substr(b, t - b + 1).c_str(), "$(%d.%d)", &obj, &col) == 2 &&
-          e.substr(b, t - b + 1) == "$(" + std::to_string(obj) + "." + std::to_string(col) + ")" && obj == 0 &&
-          col < (int)c.schemas[0].size() && c.schemas[0][col].first == oschema[q].first &&
-          c.schemas[0][col].second == oschema[q].second) {
+      const int col = bare_source_col(p.select_exp[q], c.schemas, oschema[q].first, oschema[q].second);
+      if (col >= 0) {
         out->cols[q].origin = c.src;
         out->cols[q].origin_col = col;
       }
