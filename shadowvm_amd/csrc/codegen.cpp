@@ -589,14 +589,14 @@ std::string gen_select_source(const KernelShape& s) {
     o << "    }\n"
       << "    pos += round;\n"
       << "  }\n";
-    if (pass == 2) {
-      // the last tile ends the output: the column padding behind the last row
-      o << "  if (tile + 1 == ntiles && threadIdx.x < 256) {\n";
-      for (size_t q = 0; q < s.proj.size(); ++q)
-        o << "    ((char*)a.ptr[" << SVM_SLOT_OUT + (int)q << "])[pos * " << s.proj_type[q].second
-          << "ull + threadIdx.x] = 0;\n";
-      o << "  }\n";
-    }
+    // the last tile ends the output: the column padding behind the last row (pos is
+    // then the output's row count; an output sized for its rows has it zeroed already)
+    o << "  if (tile + 1 == " << (pass == 2 ? "ntiles" : "(unsigned long long)gridDim.x")
+      << " && threadIdx.x < 256) {\n";
+    for (size_t q = 0; q < s.proj.size(); ++q)
+      o << "    ((char*)a.ptr[" << SVM_SLOT_OUT + (int)q << "])[pos * " << s.proj_type[q].second
+        << "ull + threadIdx.x] = 0;\n";
+    o << "  }\n";
     o << "}\n";
   }
   return o.str();

@@ -897,11 +897,14 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
       return res;
     }
   }
-  // A source of one bulk is compacted in one launch into an output sized for every
-  // row (trimmed to the passing rows once counted) when that capacity is modest;
-  // otherwise the tile counts are scanned first and the output is sized exactly.
+  // A source of one bulk is compacted into an output sized for every row (trimmed
+  // to the passing rows once counted) when that capacity is modest: in one launch up
+  // to kSingleSelectTiles tiles, else count + scan + write without a synchronisation
+  // between them.  Otherwise the tile counts are scanned first and the output is
+  // sized exactly.
   const uint64_t src_bulk0 = c.src->bulk0_rows();
-  const bool single = tiles && tiles <= kSingleSelectTiles && src_bulk0 >= rows && rows * row_bytes <= kSingleSelectBytes;
+  const bool capacity = tiles && src_bulk0 >= rows && rows * row_bytes <= kSingleSelectBytes;
+  const bool single = capacity && tiles <= kSingleSelectTiles;
   SvmArgs a;
   DevBuf blob;
   fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
@@ -941,6 +944,24 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     }
     total = slots.host[0];
     // the passing rows (and bulk 0: the source is one bulk)
+    out->rows = total;
+    out->bulk0 = total;
+  } else if (capacity) {
+    // count, scan and write back to back into an output sized for every source row,
+    // then one synchronisation for the row count (the write's last tile ends the
+    // output with its padding)
+    src = gen_select_source(ks);
+    DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
+    a.out[6] = offs.p;
+    a.out[7] = counts.p;
+    out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
+    bind_outputs();
+    launch(jit_function(src, "svm_sel_count"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
+                          offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
+    launch(jit_function(src, "svm_sel_write"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    SVM_HIP(hipStreamSynchronize(c.s));
+    total = slots.host[0];
     out->rows = total;
     out->bulk0 = total;
   } else {
