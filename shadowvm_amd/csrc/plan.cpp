@@ -140,8 +140,33 @@ Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduc
   Lowered out;
   std::string& o = out.code;
   size_t n = exp.size(), k = 0;
+  // Open parentheses: whether each belongs to a StrCmp call, and the commas seen at its
+  // level.  StrCmp's length (third argument) stays a literal in the generated code, so
+  // the comparison unrolls over a width known when the kernel is compiled.
+  std::vector<std::pair<bool, int>> parens;
+  bool after_strcmp = false;
   while (k < n) {
     char ch = exp[k];
+    if (ch == '(' && !(k > 0 && exp[k - 1] == '$')) {
+      parens.push_back({after_strcmp, 0});
+      after_strcmp = false;
+      o.push_back(ch);
+      k++;
+      continue;
+    }
+    if (ch == ')' && !parens.empty()) {
+      parens.pop_back();
+      o.push_back(ch);
+      k++;
+      continue;
+    }
+    if (ch == ',' && !parens.empty()) {
+      parens.back().second++;
+      o.push_back(ch);
+      k++;
+      continue;
+    }
+    if (!isspace((unsigned char)ch) && !isalpha((unsigned char)ch) && ch != '_') after_strcmp = false;
     if (ch == '$' && k + 1 < n && exp[k + 1] == '(') {
       size_t p2 = exp.find('.', k);
       size_t p3 = p2 == std::string::npos ? p2 : exp.find(')', p2);
@@ -208,7 +233,8 @@ Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduc
       std::string tok = exp.substr(k, e - k);
       bool plain = true;
       for (char c : tok) plain = plain && isdigit((unsigned char)c);
-      if (plain && (tok.size() == 1 || tok[0] != '0') && tok.size() <= 10 && lits) {
+      const bool strcmp_len = !parens.empty() && parens.back().first && parens.back().second == 2;
+      if (plain && (tok.size() == 1 || tok[0] != '0') && tok.size() <= 10 && lits && !strcmp_len) {
         long long v = atoll(tok.c_str());
         if (v <= INT_MAX) {
           o += "PI(" + std::to_string(lits->ints.size()) + ")";
@@ -225,6 +251,7 @@ Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduc
       size_t e = k;
       while (e < n && (isalnum((unsigned char)exp[e]) || exp[e] == '_')) e++;
       o += exp.substr(k, e - k);
+      after_strcmp = exp.compare(k, e - k, "StrCmp") == 0 && e - k == 6;
       k = e;
       continue;
     }

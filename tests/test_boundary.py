@@ -86,7 +86,9 @@ def test_decode_reference_quirks():
     assert d["reduce_key"] == [[1, 1], [2, 1]]
     assert d["metric"] == ["($(0.2)-$(0.0))/100"]
     f0, f1 = d["filter"]
-    assert f0[1] == "StrCmp(((const char*)(FCOL1+(long long)i*10)),PS(0),PI(0))==PI(1)"
+    # StrCmp's length stays a literal (the kernel unrolls over it); other literals
+    # become launch parameters
+    assert f0[1] == "StrCmp(((const char*)(FCOL1+(long long)i*10)),PS(0),8)==PI(0)"
     # octal and out-of-int32 literals keep their C meaning verbatim
     assert f1[1] == "(f2) < 010 || (f0)>3000000000"
     assert d["string_literals"].startswith("MFGR#12")
