@@ -775,6 +775,14 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     if (!sl.prof->ok) fail(SVM_ERR_UNSUPPORTED, "dimension profile unavailable");
     nprof += sl.prof->P + 1;
   }
+  // Slots are looked up in order, each gated by the ones before it (a row no query
+  // still wants reads the shared no-row entry): the smallest key-to-profile arrays
+  // go first, so the rows that reach a large one (which does not stay in L2 beside
+  // the column stream) are those the small ones kept.  The masks are ANDed, so the
+  // order does not change any result.
+  std::stable_sort(P.slots.begin(), P.slots.end(), [](const SlotB& a, const SlotB& b) {
+    return (uint64_t)a.prof->range * (a.prof->narrow ? 1u : 2u) < (uint64_t)b.prof->range * (b.prof->narrow ? 1u : 2u);
+  });
   if (nprof > (unsigned)bk::kMaxProfiles) fail(SVM_ERR_INVALID, "pass exceeds the LDS profile budget");
   auto slot_index = [&](const JoinL& j) {
     for (size_t k = 0; k < P.slots.size(); ++k)
