@@ -43,6 +43,23 @@ def test_published_drop_counts(engine, n, slots, want):
         P.table_drop("rh_keys")
 
 
+def test_golden_drop_counts(engine):
+    """Dropped inserts computed on the GPU equal those of the reference's own runtime
+    (golden vectors from oracle/_ref): sparse and crowded tables, duplicate keys
+    (every row takes a slot) and negative keys."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "runtime_vectors.json")) as f:
+        tables = json.load(f)["int_tables"]
+    for q, t in enumerate(tables):
+        print(f"[refhash] golden table {q}: {len(t['keys'])} keys, {t['slots']} slots", flush=True)
+        P.table_upload("rh_golden", [I], [4], [np.array(t["keys"], np.int32)])
+        try:
+            assert P.hash_drops("rh_golden", 0, t["slots"]) == t["dropped"], q
+        finally:
+            P.table_drop("rh_golden")
+
+
 def test_tiny_bulk0_and_drops(engine):
     """The oracle's brute-force cases through the product, at a bulk capacity of 16."""
     with engine_with(engine, 16):
