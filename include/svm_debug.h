@@ -39,6 +39,16 @@ int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, i
  * the GPU, as the joins do. */
 int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* dropped);
 
+/* The autotuners' record as JSON {"scan": [...], "batch": [...]}: for every tuned
+ * shape the chosen launch configuration and each trial's time in ms.  Trials are
+ * cross-checked on the device against the first trial's group tables; a negative time
+ * marks a trial that differed (never chosen; "rejected" counts them).  reset != 0
+ * clears both caches afterwards.  Environment controls read by the tuners:
+ * SVM_SCAN_VARIANT=<i> / SVM_BATCH_VARIANT=<v>,<tile>,<bits> force one candidate,
+ * SVM_TUNE_MIN_ROWS / SVM_BATCH_TUNE_MIN_ROWS lower the size from which they tune,
+ * SVM_AUTOTUNE=0 turns tuning off. */
+int svm_debug_tune_report(char* buf, size_t cap, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ The reference's *generated CPU path* for one pipeline:
 * the blocks ``GenFilterBlock`` / ``GenJoinBlock`` / ``GenSelectBlock`` /
   ``GenReduceBlock`` (codegen/codegen.cpp:100-229) of the CPU variant, inside
   the ``HostQueryTransform`` row loop (codegen/template.cpp:61-73);
-* compiled with ``g++ -O2`` as the reference JIT compiles with ``clang -O2``
+* compiled with ``g++ -O2 -fwrapv`` as the reference JIT compiles with ``clang -O2``
   (codegen/compiler.cpp:30), then run one task per bulk with per-bulk sort +
   reduce and the documented merge (oracle/svm_oracle_rt.h).
 
@@ -376,18 +376,27 @@ def compile_source(src: str):
     with _lock:
         if key in _cache:
             return _cache[key]
-        d = jit_dir()
-        so = os.path.join(d, f"q_{key}.so")
-        if not os.path.exists(so):
-            cpp = os.path.join(d, f"q_{key}.cpp")
-            with open(cpp, "w") as f:
-                f.write(src)
-            tmp = so + f".tmp{os.getpid()}"
-            r = subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread",
-                                "-I", HERE, cpp, "-o", tmp], capture_output=True, text=True)
-            if r.returncode != 0:
-                raise OracleError("oracle compile failed:\n" + r.stderr[-4000:])
-            os.replace(tmp, so)
+    # compiled outside the lock, so that threads running different plans compile
+    # concurrently (two threads with the same plan both compile; the rename is atomic)
+    d = jit_dir()
+    so = os.path.join(d, f"q_{key}.so")
+    if not os.path.exists(so):
+        tag = f"{os.getpid()}_{threading.get_ident()}"
+        cpp = os.path.join(d, f"q_{key}_{tag}.cpp")
+        with open(cpp, "w") as f:
+            f.write(src)
+        tmp = so + f".tmp{tag}"
+        # -fwrapv: signed int32 overflow in a plan's expressions wraps, as in the
+        # generated GPU kernels (the reference's clang / nvcc builds wrap in practice)
+        r = subprocess.run(["g++", "-O2", "-fwrapv", "-std=c++17", "-fPIC", "-shared", "-pthread",
+                            "-I", HERE, cpp, "-o", tmp], capture_output=True, text=True)
+        os.unlink(cpp)
+        if r.returncode != 0:
+            raise OracleError("oracle compile failed:\n" + r.stderr[-4000:])
+        os.replace(tmp, so)
+    with _lock:
+        if key in _cache:
+            return _cache[key]
         lib = ctypes.CDLL(so)
         lib.oracle_open.argtypes = [ctypes.POINTER(_Ctx)]
         lib.oracle_run.argtypes = [ctypes.POINTER(_Ctx), ctypes.POINTER(_Res)]

@@ -354,6 +354,7 @@ int bits_for(uint32_t n) {
 // the profile of each key offset.
 struct Profile {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;  // owner's data_version() when built
   std::vector<int> attrs, shift, bits;
   std::vector<std::shared_ptr<DictEntry>> dicts;
   uint32_t P = 0;
@@ -374,10 +375,12 @@ std::shared_ptr<Profile> profile_of(const TablePtr& t, int kcol, const std::vect
   {
     std::lock_guard<std::mutex> g(g_prof_mu);
     auto it = g_profiles.find(key);
-    if (it != g_profiles.end() && it->second->owner.lock() == t) return it->second;
+    if (it != g_profiles.end() && it->second->owner.lock() == t && it->second->version == t->data_version())
+      return it->second;
   }
   auto p = std::make_shared<Profile>();
   p->owner = t;
+  p->version = t->data_version();
   p->attrs = attrs;
   KeyEntry ke = table_key(t, kcol, s);
   const uint64_t n = t->rows;
@@ -728,11 +731,45 @@ struct BatchTune {
   int variant = 0;
   long long tile = 16384;
   int bits = 0;
+  // trials of the tuning run: (variant, tile, bits, ms); ms < 0 marks a trial whose
+  // group tables differed from the first trial's (never chosen)
+  std::vector<std::array<double, 4>> trials;
 };
 std::mutex g_btune_mu;
 std::map<std::tuple<int, int, int, bool>, BatchTune> g_btune;
 constexpr long long kTileChoices[] = {16384, 65536};
 constexpr uint64_t kBatchTuneMinRows = 1ull << 22;
+
+// SVM_BATCH_TUNE_MIN_ROWS=<n>: tune batch shapes from n scanned rows on.
+uint64_t batch_tune_min_rows() {
+  const char* e = getenv("SVM_BATCH_TUNE_MIN_ROWS");
+  if (e && *e) {
+    char* end = nullptr;
+    const unsigned long long v = strtoull(e, &end, 10);
+    if (end && *end == '\0') return v;
+  }
+  return kBatchTuneMinRows;
+}
+
+// SVM_BATCH_VARIANT=<variant>,<tile rows>,<key bits>: every pass runs this shared-scan
+// variant (0..scan_variants()-1), tile (16384 or 65536) and key-bit pre-test (0 or 1),
+// with no trials.  A variant with 32-bit query masks is used only by passes of at
+// most 32 queries (larger passes keep the default).  Returns false when unset.
+bool forced_batch_variant(BatchTune& t) {
+  const char* e = getenv("SVM_BATCH_VARIANT");
+  if (!e || !*e) return false;
+  int v = -1, bits = -1;
+  long long tile = 0;
+  char extra = 0;
+  if (sscanf(e, "%d,%lld,%d%c", &v, &tile, &bits, &extra) != 3 || v < 0 || v >= bk::scan_variants() ||
+      (tile != kTileChoices[0] && tile != kTileChoices[1]) || (bits != 0 && bits != 1))
+    fail(SVM_ERR_INVALID, "SVM_BATCH_VARIANT must be <variant 0.." + std::to_string(bk::scan_variants() - 1) +
+                              ">,<16384|65536>,<0|1>");
+  t.variant = v;
+  t.tile = tile;
+  t.bits = bits;
+  return true;
+}
 
 int ilog2(uint64_t v) {
   int r = 0;
@@ -760,6 +797,32 @@ bool env_off(const char* name) {
 }
 
 }  // namespace
+
+std::string batch_tune_report(bool reset) {
+  std::lock_guard<std::mutex> g(g_btune_mu);
+  std::string j = "[";
+  bool first = true;
+  for (auto& kv : g_btune) {
+    const BatchTune& t = kv.second;
+    int rejected = 0;
+    std::string tr = "[";
+    for (size_t q = 0; q < t.trials.size(); ++q) {
+      rejected += t.trials[q][3] < 0 ? 1 : 0;
+      char b[96];
+      snprintf(b, sizeof(b), "%s[%d,%lld,%d,%.4f]", q ? "," : "", (int)t.trials[q][0], (long long)t.trials[q][1],
+               (int)t.trials[q][2], t.trials[q][3]);
+      tr += b;
+    }
+    char h[160];
+    snprintf(h, sizeof(h), "{\"slots\":%d,\"log2_queries\":%d,\"log2_rows\":%d,\"mask32\":%s,\"variant\":%d,"
+                           "\"tile\":%lld,\"bits\":%d,", std::get<0>(kv.first), std::get<1>(kv.first),
+             std::get<2>(kv.first), std::get<3>(kv.first) ? "true" : "false", t.variant, t.tile, t.bits);
+    j += std::string(first ? "" : ",") + h + "\"rejected\":" + std::to_string(rejected) + ",\"trials\":" + tr + "]}";
+    first = false;
+  }
+  if (reset) g_btune.clear();
+  return j + "]";
+}
 
 static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
                      std::vector<std::unique_ptr<svm_result>*>& outs, Engine& E, hipStream_t s, Scratch& sc,
@@ -1034,10 +1097,22 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     }
   }
   if (!tuned) ch.tile = union_rows > (1ull << 26) ? 65536 : 16384;
-  if (!tuned && tune && union_rows >= kBatchTuneMinRows && !env_off("SVM_AUTOTUNE")) {
+  BatchTune forced;
+  if (forced_batch_variant(forced) && (!bk::scan_variant_mask32(forced.variant) || mask32)) {
+    ch = forced;
+    tuned = true;
+  }
+  if (!tuned && tune && union_rows >= batch_tune_min_rows() && !env_off("SVM_AUTOTUNE")) {
     hipEvent_t t0, t1;
     SVM_HIP(hipEventCreate(&t0));
     SVM_HIP(hipEventCreate(&t1));
+    // Integer results do not depend on the launch shape: the first trial's group
+    // tables are kept and every later trial's are compared with them on the device;
+    // a trial that differs is never chosen.
+    uint64_t* ref_sums = (uint64_t*)sc.get(12, sum_words * 8 + 8);
+    unsigned* ref_flags = (unsigned*)sc.get(13, flag_words * 4 + 4);
+    uint32_t* diff = (uint32_t*)sc.get(14, 16);
+    std::vector<std::array<double, 4>> trials;
     auto trial = [&](const BatchTune& c) {
       set_tile(c.tile);
       A->use_bits = c.bits;
@@ -1051,12 +1126,22 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       SVM_HIP(hipStreamSynchronize(s));
       float ms = 0;
       SVM_HIP(hipEventElapsedTime(&ms, t0, t1));
+      if (trials.empty()) {
+        SVM_HIP(hipMemcpyAsync(ref_sums, sums, sum_words * 8, hipMemcpyDeviceToDevice, s));
+        SVM_HIP(hipMemcpyAsync(ref_flags, flags, flag_words * 4, hipMemcpyDeviceToDevice, s));
+      } else if (k::words_differ(ref_sums, sums, sum_words * 2, diff, s) ||
+                 k::words_differ(ref_flags, flags, flag_words, diff, s)) {
+        fprintf(stderr, "[svm] batch tune: variant %d, tile %lld, key bits %d gave different group tables; not "
+                        "chosen\n", c.variant, c.tile, c.bits);
+        ms = -1.0f;
+      }
+      trials.push_back({(double)c.variant, (double)c.tile, (double)c.bits, (double)ms});
       if (E.cfg.verbose)
         fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots, %s lookups%s, key bits %d, tile %lld rows: %.3f ms\n",
                 bk::scan_variant_block(c.variant), bk::scan_variant_hash_slots(c.variant),
                 bk::scan_variant_parallel(c.variant) ? "parallel" : "gated",
                 bk::scan_variant_prefetch(c.variant) ? ", prefetch" : "", c.bits, c.tile, ms);
-      return ms;
+      return ms < 0 ? 1e30f : ms;
     };
     float best = 1e30f;
     ch.variant = -1;
@@ -1070,11 +1155,13 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
           ch = c;
         }
       }
+    if (ch.variant < 0) fail(SVM_ERR_DEVICE, "batch autotuner: no trial reproduced the first trial's group tables");
     BatchTune c = ch;
     c.bits = 1;
     if (trial(c) < best) ch = c;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    ch.trials = std::move(trials);
     std::lock_guard<std::mutex> g(g_btune_mu);
     g_btune[tkey] = ch;
   }
@@ -1340,17 +1427,62 @@ struct Server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Ticket>> pending;
-  std::map<uint64_t, std::shared_ptr<Ticket>> tickets;
-  uint64_t next_id = 1, bytes_in_use = 0, budget = 0, running = 0, completed = 0;
+  std::map<uint64_t, std::shared_ptr<Ticket>> tickets;  // submitted and not yet claimed by a wait
+  uint64_t bytes_in_use = 0, budget = 0, running = 0, completed = 0;
   int max_batch = 64, mode = 0;
+  int submitting = 0;  // submissions between their stop check and their enqueue
   bool stop = false;
   std::vector<std::thread> workers;
   hipStream_t submit_stream = nullptr;
 };
 
-std::unique_ptr<Server> g_server;
+// The running server, and stopped servers that still hold unclaimed tickets (a
+// wait on one of those returns the ticket's final status).  Every call copies the
+// pointer it uses under g_server_mu, so a server stays alive while any call on it
+// (a blocked wait, a submission) is in progress, whatever svm_server_stop does.
+std::shared_ptr<Server> g_server;
+std::vector<std::shared_ptr<Server>> g_stopped;
 std::mutex g_server_mu;
+// Ticket numbers are unique in the process, across servers.
+std::atomic<uint64_t> g_next_ticket{1};
 constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a query on the plain path
+
+std::shared_ptr<Server> running_server() {
+  std::lock_guard<std::mutex> g(g_server_mu);
+  return g_server;
+}
+
+// The server (running or stopped) that holds `ticket`, or null.
+std::shared_ptr<Server> server_of(uint64_t ticket) {
+  std::lock_guard<std::mutex> g(g_server_mu);
+  auto holds = [&](const std::shared_ptr<Server>& S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    return S->tickets.count(ticket) > 0;
+  };
+  if (g_server && holds(g_server)) return g_server;
+  for (auto& S : g_stopped)
+    if (holds(S)) return S;
+  return nullptr;
+}
+
+// A served query's pipelines write their outputs into the shared store under their
+// plan ids.  Two queries in flight may carry the same ids (the reference's drivers
+// use fixed ones, e.g. test/test-SSB-Q21.cu:21), so every id a ticket's pipelines
+// produce is renamed to one private to the ticket, in the pipelines' own ids and in
+// the pre_pipeline_id lists that name them; objects the query reads but does not
+// produce (its fact table and dimension tables) keep their ids.
+void namespace_ticket(std::vector<Job>& jobs, uint64_t tag) {
+  std::map<std::string, std::string> ren;
+  const std::string suffix = std::string("\x01ticket") + std::to_string(tag);
+  for (auto& j : jobs) ren[j.info.id] = j.info.id + suffix;
+  for (auto& j : jobs) {
+    j.info.id = ren[j.info.id];
+    for (auto& pre : j.info.pre_pipeline_id) {
+      auto it = ren.find(pre);
+      if (it != ren.end()) pre = it->second;
+    }
+  }
+}
 
 // A served query is self-contained (its dimension pipelines, then its fact
 // pipeline): the objects it materialises are its intermediates and are dropped
@@ -1358,11 +1490,16 @@ constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a query on the 
 void run_plain_query(Ticket& t, hipStream_t s) {
   std::unique_ptr<svm_result> last;
   Engine& E = engine();
-  for (auto& j : t.jobs) {
-    RunOptions opt;
-    opt.stream = s;
-    opt.want_host = true;
-    last = run_pipeline(j.info, opt);
+  try {
+    for (auto& j : t.jobs) {
+      RunOptions opt;
+      opt.stream = s;
+      opt.want_host = true;
+      last = run_pipeline(j.info, opt);
+    }
+  } catch (...) {
+    for (auto& j : t.jobs) E.store.drop(j.info.id);
+    throw;
   }
   t.result = std::move(last);
   for (auto& j : t.jobs) E.store.drop(j.info.id);
@@ -1383,7 +1520,6 @@ void worker_main(Server* S, int index) {
         if (S->stop) return true;
         return !S->pending.empty() && S->bytes_in_use + S->pending.front()->need <= S->budget;
       });
-      if (S->stop && S->pending.empty()) break;
       if (S->stop) break;
       // admit in arrival order while memory and the batch size allow
       const int cap = S->mode == 1 ? 1 : S->max_batch;
@@ -1450,7 +1586,7 @@ int server_start(int workers, uint64_t budget, int max_batch, int mode) {
   if (g_server) fail(SVM_ERR_STATE, "server already running");
   if (workers <= 0 || workers > 16) fail(SVM_ERR_INVALID, "workers must be 1..16");
   if (mode != 0 && mode != 1) fail(SVM_ERR_INVALID, "mode is 0 (continuous batching) or 1 (plain)");
-  auto S = std::make_unique<Server>();
+  auto S = std::make_shared<Server>();
   S->budget = budget ? budget : E.cfg.hbm_budget;
   S->max_batch = max_batch > 0 ? max_batch : 64;
   S->mode = mode;
@@ -1462,16 +1598,31 @@ int server_start(int workers, uint64_t budget, int max_batch, int mode) {
 }
 
 int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64_t* ticket) {
-  Server* S = g_server.get();
+  std::shared_ptr<Server> S = running_server();
   if (!S) fail(SVM_ERR_STATE, "server not running");
   if (n <= 0 || !plans || !lens) fail(SVM_ERR_INVALID, "a query needs at least its fact pipeline");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) fail(SVM_ERR_STATE, "server stopping");
+    S->submitting++;  // svm_server_stop waits for this submission (it uses submit_stream)
+  }
+  struct Leave {
+    Server* S;
+    ~Leave() {
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->submitting--;
+      }
+      S->cv_done.notify_all();
+    }
+  } leave{S.get()};
   Engine& E = engine();
   auto t = std::make_shared<Ticket>();
+  t->id = g_next_ticket++;
   t->jobs.resize(n);
-  for (int i = 0; i < n; ++i) {
-    t->jobs[i].info = decode_pipeline_info(plans[i], lens[i]);
-    t->jobs[i].plan = decode_plan(t->jobs[i].info);
-  }
+  for (int i = 0; i < n; ++i) t->jobs[i].info = decode_pipeline_info(plans[i], lens[i]);
+  namespace_ticket(t->jobs, t->id);
+  for (int i = 0; i < n; ++i) t->jobs[i].plan = decode_plan(t->jobs[i].info);
   if (S->mode == 0) {
     std::map<std::string, DimLowered> dims;
     for (int i = 0; i + 1 < n; ++i) {
@@ -1488,7 +1639,6 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) fail(SVM_ERR_STATE, "server stopping");
     t->submitted = std::chrono::steady_clock::now();
-    t->id = S->next_id++;
     S->tickets[t->id] = t;
     S->pending.push_back(t);
     if (ticket) *ticket = t->id;
@@ -1498,8 +1648,11 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
 }
 
 int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms, double* total_ms) {
-  Server* S = g_server.get();
-  if (!S) fail(SVM_ERR_STATE, "server not running");
+  std::shared_ptr<Server> S = server_of(ticket);
+  if (!S) {
+    if (!running_server() && ticket < g_next_ticket.load()) fail(SVM_ERR_STATE, "server not running");
+    fail(SVM_ERR_NOT_FOUND, "unknown ticket " + std::to_string(ticket));
+  }
   std::unique_lock<std::mutex> lk(S->mu);
   auto it = S->tickets.find(ticket);
   if (it == S->tickets.end()) fail(SVM_ERR_NOT_FOUND, "unknown ticket " + std::to_string(ticket));
@@ -1508,11 +1661,18 @@ int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queue
   if (timeout_ms < 0) S->cv_done.wait(lk, ready);
   else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
     return SVM_ERR_BUSY;
-  S->tickets.erase(ticket);
+  if (!S->tickets.erase(ticket)) fail(SVM_ERR_NOT_FOUND, "ticket " + std::to_string(ticket) + " already claimed");
+  const bool drained = S->stop && S->tickets.empty();
+  lk.unlock();
+  if (drained) {
+    // a stopped server whose last ticket is claimed is forgotten
+    std::lock_guard<std::mutex> g(g_server_mu);
+    g_stopped.erase(std::remove(g_stopped.begin(), g_stopped.end(), S), g_stopped.end());
+  }
   auto ms = [&](std::chrono::steady_clock::time_point a) {
     return std::chrono::duration<double, std::milli>(a - t->submitted).count();
   };
-  if (queued_ms) *queued_ms = ms(t->admitted);
+  if (queued_ms) *queued_ms = t->admitted.time_since_epoch().count() ? ms(t->admitted) : 0.0;
   if (total_ms) *total_ms = ms(t->finished);
   if (t->status != SVM_OK) fail(t->status, t->error);
   if (out) *out = t->result.release();
@@ -1520,7 +1680,7 @@ int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queue
 }
 
 int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes) {
-  Server* S = g_server.get();
+  std::shared_ptr<Server> S = running_server();
   if (!S) fail(SVM_ERR_STATE, "server not running");
   std::lock_guard<std::mutex> lk(S->mu);
   if (queued) *queued = S->pending.size();
@@ -1531,27 +1691,43 @@ int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* 
 }
 
 int server_stop() {
-  std::lock_guard<std::mutex> g(g_server_mu);
-  if (!g_server) return SVM_OK;
+  std::shared_ptr<Server> S;
   {
-    std::lock_guard<std::mutex> lk(g_server->mu);
-    g_server->stop = true;
+    std::lock_guard<std::mutex> g(g_server_mu);
+    S = std::move(g_server);
+    g_server.reset();
   }
-  g_server->cv_work.notify_all();
-  for (auto& w : g_server->workers) w.join();
+  if (!S) return SVM_OK;
   {
-    // queries never admitted fail with SVM_ERR_STATE
-    std::lock_guard<std::mutex> lk(g_server->mu);
-    for (auto& t : g_server->pending) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  // admitted batches finish; workers take nothing new
+  for (auto& w : S->workers) w.join();
+  S->workers.clear();
+  {
+    // queries never admitted fail with SVM_ERR_STATE; submissions in progress end
+    // before the submission stream goes away
+    std::unique_lock<std::mutex> lk(S->mu);
+    const auto now = std::chrono::steady_clock::now();
+    for (auto& t : S->pending) {
       t->status = SVM_ERR_STATE;
       t->error = "server stopped before the query was admitted";
+      t->finished = now;
       t->done = true;
     }
-    g_server->pending.clear();
+    S->pending.clear();
+    S->cv_done.notify_all();
+    S->cv_done.wait(lk, [&] { return S->submitting == 0; });
   }
-  g_server->cv_done.notify_all();
-  (void)hipStreamDestroy(g_server->submit_stream);
-  g_server.reset();
+  (void)hipStreamDestroy(S->submit_stream);
+  S->submit_stream = nullptr;
+  std::lock_guard<std::mutex> g(g_server_mu);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->tickets.empty()) g_stopped.push_back(S);
+  }
   return SVM_OK;
 }
 

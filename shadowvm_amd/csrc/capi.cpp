@@ -13,6 +13,8 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
 int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queued_ms, double* total_ms);
 int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes);
 int server_stop();
+std::string scan_tune_report(bool reset);
+std::string batch_tune_report(bool reset);
 }  // namespace svm
 
 using namespace svm;
@@ -97,7 +99,13 @@ int svm_table_put_column(const char* id, int col, const void* host_src, uint64_t
     if (bytes != need) fail(SVM_ERR_INVALID, "column needs " + std::to_string(need) + " bytes, got " + std::to_string(bytes));
     if (!c.dev) fail(SVM_ERR_INVALID, "column is not materialised");
     if (!c.owns) fail(SVM_ERR_INVALID, "'" + std::string(id) + "' is a view and read-only");
-    if (bytes) SVM_HIP(hipMemcpy(c.dev, host_src, bytes, hipMemcpyHostToDevice));
+    if (bytes) {
+      // work already queued on any stream (which may read the old bytes) finishes first
+      SVM_HIP(hipDeviceSynchronize());
+      SVM_HIP(hipMemcpy(c.dev, host_src, bytes, hipMemcpyHostToDevice));
+    }
+    // facts cached about the old contents are recomputed on next use
+    t->version++;
   });
 }
 
@@ -349,6 +357,19 @@ int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* drop
       fail(SVM_ERR_INVALID, "not a materialised Int32 column");
     if (!slots) slots = (uint64_t)(uint32_t)(kHashRate * (t->max_tuple_count ? t->max_tuple_count : t->rows));
     *dropped = k::ht_kept((const int32_t*)t->col_ptr(col), t->rows, slots, nullptr, E.stream);
+  });
+}
+
+// Debug: the autotuners' record as JSON, {"scan": [...], "batch": [...]}: per tuned
+// shape the chosen candidate and every trial's time (negative: the trial's group
+// tables differed from the first trial's and it was not chosen).  reset != 0 clears
+// both caches afterwards, so the next run of each shape tunes again.
+int svm_debug_tune_report(char* buf, size_t cap, int reset) {
+  return guard([&] {
+    if (!buf || cap == 0) fail(SVM_ERR_INVALID, "null output buffer");
+    const std::string j = "{\"scan\":" + scan_tune_report(reset != 0) + ",\"batch\":" + batch_tune_report(reset != 0) + "}";
+    if (j.size() + 1 > cap) fail(SVM_ERR_INVALID, "output buffer too small (" + std::to_string(j.size() + 1) + " bytes needed)");
+    memcpy(buf, j.c_str(), j.size() + 1);
   });
 }
 

@@ -72,6 +72,9 @@ struct Event {
     }
     SVM_HIP(hipEventCreate(&e));
   }
+  void record(hipStream_t s) const {
+    if (e) SVM_HIP(hipEventRecord(e, s));
+  }
   ~Event() {
     if (!e) return;
     std::lock_guard<std::mutex> g(g_event_mu);
@@ -137,7 +140,7 @@ double host_ms(std::chrono::steady_clock::time_point t0) {
 
 float elapsed(const Event& a, const Event& b) {
   float ms = 0;
-  SVM_HIP(hipEventElapsedTime(&ms, a.e, b.e));
+  if (a.e && b.e) SVM_HIP(hipEventElapsedTime(&ms, a.e, b.e));
   return ms;
 }
 
@@ -497,7 +500,11 @@ struct TuneChoice {
 struct TuneState {
   bool done = false;
   TuneChoice best;
+  // time of each candidate; a negative time marks a trial whose group table
+  // differed from the first candidate's (never chosen)
   std::vector<std::pair<TuneChoice, float>> trials;
+  uint64_t rows = 0;
+  std::string pipeline;
 };
 std::mutex g_tune_mu;
 std::map<uint64_t, TuneState> g_tune;
@@ -515,6 +522,31 @@ const TuneChoice kTuneCandidates[] = {
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
+constexpr int kNumTuneCandidates = (int)(sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]));
+
+// SVM_TUNE_MIN_ROWS=<n>: tune shapes from n source rows on (so that the trials and
+// their cross-check run on test-sized tables).
+uint64_t tune_min_rows() {
+  const char* e = getenv("SVM_TUNE_MIN_ROWS");
+  if (e && *e) {
+    char* end = nullptr;
+    const unsigned long long v = strtoull(e, &end, 10);
+    if (end && *end == '\0') return v;
+  }
+  return kTuneMinRows;
+}
+
+// SVM_SCAN_VARIANT=<i>: run candidate i of kTuneCandidates, whatever the size, with
+// no trials; -1 when unset.
+int forced_scan_variant() {
+  const char* e = getenv("SVM_SCAN_VARIANT");
+  if (!e || !*e) return -1;
+  char* end = nullptr;
+  const long v = strtol(e, &end, 10);
+  if (!end || *end != '\0' || v < 0 || v >= kNumTuneCandidates)
+    fail(SVM_ERR_INVALID, std::string("SVM_SCAN_VARIANT must be 0..") + std::to_string(kNumTuneCandidates - 1));
+  return (int)v;
+}
 
 
 bool autotune_enabled() {
@@ -534,7 +566,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   if ((int)outcols.size() < K + M) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than keys + metrics");
   const auto t_host0 = std::chrono::steady_clock::now();
   Event ev0, ev1, ev2, ev3;
-  SVM_HIP(hipEventRecord(ev0.e, c.s));
+  ev0.record(c.s);
 
   KernelShape ks;
   ks.block = opt.block;
@@ -664,14 +696,25 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
     words = t.stage ? lds_words : agg_words;
   };
-  if (run_scan && autotune_enabled() && rows >= kTuneMinRows && opt.block == 256 && opt.unroll == 1) {
+  const int forced = forced_scan_variant();
+  if (forced >= 0) {
+    choice = kTuneCandidates[forced];
+  } else if (run_scan && autotune_enabled() && rows >= tune_min_rows() && opt.block == 256 && opt.unroll == 1) {
     const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
     std::unique_lock<std::mutex> lk(g_tune_mu);
     TuneState& st = g_tune[tkey];
     if (!st.done) {
-      // trial runs of every candidate (the results are discarded)
+      // Trial runs of every candidate.  Integer results do not depend on the launch
+      // shape, so the first candidate's group table is kept and every later trial's
+      // is compared with it on the device: a candidate that differs is never chosen.
+      const uint64_t sum_words = K ? G * M * 2 : 0, flag_words = K ? nflag : 0;
+      DevBuf ref_sum(sum_words * 4 + 8, c.s), ref_flag(flag_words * 4 + 4, c.s), ref_ssum(64, c.s),
+          diff(16, c.s);
+      st.rows = rows;
+      st.pipeline = c.info.id;
       float best = 1e30f;
-      for (const TuneChoice& cand : kTuneCandidates) {
+      for (int ci = 0; ci < kNumTuneCandidates; ++ci) {
+        const TuneChoice& cand = kTuneCandidates[ci];
         KernelShape k2;
         uint64_t w2 = 0;
         shape_for(cand, k2, w2);
@@ -684,8 +727,24 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         SVM_HIP(hipEventRecord(tb.e, c.s));
         SVM_HIP(hipEventSynchronize(tb.e));
         float ms = elapsed(ta, tb);
+        bool same = true;
+        if (ci == 0) {
+          SVM_HIP(hipMemcpyAsync(ref_ssum.p, ssum.p, 64, hipMemcpyDeviceToDevice, c.s));
+          if (sum_words) SVM_HIP(hipMemcpyAsync(ref_sum.p, gsum.p, sum_words * 4, hipMemcpyDeviceToDevice, c.s));
+          if (flag_words) SVM_HIP(hipMemcpyAsync(ref_flag.p, gflag.p, flag_words * 4, hipMemcpyDeviceToDevice, c.s));
+        } else {
+          same = !k::words_differ(ref_ssum.p, ssum.p, 16, diff.as<uint32_t>(), c.s) &&
+                 !k::words_differ(ref_sum.p, gsum.p, sum_words, diff.as<uint32_t>(), c.s) &&
+                 !k::words_differ(ref_flag.p, gflag.p, flag_words, diff.as<uint32_t>(), c.s);
+        }
+        if (!same) {
+          fprintf(stderr, "[svm] autotune %s: candidate %d (%d,%d,%d,%d,%d,%d) gave a different group table; "
+                          "not chosen\n", c.info.id.c_str(), ci, cand.block, cand.unroll, cand.bpc, (int)cand.stage,
+                  (int)cand.prefetch, (int)cand.flat);
+          ms = -1.0f;
+        }
         st.trials.push_back({cand, ms});
-        if (ms < best) { best = ms; st.best = cand; }
+        if (same && ms < best) { best = ms; st.best = cand; }
       }
       st.done = true;
       clear_outputs();
@@ -712,12 +771,12 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   hipFunction_t fn = nullptr;
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
   const double t_launch_host = host_ms(t_host0);
-  SVM_HIP(hipEventRecord(ev1.e, c.s));
+  ev1.record(c.s);
   if (run_scan) {
     unsigned grid = scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc);
     launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
   }
-  SVM_HIP(hipEventRecord(ev2.e, c.s));
+  ev2.record(c.s);
 
   // finish: ordered group emission into the output table
   std::vector<std::pair<int, int64_t>> oschema;
@@ -756,7 +815,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     }
   }
   out->max_tuple_count = c.E.cfg.tuple_per_bulk;
-  SVM_HIP(hipEventRecord(ev3.e, c.s));
+  ev3.record(c.s);
   res->kind = K == 0 ? 2 : 1;
   res->rows = (int64_t)out_rows;
   res->types = oschema;
@@ -785,7 +844,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   if ((int)outcols.size() < P) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than the Map select list");
   const auto t_host0 = std::chrono::steady_clock::now();
   Event ev0, ev1, ev2, ev3;
-  SVM_HIP(hipEventRecord(ev0.e, c.s));
+  ev0.record(c.s);
   KernelShape ks;
   ks.block = 256;
   Literals lits;
@@ -879,6 +938,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
           col.bytes = sc.bytes;
           col.origin = c.src;
           col.origin_col = src_col[q];
+          col.origin_version = c.src->data_version();
         }
         out->cols.push_back(col);
         ColumnInfo ci;
@@ -908,7 +968,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   SvmArgs a;
   DevBuf blob;
   fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
-  SVM_HIP(hipEventRecord(ev1.e, c.s));
+  ev1.record(c.s);
   uint64_t total = 0;
   std::string src;
   const HostSlots slots = host_slots(c.s);
@@ -923,6 +983,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
       if (col >= 0) {
         out->cols[q].origin = c.src;
         out->cols[q].origin_col = col;
+        out->cols[q].origin_version = c.src->data_version();
       }
     }
   };
@@ -1006,8 +1067,8 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
       launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     }
   }
-  SVM_HIP(hipEventRecord(ev2.e, c.s));
-  SVM_HIP(hipEventRecord(ev3.e, c.s));
+  ev2.record(c.s);
+  ev3.record(c.s);
   res->kind = 0;
   res->rows = (int64_t)total;
   for (int q = 0; q < P; ++q) res->types.push_back(oschema[q]);
@@ -1026,6 +1087,42 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
 }
 
 }  // namespace
+
+std::string scan_tune_report(bool reset) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  std::string j = "[";
+  bool first = true;
+  for (auto& kv : g_tune) {
+    const TuneState& st = kv.second;
+    if (!st.done) continue;
+    int rejected = 0;
+    std::string tr = "[";
+    for (size_t q = 0; q < st.trials.size(); ++q) {
+      rejected += st.trials[q].second < 0 ? 1 : 0;
+      char b[32];
+      snprintf(b, sizeof(b), "%s%.4f", q ? "," : "", st.trials[q].second);
+      tr += b;
+    }
+    tr += "]";
+    int best = -1;
+    for (int ci = 0; ci < kNumTuneCandidates; ++ci) {
+      const TuneChoice& t = kTuneCandidates[ci];
+      if (t.block == st.best.block && t.unroll == st.best.unroll && t.bpc == st.best.bpc && t.stage == st.best.stage &&
+          t.prefetch == st.best.prefetch && t.flat == st.best.flat)
+        best = ci;
+    }
+    std::string id;
+    for (char ch : st.pipeline) {
+      if (ch == '"' || ch == '\\') id += '\\';
+      id += (unsigned char)ch < 0x20 ? ' ' : ch;
+    }
+    j += std::string(first ? "" : ",") + "{\"pipeline\":\"" + id + "\",\"rows\":" + std::to_string(st.rows) +
+         ",\"best\":" + std::to_string(best) + ",\"rejected\":" + std::to_string(rejected) + ",\"trials\":" + tr + "}";
+    first = false;
+  }
+  if (reset) g_tune.clear();
+  return j + "]";
+}
 
 std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOptions& opt) {
   Engine& E = engine();

@@ -70,6 +70,10 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scra
                      unsigned long long* scratch_offsets, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
 
+// True when the nwords 32-bit words at a and b differ (flag: one device word of
+// scratch); synchronises.
+bool words_differ(const void* a, const void* b, uint64_t nwords, uint32_t* flag, hipStream_t s);
+
 // Zeroes words[r] 32-bit words at ptrs[r] for r < n (n <= 4), in one launch.
 void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s);
 

@@ -265,6 +265,15 @@ __global__ void k_zero_ranges(ZeroRanges z) {
     for (uint64_t i = t; i < z.n[r]; i += stride) z.p[r][i] = 0u;
 }
 
+// *flag = 1 when a[i] != b[i] for some i < n.
+__global__ void k_words_differ(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t n,
+                               uint32_t* flag) {
+  uint32_t d = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    d |= a[i] ^ b[i];
+  if (__any(d != 0u) && (threadIdx.x & 63u) == 0u) atomicOr(flag, 1u);
+}
+
 // Zeroes kColumnPad bytes at each pointer (one block per pointer).
 __global__ void k_zero_pads(PadList l) {
   if (blockIdx.x < (unsigned)l.n && threadIdx.x < kPadBytes) l.p[blockIdx.x][threadIdx.x] = 0;
@@ -420,6 +429,18 @@ void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s)
   if (!most) return;
   k_zero_ranges<<<grid_for(most), kBlock, 0, s>>>(z);
   LAUNCH_CHECK("zero_ranges");
+}
+
+bool words_differ(const void* a, const void* b, uint64_t nwords, uint32_t* flag, hipStream_t s) {
+  if (!nwords) return false;
+  k_fill_u32<<<1, 64, 0, s>>>(flag, 1, 0u);
+  k_words_differ<<<grid_for(nwords, kBlock, 4096), kBlock, 0, s>>>((const uint32_t*)a, (const uint32_t*)b, nwords,
+                                                                    flag);
+  LAUNCH_CHECK("words_differ");
+  uint32_t h = 0;
+  SVM_HIP(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipStreamSynchronize(s));
+  return h != 0;
 }
 
 void zero_pads(char* const* ptrs, int n, hipStream_t s) {

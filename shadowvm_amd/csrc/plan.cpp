@@ -135,6 +135,49 @@ static const char* native(int t) {
   }
 }
 
+// Identifiers an expression may use: the reference runtime's functions
+// (com/cpp/cpu-runtime.hpp:37-78), C++'s alternative operator spellings (the Spark
+// control plane writes "and", java_vRDD/src/test/java/SSBHelper.java:371), boolean
+// literals and integer type names for casts.  Anything else would name the generated
+// kernel's own variables or functions, and is refused.
+static bool allowed_identifier(const std::string& id) {
+  static const char* const kAllowed[] = {"StrCmp", "Hash", "StrHash", "true", "false", "and", "or", "not",
+                                         "not_eq", "bitand", "bitor", "xor", "compl", "char", "short", "int",
+                                         "long", "signed", "unsigned"};
+  for (const char* a : kAllowed)
+    if (id == a) return true;
+  return false;
+}
+
+// A numeric token must be a C integer or floating literal.
+static bool numeric_literal(const std::string& t) {
+  size_t k = 0, n = t.size();
+  if (n > 2 && t[0] == '0' && (t[1] == 'x' || t[1] == 'X')) {
+    k = 2;
+    while (k < n && isxdigit((unsigned char)t[k])) k++;
+    if (k == 2) return false;
+  } else {
+    while (k < n && isdigit((unsigned char)t[k])) k++;
+    bool frac = false;
+    if (k < n && t[k] == '.') {
+      frac = true;
+      k++;
+      while (k < n && isdigit((unsigned char)t[k])) k++;
+    }
+    if (k < n && (t[k] == 'e' || t[k] == 'E')) {
+      frac = true;
+      k++;
+      if (k < n && (t[k] == '+' || t[k] == '-')) k++;
+      const size_t d = k;
+      while (k < n && isdigit((unsigned char)t[k])) k++;
+      if (k == d) return false;
+    }
+    if (frac) return k == n || (k + 1 == n && (t[k] == 'f' || t[k] == 'F'));
+  }
+  while (k < n && (t[k] == 'u' || t[k] == 'U' || t[k] == 'l' || t[k] == 'L')) k++;
+  return k == n;
+}
+
 Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduce_key,
                    Literals* lits) {
   Lowered out;
@@ -231,6 +274,7 @@ Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduc
       size_t e = k;
       while (e < n && (isalnum((unsigned char)exp[e]) || exp[e] == '_' || exp[e] == '.')) e++;
       std::string tok = exp.substr(k, e - k);
+      if (!numeric_literal(tok)) fail(SVM_ERR_INVALID, "malformed number '" + tok + "' in '" + exp + "'");
       bool plain = true;
       for (char c : tok) plain = plain && isdigit((unsigned char)c);
       const bool strcmp_len = !parens.empty() && parens.back().first && parens.back().second == 2;
@@ -250,10 +294,32 @@ Lowered lower_expr(const std::string& exp, const Schemas& schemas, bool is_reduc
     if (isalpha((unsigned char)ch) || ch == '_') {
       size_t e = k;
       while (e < n && (isalnum((unsigned char)exp[e]) || exp[e] == '_')) e++;
-      o += exp.substr(k, e - k);
-      after_strcmp = exp.compare(k, e - k, "StrCmp") == 0 && e - k == 6;
+      const std::string id = exp.substr(k, e - k);
+      if (!allowed_identifier(id))
+        fail(SVM_ERR_INVALID, "identifier '" + id + "' is not allowed in '" + exp +
+                                  "' (operands are $(obj.col); functions StrCmp, Hash, StrHash)");
+      o += id;
+      after_strcmp = id == "StrCmp";
       k = e;
       continue;
+    }
+    // Operators and punctuation of C expressions only: no statements (; { }), no
+    // member access or indexing (. [ ] ->), no comments, no assignments or
+    // increments, no preprocessor text.
+    if (strchr(";{}[]#\\@`.", ch))
+      fail(SVM_ERR_INVALID, std::string("character '") + ch + "' is not allowed in '" + exp + "'");
+    if (!isspace((unsigned char)ch) && !strchr("+-*/%<>=!&|^~?:,()", ch))
+      fail(SVM_ERR_INVALID, "character code " + std::to_string((unsigned char)ch) + " is not allowed in '" + exp + "'");
+    const char nx = k + 1 < n ? exp[k + 1] : '\0';
+    if (ch == '/' && (nx == '/' || nx == '*')) fail(SVM_ERR_INVALID, "comments are not allowed in '" + exp + "'");
+    if ((ch == '+' && nx == '+') || (ch == '-' && (nx == '-' || nx == '>')))
+      fail(SVM_ERR_INVALID, "increments and member access are not allowed in '" + exp + "'");
+    if (ch == '=') {
+      // '=' only inside ==, !=, <= and >= (not <<= or >>=, shift assignments)
+      const char pv = k > 0 ? exp[k - 1] : '\0';
+      const bool compare = nx == '=' || (pv && strchr("=!<>", pv));
+      const bool shift_assign = k >= 2 && (pv == '<' || pv == '>') && exp[k - 2] == pv;
+      if (!compare || shift_assign) fail(SVM_ERR_INVALID, "assignments are not allowed in '" + exp + "'");
     }
     o.push_back(ch);
     k++;

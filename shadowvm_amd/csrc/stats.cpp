@@ -1,8 +1,9 @@
 // stats.cpp — cached per-column facts of resident tables (see stats.h).
 //
-// Tables in the store are immutable once loaded, so a column's key range and its
-// dictionary are computed on first use and reused by every later query; the cache
-// holds weak pointers and recomputes when a table id is reused for a new table.
+// A column's key range, dictionary, kept rows and hash tables are computed on first
+// use and reused by every later query.  Each entry holds a weak pointer to its table
+// and the table's data_version() at the time: it is recomputed when the id is reused
+// for a new table or when svm_table_put_column has rewritten the table's columns.
 #include "stats.h"
 
 #include "kernels.h"
@@ -16,6 +17,7 @@ std::map<std::pair<const Table*, int>, KeyEntry> g_keys;
 
 struct KeptEntry {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;
   uint64_t dropped = 0;
   DevBuf kept;  // one byte per row; empty when nothing is dropped
 };
@@ -24,6 +26,7 @@ std::map<std::tuple<const Table*, int, uint64_t, uint64_t>, KeptEntry> g_kept;
 
 struct TableEntry {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;
   uint64_t dropped = 0;
   DevBuf table;  // slots int32
 };
@@ -40,8 +43,15 @@ void prune_dead(M& m) {
 
 struct ClusterEntry {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;
   std::shared_ptr<const k::HtClusters> clusters;  // null: nothing dropped
 };
+
+// An entry computed for table t at its current contents.
+template <typename E>
+bool current(const E& e, const TablePtr& t) {
+  return e.owner.lock() == t && e.version == t->data_version();
+}
 std::map<std::tuple<const Table*, int, uint64_t>, ClusterEntry> g_clusters;
 
 }  // namespace
@@ -53,7 +63,9 @@ std::pair<TablePtr, int> column_root(const TablePtr& t, int col) {
     if (c < 0 || c >= (int)cur->cols.size()) break;
     const Column& k = cur->cols[c];
     TablePtr up = k.origin.lock();
-    if (!up || k.origin_col < 0) break;
+    // a projection holds a subset of its origin's values only while the origin still
+    // holds the values it was projected from
+    if (!up || k.origin_col < 0 || up->data_version() != k.origin_version) break;
     cur = up;
     c = k.origin_col;
   }
@@ -65,9 +77,10 @@ std::shared_ptr<DictEntry> table_dict(const TablePtr& t, int col, hipStream_t s)
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto key = std::make_pair(t.get(), col);
   auto it = g_dicts.find(key);
-  if (it != g_dicts.end() && it->second->owner.lock() == t) return it->second;
+  if (it != g_dicts.end() && current(*it->second, t)) return it->second;
   auto d = std::make_shared<DictEntry>();
   d->owner = t;
+  d->version = t->data_version();
   uint64_t n = t->rows;
   DevBuf vals(n * 4 + 4);
   k::values_of(t->col_ptr(col), t->cols[col].type, t->cols[col].size, n, vals.as<int32_t>(), s);
@@ -84,9 +97,10 @@ KeyEntry table_key(const TablePtr& t, int col, hipStream_t s) {
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto key = std::make_pair(t.get(), col);
   auto it = g_keys.find(key);
-  if (it != g_keys.end() && it->second.owner.lock() == t) return it->second;
+  if (it != g_keys.end() && current(it->second, t)) return it->second;
   KeyEntry e;
   e.owner = t;
+  e.version = t->data_version();
   if (t->rows) {
     DevBuf mm(8);
     k::minmax_i32((const int32_t*)t->col_ptr(col), t->rows, mm.as<int32_t>(), s);
@@ -117,7 +131,7 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
   {
     std::lock_guard<std::mutex> g(g_cache_mu);
     auto it = g_kept.find(key);
-    if (it != g_kept.end() && it->second.owner.lock() == t) {
+    if (it != g_kept.end() && current(it->second, t)) {
       *dropped = it->second.dropped;
       return it->second.dropped ? it->second.kept.as<uint8_t>() : nullptr;
     }
@@ -142,6 +156,7 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
   }
   KeptEntry e;
   e.owner = t;
+  e.version = t->data_version();
   if (replay) {
     DevBuf kept(n + 16);  // cached past this run: not from the stream's per-run arena
     e.dropped = k::ht_kept(keys, n, slots, kept.as<uint8_t>(), s);
@@ -149,7 +164,7 @@ const uint8_t* join_kept(const TablePtr& t, int col, uint64_t* dropped, hipStrea
   }
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto it = g_kept.find(key);
-  if (it == g_kept.end() || it->second.owner.lock() != t) {
+  if (it == g_kept.end() || !current(it->second, t)) {
     prune_dead(g_kept);
     g_kept[key] = std::move(e);
     it = g_kept.find(key);
@@ -165,9 +180,10 @@ std::shared_ptr<const k::HtClusters> table_drop_clusters(const TablePtr& t, int 
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto key = std::make_tuple(t.get(), col, slots);
   auto it = g_clusters.find(key);
-  if (it != g_clusters.end() && it->second.owner.lock() == t) return it->second.clusters;
+  if (it != g_clusters.end() && current(it->second, t)) return it->second.clusters;
   ClusterEntry e;
   e.owner = t;
+  e.version = t->data_version();
   auto hc = std::make_shared<k::HtClusters>();
   k::ht_analyse((const int32_t*)t->col_ptr(col), t->rows, slots, nullptr, hc.get(), s);
   if (hc->dropped) e.clusters = hc;
@@ -190,18 +206,19 @@ const int32_t* join_table(const TablePtr& t, int col, uint64_t* slots, uint64_t*
   {
     std::lock_guard<std::mutex> g(g_cache_mu);
     auto it = g_tables.find(key);
-    if (it != g_tables.end() && it->second.owner.lock() == t) {
+    if (it != g_tables.end() && current(it->second, t)) {
       *dropped = it->second.dropped;
       return it->second.table.as<int32_t>();
     }
   }
   TableEntry e;
   e.owner = t;
+  e.version = t->data_version();
   e.table = DevBuf(*slots * 4 + 16);  // cached past this run: not from the stream's arena
   e.dropped = k::ht_build(t->col_ptr(col), c.size, c.type == String, n, *slots, e.table.as<int32_t>(), s);
   std::lock_guard<std::mutex> g(g_cache_mu);
   auto it = g_tables.find(key);
-  if (it == g_tables.end() || it->second.owner.lock() != t) {
+  if (it == g_tables.end() || !current(it->second, t)) {
     prune_dead(g_tables);
     g_tables[key] = std::move(e);
     it = g_tables.find(key);

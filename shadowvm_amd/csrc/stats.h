@@ -10,11 +10,13 @@ namespace svm {
 
 struct DictEntry {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;  // owner's data_version() when computed
   DevBuf dict, codes;  // codes[r] for the rows of the owner table
   uint32_t n = 0;
 };
 struct KeyEntry {
   std::weak_ptr<Table> owner;
+  uint64_t version = 0;
   int32_t kmin = 0, kmax = -1;
   bool unique = true;
 };

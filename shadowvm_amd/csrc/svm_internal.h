@@ -79,6 +79,7 @@ struct Column {
   // origin bound this column's).
   std::weak_ptr<Table> origin;
   int origin_col = -1;
+  uint64_t origin_version = 0;  // origin's data_version() when the projection was made
 };
 struct Table {
   std::string id;
@@ -97,6 +98,13 @@ struct Table {
   std::vector<Column> cols;
   ObjectInfo info;          // schema as ObjectInfo (ref object info_)
   TablePtr base;            // views keep their base table alive
+  // Bumped by every svm_table_put_column on this table: facts cached about its
+  // columns (key ranges, dictionaries, kept rows, hash tables, profiles) record the
+  // data_version() they were computed at and are recomputed when it has moved.
+  std::atomic<uint64_t> version{0};
+  // Version of the bytes this table reads: its own, plus its base's for a view or for
+  // an object whose columns are shared with its source.
+  uint64_t data_version() const { return version.load() + (base ? base->data_version() : 0); }
   // address of row 0 of column c (views offset into the base allocation)
   const char* col_ptr(int c) const {
     return cols[c].dev ? (const char*)cols[c].dev + row0 * (uint64_t)cols[c].size : nullptr;

@@ -1,6 +1,8 @@
 """Shared helpers: host copies of workload tables and oracle runs of their plans."""
 from __future__ import annotations
 
+import concurrent.futures as cf
+
 import numpy as np
 
 from oracle import oracle as O
@@ -75,3 +77,22 @@ def edge_query(wl, qid, row0, rows, dims, fact_filter, keys, metric):
     fq.AddOperator(P.REDUCEBY, {"dim": keys, "metric": metric}, res)
     fq.AddOperator(P.SINK, {}, res)
     return ssb.Query(qid, view, dpts, (fq, view[0], refs), {})
+
+
+def oracle_slice_query(q, tabs, fact, threads=4):
+    """The oracle's answer to a batch-template query: its dimension pipelines over
+    the dimension tables, then its fact pipeline over its slice of `fact`."""
+    vid, base, row0, rows = q.view
+    view = O.OTable(fact.types, fact.sizes, [c[row0:row0 + rows] for c in fact.columns],
+                    max_tuple_count=3_000_000)
+    dims = {}
+    for pt, src, refs in q.dims:
+        dims[pt.pipe_info["id"]] = O.run_pipeline(pt.plan(src, refs), tabs[src], [], threads=threads)
+    fq, src, refs = q.fact
+    return O.run_pipeline(fq.plan(src, refs), view, [dims[r].table for r in refs], threads=threads)
+
+
+def oracle_slice_queries(qs, tabs, fact, workers=8):
+    """oracle_slice_query for every query, several at once (the plans compile in parallel)."""
+    with cf.ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda q: oracle_slice_query(q, tabs, fact), qs))

@@ -94,6 +94,47 @@ def test_decode_reference_quirks():
     assert d["string_literals"].startswith("MFGR#12")
 
 
+def _lowered_filter(pred, types=(P.INT32, P.INT32, P.STRING), sizes=(4, 4, 10)):
+    pt = P.PipelineTest("w")
+    s = P.create_schema(list(types), list(sizes))
+    pt.AddOperator(P.SOURCE, {"type": "object"}, s)
+    pt.AddOperator(P.FILTER, {"predication": pred}, s)
+    pt.AddOperator(P.MAP, {"select": "$(0.0)"}, s)
+    pt.AddOperator(P.SINK, {}, s)
+    rc, js = describe(pt.plan("src", []))
+    assert rc == 0, js
+    return json.loads(js)["filter"][0][1]
+
+
+@pytest.mark.parametrize("pred", [
+    "$(0.0) > svm_lds[0]",                 # a kernel variable
+    "atomicAdd((unsigned*)0, 1) > 0",      # a device function
+    "$(0.0) > a.pi[0]",                    # the launch arguments
+    "$(0.0) > 0; }",                       # closing the generated statement
+    "$(0.0) > 0 /* && 0 */",               # a comment
+    "$(0.0) > 0 // rest of the line",
+    "5 = 6",                               # an assignment (to a literal parameter)
+    "$(0.0) += 1",
+    "$(0.0) <<= 1",
+    "$(0.0)++ > 0",
+    "$(0.0) > 1.ptr",                      # a malformed number
+    "#define x 1",
+])
+def test_lowering_rejects_foreign_code(pred):
+    low = _lowered_filter(pred)
+    assert low.startswith("!"), low
+    assert "not allowed" in low or "malformed" in low, low
+
+
+def test_lowering_accepts_the_reference_language():
+    # C operators, casts, the alternative spellings the Spark control plane emits
+    # (java_vRDD/src/test/java/SSBHelper.java:371) and the runtime functions
+    low = _lowered_filter('StrCmp($(0.2),"MFGR#2221",10)>=0 and not ($(0.0) != 3) or '
+                          '(long long)$(0.0)*$(0.1) >= 0x10 && $(0.1) % 7 <= 2u && !($(0.0) == 1) ? 1 : 0')
+    assert not low.startswith("!"), low
+    assert "and not" in low and "(long long)(f0)*(f1)" in low
+
+
 def test_decode_rejects_malformed_bytes():
     rc, _ = describe(b"\x0a\xff\xff\xff\xff\x0f")  # length beyond the buffer
     assert rc == _lib.ERR_INVALID

@@ -62,3 +62,19 @@ def test_three_way_forms(engine, monkeypatch, flat):
     wl = ssb.three_way(seed=11, fact_rows=2_000_000, n_part=100_000, n_supp=10_000, prefix=f"tf{flat}")
     got, _ = _run_both(engine, wl)
     assert got.rows > 0
+
+
+def test_int32_metric_wraps(engine):
+    """Per-row int32 products that overflow wrap on the GPU exactly as in the oracle
+    (both compile with -fwrapv); the sums are int64."""
+    from oracle import oracle as O
+
+    from .test_oracle_brute import wrapping_plan, wrapping_table
+    k, a, b = wrapping_table(n=2_000_003, seed=12)
+    P.table_upload("wrap_src", [P.INT32] * 3, [4] * 3, [k, a, b])
+    plan = wrapping_plan()
+    want = O.run_pipeline(plan.plan("wrap_src", []), O.OTable([2, 2, 2], [4, 4, 4], [k, a, b]), [])
+    got = plan.Run("wrap_src", [])
+    assert got.rows == want.rows == 7
+    assert_same_columns(got.columns, want.columns)
+    P.table_drop("wrap_src")

@@ -1,6 +1,7 @@
 """gRPC front end on the GPU: the reference's call sequence (dimension pipelines,
 then the fact pipeline, each one RunPipelineCMD resolving its inputs among cached
-objects) leaves the same output object as running the plans directly; a missing
+objects) leaves the oracle's output objects (every dimension object and the result) and the
+same output object as running the plans directly; a missing
 dependency is answered with CANCELLED as in service/service.cu:39,46."""
 import grpc
 import numpy as np
@@ -9,6 +10,8 @@ import pytest
 from shadowvm_amd import pipeline as P
 from shadowvm_amd import service as S
 from shadowvm_amd import ssb
+
+from .helpers import assert_same_columns, host_tables, oracle_query
 
 pytestmark = pytest.mark.gpu
 
@@ -36,8 +39,18 @@ def test_star_join_through_grpc_equals_direct(server):
     fid = fq.pipe_info["id"]
     rows, ncols = P.table_info(fid)
     got = [P.table_column(fid, c) for c in range(ncols)]
+    # the object the gRPC call cached equals the oracle's answer to the same plans
+    want, want_dims = oracle_query(wl, host_tables(wl))
+    assert rows == want.rows and rows > 0
+    assert_same_columns(got, want.columns)
+    for pt, _, _ in wl.dims:
+        pid = pt.pipe_info["id"]
+        n, nc = P.table_info(pid)
+        w = want_dims[pid]
+        assert n == w.rows
+        assert_same_columns([P.table_column(pid, c) for c in range(len(w.columns))], w.columns)
     direct = fq.Run(src, refs)  # the same plan run in-process (re-caches the same id)
-    assert rows == direct.rows and rows > 0
+    assert rows == direct.rows
     for g, d in zip(got, direct.columns):
         assert np.array_equal(np.asarray(g), np.asarray(d))
 

@@ -156,3 +156,40 @@ def test_group_exists_even_when_sum_is_zero():
     r = O.run_pipeline(fq, fact, [dq.table])
     assert r.columns[0].tolist() == [10, 20]
     assert r.columns[1].tolist() == [0, 0]
+
+
+def wrapping_plan():
+    """Group by a source column; the metric's int32 product overflows on most rows."""
+    from shadowvm_amd import pipeline as P
+    I = P.INT32
+    s = P.create_schema([I, I, I], [4, 4, 4])
+    out = P.create_schema([I, I], [4, 4])
+    pt = P.PipelineTest("wrap")
+    pt.AddOperator(P.SOURCE, {"type": "object"}, s)
+    pt.AddOperator(P.FILTER, {"predication": "$(0.1) != 7"}, out)
+    pt.AddOperator(P.REDUCEBY, {"dim": "$(0.0)", "metric": "$(0.1)*$(0.2)+$(0.2)"}, out)
+    pt.AddOperator(P.SINK, {}, out)
+    return pt
+
+
+def wrapping_table(n=50_000, seed=5):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(-3, 4, n, dtype=np.int32)
+    a = rng.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    b = rng.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    return k, a, b
+
+
+def test_int32_metric_wraps():
+    """Per-row int32 arithmetic wraps in two's complement (both sides compile with
+    -fwrapv); only the accumulator is widened to int64."""
+    k, a, b = wrapping_table()
+    t = O.OTable([2, 2, 2], [4, 4, 4], [k, a, b], max_tuple_count=3_000_000)
+    got = O.run_pipeline(wrapping_plan().plan("src", []), t, [])
+    m = a != 7
+    with np.errstate(over="ignore"):
+        v = (a[m] * b[m] + b[m]).astype(np.int32)  # numpy int32 arithmetic wraps
+    want = _group([k[m]], v.astype(np.int64))
+    assert got.rows == len(want[0])
+    for g, w in zip(got.columns, want):
+        assert np.array_equal(g, w)
