@@ -91,6 +91,26 @@ typedef struct svm_result svm_result;
  * output table under PipelineInfo.id and (if out != NULL) return it on the host. */
 int svm_pipeline_run(const uint8_t* pipelineinfo_pb, size_t len, svm_result** out);
 
+/* svm_pipeline_run without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Runs a ReduceBy (fact) pipeline
+ * and writes its result into caller device buffers: out_cols_device[0..K) are K Int32
+ * key columns, [K..K+M) M Int64 sum columns (ncols = K + M; a scalar has K = 0 and
+ * one row), each max_rows rows; *rows receives the number of result rows (also when
+ * it exceeds max_rows, which fails with SVM_ERR_INVALID).  Calls no hipMalloc /
+ * hipFree / allocating routine: the run's temporaries (join maps, group tables,
+ * emission offsets) come from a static 64 MiB device scratch array per GPU, and a
+ * run that needs more fails with SVM_ERR_NO_MEMORY.  Everything a first run builds
+ * and caches (key ranges, dictionaries, join hash tables, compiled kernels, pinned
+ * host words) must exist already: run the same pipeline shape once with
+ * svm_pipeline_run; otherwise the call fails with SVM_ERR_STATE and allocates
+ * nothing.  The output object is not cached under PipelineInfo.id, no autotuning
+ * trial runs and the run is untimed.  Calls are serialised with each other and with
+ * svm_pipeline_run (one engine stream); the scratch array keeps the last call's
+ * contents between calls. */
+int svm_pipeline_run_noalloc(const uint8_t* pipelineinfo_pb, size_t len, void* const* out_cols_device, int ncols,
+                             uint64_t max_rows, uint64_t* rows);
+
 /* Result access.  Group-by results: K Int32 key columns then M Int64 sums,
  * ordered by key; scalar: one row of M Int64; select: the sink schema's
  * projected columns. */

@@ -61,6 +61,8 @@ SIGNATURES = {
                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(c_vp), ctypes.c_int]),
     "svm_gen_schema": (ctypes.c_int, [ctypes.c_int, c_i32p, c_i64p]),
     "svm_pipeline_run": (ctypes.c_int, [c_u8p, ctypes.c_size_t, ctypes.POINTER(c_vp)]),
+    "svm_pipeline_run_noalloc": (ctypes.c_int, [c_u8p, ctypes.c_size_t, ctypes.POINTER(c_vp), ctypes.c_int,
+                                                ctypes.c_uint64, c_u64p]),
     "svm_result_kind": (ctypes.c_int, [c_vp]),
     "svm_result_rows": (ctypes.c_int64, [c_vp]),
     "svm_result_cols": (ctypes.c_int, [c_vp]),

@@ -212,6 +212,27 @@ int svm_pipeline_run(const uint8_t* pb, size_t len, svm_result** out) {
   });
 }
 
+int svm_pipeline_run_noalloc(const uint8_t* pb, size_t len, void* const* out_cols_device, int ncols,
+                             uint64_t max_rows, uint64_t* rows) {
+  return guard([&] {
+    if (rows) *rows = 0;
+    PipelineInfo info = decode_pipeline_info(pb, len);
+    Engine& E = engine();
+    // serialised with every other run on the engine stream: calls never overlap on
+    // the scratch region, which keeps the last call's contents
+    std::lock_guard<std::mutex> g(E.run_mu);
+    NoAllocScope scope(scratch_region(E.cfg.device), kNoAllocScratchBytes);
+    RunOptions opt;
+    opt.want_host = false;
+    opt.noalloc = true;
+    opt.out_dev = out_cols_device;
+    opt.out_ncols = ncols;
+    opt.out_cap = max_rows;
+    opt.out_rows = rows;
+    (void)run_pipeline(info, opt);
+  });
+}
+
 int svm_result_kind(const svm_result* r) { return r ? r->kind : -1; }
 int64_t svm_result_rows(const svm_result* r) { return r ? r->rows : -1; }
 int svm_result_cols(const svm_result* r) { return r ? (int)r->types.size() : -1; }

@@ -36,7 +36,16 @@ uint64_t bucket_for(uint64_t b) {
   return s;
 }
 
+// The calling thread's no-allocation scope (svm_pipeline_run_noalloc).
+struct NoAlloc {
+  bool on = false;
+  char* base = nullptr;
+  uint64_t cap = 0, off = 0;
+};
+thread_local NoAlloc t_noalloc;
+
 void* raw_alloc(uint64_t bytes) {
+  noalloc_refuse("allocating " + std::to_string(bytes) + " bytes of device memory");
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
   if (e != hipSuccess) {
@@ -47,7 +56,24 @@ void* raw_alloc(uint64_t bytes) {
 }
 }  // namespace
 
+NoAllocScope::NoAllocScope(char* base, uint64_t bytes) {
+  if (t_noalloc.on) fail(SVM_ERR_STATE, "no-allocation scopes do not nest");
+  t_noalloc.on = true;
+  t_noalloc.base = base;
+  t_noalloc.cap = bytes;
+  t_noalloc.off = 0;
+}
+NoAllocScope::~NoAllocScope() { t_noalloc = NoAlloc(); }
+bool noalloc_active() { return t_noalloc.on; }
+void noalloc_refuse(const std::string& what) {
+  if (t_noalloc.on)
+    fail(SVM_ERR_STATE, "svm_pipeline_run_noalloc would need " + what +
+                            "; run the same pipeline shape once with svm_pipeline_run first (it builds the cached "
+                            "key ranges, dictionaries, join tables and kernels), or use svm_pipeline_run");
+}
+
 void* dev_alloc(uint64_t bytes) {
+  noalloc_refuse("a device buffer of " + std::to_string(bytes) + " bytes");
   if (bytes >= kCacheLimit) return raw_alloc(bytes);
   uint64_t b = bucket_for(bytes);
   {
@@ -93,6 +119,16 @@ constexpr uint64_t kArenaChunk = 256ull << 20;
 
 void* arena_alloc(hipStream_t s, uint64_t bytes) {
   bytes = (bytes + 255) & ~255ull;
+  if (t_noalloc.on) {
+    // the run's temporaries come from the scratch region (a bump pointer; the scope
+    // ends with the run)
+    if (bytes > t_noalloc.cap - t_noalloc.off)
+      fail(SVM_ERR_NO_MEMORY, "the pipeline's temporaries exceed the " + std::to_string(t_noalloc.cap >> 20) +
+                                  " MiB scratch region of svm_pipeline_run_noalloc (use svm_pipeline_run)");
+    char* p = t_noalloc.base + t_noalloc.off;
+    t_noalloc.off += bytes;
+    return p;
+  }
   std::lock_guard<std::mutex> g(g_arena_mu);
   Arena& a = g_arenas[s];
   while (a.cur < a.chunks.size()) {
@@ -134,6 +170,7 @@ ScanState scan_state(hipStream_t s, uint64_t tiles) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   ScanBuf& b = g_scan_bufs[s];
   if (b.cap < tiles) {
+    noalloc_refuse("a larger ordered-compaction status buffer");
     // runs on a stream are sequential and synchronised, so the old buffer is idle
     if (b.buf) (void)hipFree(b.buf);
     uint64_t cap = 4096;
@@ -172,6 +209,7 @@ void* host_staging(hipStream_t s, uint64_t bytes) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   Staging& st = g_staging[s];
   if (st.cap < bytes) {
+    noalloc_refuse("a larger pinned staging buffer");
     if (st.p) (void)hipHostFree(st.p);
     uint64_t cap = 1 << 20;
     while (cap < bytes) cap <<= 1;
@@ -193,6 +231,7 @@ HostSlots host_slots(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   auto it = g_host_slots.find(s);
   if (it != g_host_slots.end()) return it->second;
+  noalloc_refuse("the stream's pinned host words");
   void* h = nullptr;
   void* d = nullptr;
   SVM_HIP(hipHostMalloc(&h, kHostSlots * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent));
@@ -219,6 +258,7 @@ void arena_release_all() {
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {
+  noalloc_refuse("allocating " + std::to_string(bytes) + " bytes of device memory");
   void* p = nullptr;
   hipError_t e = hipMallocAsync(&p, bytes ? bytes : 1, s);
   if (e != hipSuccess) {
@@ -423,6 +463,8 @@ void engine_init(const Config& cfg) {
   e->num_cus = prop.multiProcessorCount;
   e->max_lds = (int)prop.sharedMemPerBlock;
   SVM_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  // the device address of this device's no-allocation scratch array
+  scratch_init(cfg.device);
   // keep freed stream-ordered memory in the pool instead of returning it to the
   // driver at every synchronisation
   hipMemPool_t pool;

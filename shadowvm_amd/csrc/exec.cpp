@@ -59,6 +59,8 @@ const char* ctype(int t) {
 // returns, so an event handed back is idle; re-recording one is allowed anyway).
 std::mutex g_event_mu;
 std::vector<hipEvent_t> g_event_pool;
+// Inside a no-allocation run an event is taken only from the pool (none is created:
+// the run then goes untimed).
 struct Event {
   hipEvent_t e = nullptr;
   Event() {
@@ -70,7 +72,7 @@ struct Event {
         return;
       }
     }
-    SVM_HIP(hipEventCreate(&e));
+    if (!noalloc_active()) SVM_HIP(hipEventCreate(&e));
   }
   void record(hipStream_t s) const {
     if (e) SVM_HIP(hipEventRecord(e, s));
@@ -564,6 +566,14 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     fail(SVM_ERR_UNSUPPORTED, "ReduceBy supports 1-3 keys x 1-3 metrics (core/pipeline.cu:380-471) or a scalar");
   const auto& outcols = p.output_info.column_list;
   if ((int)outcols.size() < K + M) fail(SVM_ERR_INVALID, "Sink schema has fewer columns than keys + metrics");
+  if (opt.noalloc) {
+    if (opt.out_ncols != K + M)
+      fail(SVM_ERR_INVALID, "the pipeline's result has " + std::to_string(K + M) + " columns (" + std::to_string(K) +
+                                " keys, " + std::to_string(M) + " sums), " + std::to_string(opt.out_ncols) + " given");
+    for (int q = 0; q < K + M; ++q)
+      if (!opt.out_dev || !opt.out_dev[q]) fail(SVM_ERR_INVALID, "null output column " + std::to_string(q));
+    if (opt.out_cap < 1) fail(SVM_ERR_INVALID, "output capacity of 0 rows");
+  }
   const auto t_host0 = std::chrono::steady_clock::now();
   Event ev0, ev1, ev2, ev3;
   ev0.record(c.s);
@@ -699,6 +709,14 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   const int forced = forced_scan_variant();
   if (forced >= 0) {
     choice = kTuneCandidates[forced];
+  } else if (opt.noalloc) {
+    // no trials: the configuration an earlier run of this shape chose, else the default
+    if (run_scan && autotune_enabled() && opt.block == 256 && opt.unroll == 1) {
+      const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(tkey);
+      if (it != g_tune.end() && it->second.done) choice = it->second.best;
+    }
   } else if (run_scan && autotune_enabled() && rows >= tune_min_rows() && opt.block == 256 && opt.unroll == 1) {
     const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
     std::unique_lock<std::mutex> lk(g_tune_mu);
@@ -784,6 +802,47 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   for (int m = 0; m < M; ++m) oschema.push_back({Int64, 8});
   uint64_t out_rows = 1;
   TablePtr out;
+  if (opt.noalloc) {
+    // into the caller's device buffers; no output object
+    if (K == 0) {
+      for (int m = 0; m < M; ++m)
+        SVM_HIP(hipMemcpyAsync(opt.out_dev[m], (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));
+    } else {
+      uint64_t nb = (nflag + 255) / 256;
+      DevBuf blocks(nb * 8 + 8, c.s), offs(nb * 8 + 16, c.s);
+      out_rows = G ? k::group_count(gflag.as<uint32_t>(), G, blocks.as<unsigned long long>(),
+                                    offs.as<unsigned long long>(), c.s)
+                   : 0;
+      if (opt.out_rows) *opt.out_rows = out_rows;
+      if (out_rows > opt.out_cap)
+        fail(SVM_ERR_INVALID, "the result has " + std::to_string(out_rows) + " groups, the output buffers hold " +
+                                  std::to_string(opt.out_cap));
+      if (out_rows) {
+        k::EmitArgs ea;
+        memset(&ea, 0, sizeof(ea));
+        ea.flags = gflag.as<uint32_t>();
+        ea.sums = gsum.as<uint64_t>();
+        ea.G = G;
+        ea.K = K;
+        ea.M = M;
+        for (int q = 0; q < K; ++q) {
+          ea.n[q] = keys[q].n;
+          ea.stride[q] = keys[q].stride;
+          ea.fmin[q] = keys[q].fmin;
+          ea.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict->dict.as<int32_t>();
+          ea.keys_out[q] = (int32_t*)opt.out_dev[q];
+        }
+        for (int m = 0; m < M; ++m) ea.sums_out[m] = (int64_t*)opt.out_dev[K + m];
+        k::group_emit(ea, offs.as<unsigned long long>(), c.s);
+      }
+    }
+    if (opt.out_rows) *opt.out_rows = out_rows;
+    SVM_HIP(hipStreamSynchronize(c.s));
+    res->kind = K == 0 ? 2 : 1;
+    res->rows = (int64_t)out_rows;
+    res->types = oschema;
+    return res;
+  }
   if (K == 0) {
     out = c.E.store.create(c.info.id, oschema, 1, nullptr, c.s);
     for (int m = 0; m < M; ++m)
@@ -1127,7 +1186,10 @@ std::string scan_tune_report(bool reset) {
 std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOptions& opt) {
   Engine& E = engine();
   Plan plan = decode_plan(info);
-  if (plan.has_loader) return run_load(info, plan, opt);
+  if (plan.has_loader) {
+    if (opt.noalloc) fail(SVM_ERR_UNSUPPORTED, "svm_pipeline_run_noalloc runs ReduceBy pipelines only");
+    return run_load(info, plan, opt);
+  }
   Ctx c{E, plan, info, opt.stream ? opt.stream : E.stream, nullptr, {}, {}};
   // runs on one stream are sequential and each ends synchronised, so the previous
   // run's temporaries on this stream are free
@@ -1136,6 +1198,7 @@ std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOpti
   if (plan.is_reduce && !plan.select_exp.empty())
     fail(SVM_ERR_UNSUPPORTED, "a pipeline with both Map and ReduceBy is not well defined in the reference");
   if (plan.is_reduce) return run_reduce(c, opt);
+  if (opt.noalloc) fail(SVM_ERR_UNSUPPORTED, "svm_pipeline_run_noalloc runs ReduceBy pipelines only");
   if (!plan.select_exp.empty()) return run_select(c, opt);
   fail(SVM_ERR_UNSUPPORTED, "pipeline '" + info.id + "' has neither Map nor ReduceBy");
 }

@@ -24,6 +24,16 @@ struct RunOptions {
   int block = 256;
   int unroll = 1;
   int blocks_per_cu = 0;        // 0: derived from LDS use
+  // No-allocation run of a fact (ReduceBy) pipeline, inside a NoAllocScope: the
+  // group keys and sums are written into these caller device buffers (K Int32 key
+  // columns then M Int64 sum columns, out_cap rows each), no output object is
+  // created or cached, no autotuning trial runs, and the run is untimed.  *out_rows
+  // receives the group count (also when it exceeds out_cap, which is an error).
+  bool noalloc = false;
+  void* const* out_dev = nullptr;
+  int out_ncols = 0;
+  uint64_t out_cap = 0;
+  uint64_t* out_rows = nullptr;
 };
 
 // Runs `info` against the store (pre_pipeline_id resolution as RunPipelineCMD,

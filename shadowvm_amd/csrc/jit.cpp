@@ -192,6 +192,7 @@ hipFunction_t jit_function(const std::string& src, const char* name) {
   auto it = g_mods.find(h);
   if (it != g_mods.end() && it->second.src != src) it = g_mods.end();  // a 64-bit hash collision
   if (it == g_mods.end()) {
+    noalloc_refuse("to compile and load a kernel not loaded yet");
     std::string code = jit_code_object(src);
     Module m;
     m.src = src;
@@ -207,6 +208,7 @@ hipFunction_t jit_function(const std::string& src, const char* name) {
   }
   auto f = it->second.fns.find(name);
   if (f != it->second.fns.end()) return f->second;
+  noalloc_refuse("to look up a kernel function not used yet");
   hipFunction_t fn = nullptr;
   SVM_HIP(hipModuleGetFunction(&fn, it->second.mod, name));
   it->second.fns[name] = fn;

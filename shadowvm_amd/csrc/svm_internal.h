@@ -212,6 +212,27 @@ struct DevBuf {
   template <typename T> T* as() const { return (T*)p; }
 };
 
+// ----------------------------------------------------------------- no-allocation runs
+// svm_pipeline_run_noalloc runs a fact pipeline without allocating: while a
+// NoAllocScope is alive on the calling thread, arena_alloc hands out memory of the
+// device scratch region (scratch.hip) instead of the stream arenas, and everything
+// that would allocate device or pinned host memory, create an event or load a code
+// object (dev_alloc, host staging, a JIT cache miss, ...) fails with SVM_ERR_STATE
+// (the pipeline's cached state -- key ranges, dictionaries, compiled kernels -- is
+// built by one svm_pipeline_run of the same shape beforehand).
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+void scratch_init(int device);     // once per device, with the device current
+char* scratch_region(int device);  // device address of that device's scratch array
+struct NoAllocScope {
+  NoAllocScope(char* base, uint64_t bytes);
+  ~NoAllocScope();
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+};
+bool noalloc_active();
+// Fails with SVM_ERR_STATE, naming `what`, when a NoAllocScope is active.
+void noalloc_refuse(const std::string& what);
+
 // ----------------------------------------------------------------- engine
 struct Config {
   int device = 0;

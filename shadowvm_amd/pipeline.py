@@ -85,6 +85,26 @@ def table_view(id, base_id, row0, rows):
     _lib.check(_lib.lib().svm_table_view(id.encode(), base_id.encode(), row0, rows))
 
 
+def table_column_device(id, col):
+    """Device address of column `col` of table `id`."""
+    p = ctypes.c_void_p()
+    _lib.check(_lib.lib().svm_table_column_device(id.encode(), col, ctypes.byref(p)))
+    return p.value
+
+
+def run_noalloc(plan, out_device_ptrs, max_rows):
+    """svm_pipeline_run_noalloc: run a ReduceBy pipeline without allocating, writing its
+    K key columns (int32) and M sums (int64) into the given device buffers of
+    `max_rows` rows each; returns the number of result rows."""
+    data = _bytes(plan)
+    buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    n = len(out_device_ptrs)
+    ptrs = (ctypes.c_void_p * n)(*out_device_ptrs)
+    rows = ctypes.c_uint64()
+    _lib.check(_lib.lib().svm_pipeline_run_noalloc(buf, len(data), ptrs, n, max_rows, ctypes.byref(rows)))
+    return int(rows.value)
+
+
 def batch_run(plans, want_result=True):
     """svm_batch_run over plan dicts; returns (results list, number fused).
     Absorbed dimension pipelines give None."""
