@@ -102,6 +102,19 @@ __device__ __forceinline__ int svm_probe_str(const char* key, int key_size, cons
   return 0;
 }
 
+/* Reserves one entry of an append buffer per active lane with `want` set: one atomic
+   per wave on the counter, each lane's position its rank among the wanting lanes. */
+__device__ __forceinline__ svm_u64 svm_wave_reserve(svm_u64* counter, bool want) {
+  const svm_u64 m = __ballot(want);
+  if (!m) return 0ull;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  svm_u64 base = 0ull;
+  if (lane == leader) base = atomicAdd(counter, (svm_u64)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (svm_u64)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 #define PI(k) (a.pi[k])
 #define PS(off) ((const char*)(a.sblob + (off)))
 )SVMRT";
@@ -217,8 +230,23 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
 // `if (cond) {` with the row's f<c>, p<j> and JI<o> in scope.
 void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
   const int M = (int)s.metrics.size();
+  if (s.agg == AGG_SORT) {
+    // packed 64-bit key: each component's code at its bit offset
+    o << "        svm_u64 gk = 0ull;\n";
+    for (size_t q = 0; q < s.keys.size(); ++q) {
+      const KeyShape& K = s.keys[q];
+      std::string code;
+      if (K.src == KEY_DIMCODE)
+        code = "(p" + I(K.join) + " - 1u)";
+      else if (K.src == KEY_DIMROW)
+        code = "(unsigned)((const int*)a.ptr[" + I(SVM_SLOT_CODE + (int)q) + "])[JI" + I(K.obj) + "]";
+      else
+        code = "(unsigned)(f" + I(K.col) + " - a.iv[" + I(32 + 4 * (int)q) + "])";
+      o << "        gk |= (svm_u64)" << code << " << a.iv[" << 32 + 4 * q + 3 << "];\n";
+    }
+  }
   // group id
-  if (s.agg != AGG_SCALAR) {
+  if (s.agg != AGG_SCALAR && s.agg != AGG_SORT) {
     o << "        unsigned gid = 0u;\n";
     for (size_t q = 0; q < s.keys.size(); ++q) {
       const KeyShape& K = s.keys[q];
@@ -242,6 +270,28 @@ void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
     for (int m = 0; m < M; ++m)
       o << "        atomicAdd(&lsum[gid * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
     o << "        atomicOr(&lflag[gid >> 5], 1u << (gid & 31u));\n";
+  } else if (s.agg == AGG_SORT) {
+    o << "        unsigned h = (unsigned)((gk * 0x9E3779B97F4A7C15ull) >> (64 - SVM_HBITS));\n"
+      << "        int slot = -1;\n"
+      << "        for (int t = 0; t < " << s.hash_probes << "; ++t) {\n"
+      << "          const svm_u64 k = hkey64[h];\n"
+      << "          if (k == gk) { slot = (int)h; break; }\n"
+      << "          if (k == ~0ull) {\n"
+      << "            const svm_u64 old = atomicCAS(&hkey64[h], ~0ull, gk);\n"
+      << "            if (old == ~0ull || old == gk) { slot = (int)h; break; }\n"
+      << "          }\n"
+      << "          h = (h + 1u) & (SVM_HSLOTS - 1u);\n"
+      << "        }\n"
+      << "        if (slot >= 0) {\n";
+    for (int m = 0; m < M; ++m)
+      o << "          atomicAdd(&hsum[slot * SVM_M + " << m << "], (svm_u64)mv" << m << ");\n";
+    o << "        }\n"
+      << "        const svm_u64 pos = svm_wave_reserve((svm_u64*)a.out[5], slot < 0);\n"
+      << "        if (slot < 0 && pos < svm_cap) {\n"
+      << "          ((svm_u64*)a.out[3])[pos] = gk;\n";
+    for (int m = 0; m < M; ++m)
+      o << "          ((svm_i64*)a.out[4])[pos * SVM_M + " << m << "] = mv" << m << ";\n";
+    o << "        }\n";
   } else {
     o << "        unsigned h = (gid * 2654435761u) >> (32 - SVM_HBITS);\n"
       << "        int slot = -1;\n"
@@ -273,7 +323,7 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
   const int M = (int)s.metrics.size();
   const int U = s.unroll < 1 ? 1 : s.unroll;
   o << "#define SVM_M " << M << "\n";
-  if (s.agg == AGG_HASH) {
+  if (s.agg == AGG_HASH || s.agg == AGG_SORT) {
     o << "#define SVM_HSLOTS " << (1 << s.hash_bits) << "\n";
     o << "#define SVM_HBITS " << s.hash_bits << "\n";
   }
@@ -288,6 +338,12 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       << "  unsigned* lflag = (unsigned*)(lsum + (size_t)G * SVM_M);\n"
       << "  for (unsigned g = threadIdx.x; g < G * SVM_M; g += SVM_BLOCK) lsum[g] = 0ull;\n"
       << "  for (unsigned w = threadIdx.x; w < (G + 31u) / 32u; w += SVM_BLOCK) lflag[w] = 0u;\n";
+  } else if (s.agg == AGG_SORT) {
+    o << "  svm_u64* hkey64 = (svm_u64*)(svm_lds + a.iv[45]);\n"
+      << "  svm_u64* hsum = hkey64 + SVM_HSLOTS;\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS; q += SVM_BLOCK) hkey64[q] = ~0ull;\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS * SVM_M; q += SVM_BLOCK) hsum[q] = 0ull;\n"
+      << "  const svm_u64 svm_cap = (svm_u64)(unsigned)a.iv[46] | ((svm_u64)(unsigned)a.iv[47] << 32);\n";
   } else if (s.agg == AGG_HASH) {
     o << "  unsigned* hkey = svm_lds + a.iv[45];\n"
       << "  svm_u64* hsum = (svm_u64*)(hkey + SVM_HSLOTS);\n"
@@ -414,6 +470,18 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
     o << "    }\n"
       << "  for (unsigned w = threadIdx.x; w < (G + 31u) / 32u; w += SVM_BLOCK)\n"
       << "    if (lflag[w]) atomicOr(&gflag[w], lflag[w]);\n";
+  } else if (s.agg == AGG_SORT) {
+    // every occupied LDS slot becomes one appended entry (SVM_HSLOTS is a multiple of
+    // SVM_BLOCK, so every lane runs the same iterations)
+    o << "  __syncthreads();\n"
+      << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS; q += SVM_BLOCK) {\n"
+      << "    const svm_u64 k = hkey64[q];\n"
+      << "    const svm_u64 pos = svm_wave_reserve((svm_u64*)a.out[5], k != ~0ull);\n"
+      << "    if (k != ~0ull && pos < svm_cap) {\n"
+      << "      ((svm_u64*)a.out[3])[pos] = k;\n";
+    for (int m = 0; m < M; ++m)
+      o << "      ((svm_i64*)a.out[4])[pos * SVM_M + " << m << "] = (svm_i64)hsum[q * SVM_M + " << m << "];\n";
+    o << "    }\n  }\n";
   } else {
     o << "  __syncthreads();\n"
       << "  for (unsigned q = threadIdx.x; q < SVM_HSLOTS; q += SVM_BLOCK) {\n"

@@ -45,7 +45,11 @@ struct KeyShape {
   int col = 0;   // KEY_FACT: source column
 };
 
-enum AggMode { AGG_SCALAR = 0, AGG_DENSE = 1, AGG_HASH = 2 };
+// AGG_SORT: group spaces a dense table cannot hold.  The key codes are packed into
+// one 64-bit key (key q at bit a.iv[32+4q+3]), pre-aggregated in an LDS hash table
+// and appended as (key, sums) entries to HBM (a.out[3] keys, a.out[4] sums, a.out[5]
+// entry counter, capacity a.iv[46] | a.iv[47] << 32) for a sort + reduce by key.
+enum AggMode { AGG_SCALAR = 0, AGG_DENSE = 1, AGG_HASH = 2, AGG_SORT = 3 };
 
 struct KernelShape {
   int block = 256;

@@ -162,6 +162,8 @@ struct JoinBuild {
 struct KeyBuild {
   KeyShape shape;
   uint32_t n = 0, stride = 1;
+  uint64_t card = 0;  // codes 0..card-1 (a source-column key's range may reach 2^32)
+  int bits = 0, shift = 0;  // its field in a packed 64-bit key (wide group spaces)
   int32_t fmin = 0;
   std::shared_ptr<DictEntry> dict;  // sorted values of the key column's origin
   DevBuf codes;                     // code of every dimension row (row-id payloads)
@@ -636,16 +638,38 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     // range of the source column's origin (cached per resident table)
     auto root = column_root(c.src, keys[q].shape.col);
     KeyEntry ke = c.src->rows ? table_key(root.first, root.second, c.s) : KeyEntry();
-    uint64_t range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
-    if (range > kMaxFactKeyRange) fail(SVM_ERR_UNSUPPORTED, "source-column group key range exceeds 2^24");
+    const uint64_t range = c.src->rows ? (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1) : 1;
     keys[q].fmin = ke.kmin;
-    keys[q].n = (uint32_t)range;
+    keys[q].card = range;
+    keys[q].n = (uint32_t)std::min<uint64_t>(range, UINT32_MAX);
   }
+  // Dense group ids (mixed radix over the codes) while the space is small enough for a
+  // group table; beyond that (or a source-column key range above 2^24) the codes are
+  // packed into a 64-bit key and the groups found by sort + reduce (AGG_SORT).
+  bool wide = false;
   uint64_t G = 1;
-  for (int q = K - 1; q >= 0; --q) {
+  for (int q = 0; q < K; ++q) {
+    if (keys[q].shape.src != KEY_FACT) keys[q].card = keys[q].n;
+    if (keys[q].card > kMaxFactKeyRange && keys[q].shape.src == KEY_FACT) wide = true;
+  }
+  for (int q = K - 1; q >= 0 && !wide; --q) {
     keys[q].stride = (uint32_t)G;
-    G *= keys[q].n;
-    if (G > kMaxGroups) fail(SVM_ERR_UNSUPPORTED, "group space exceeds 2^27");
+    G *= keys[q].card;
+    if (G > kMaxGroups) wide = true;
+  }
+  int key_bits = 0;
+  if (wide) {
+    G = 0;
+    for (int q = K - 1; q >= 0; --q) {
+      int b = 0;
+      while (b < 32 && (1ull << b) < keys[q].card) b++;
+      keys[q].bits = b;
+      keys[q].shift = key_bits;
+      key_bits += b;
+    }
+    if (key_bits > 63)
+      fail(SVM_ERR_UNSUPPORTED, "group keys need " + std::to_string(key_bits) + " bits of codes (at most 63)");
+    if (M > 3) fail(SVM_ERR_UNSUPPORTED, "ReduceBy supports 1-3 metrics (core/pipeline.cu:380-471)");
   }
   for (int q = 0; q < K; ++q) ks.keys.push_back(keys[q].shape);
   ks.dcol_slot = assign_dcols(metric_ops);
@@ -653,7 +677,11 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   // aggregation mode and LDS layout
   uint64_t agg_bytes = 0;
   if (K == 0) ks.agg = AGG_SCALAR;
-  else if (G * M * 8 + (G + 31) / 32 * 4 <= kDenseAggBytes) {
+  else if (wide) {
+    ks.agg = AGG_SORT;
+    ks.hash_bits = kHashBits;
+    agg_bytes = (uint64_t)(1 << kHashBits) * (8 + 8 * M);
+  } else if (G * M * 8 + (G + 31) / 32 * 4 <= kDenseAggBytes) {
     ks.agg = AGG_DENSE;
     agg_bytes = G * M * 8 + (G + 31) / 32 * 4;
   } else {
@@ -667,11 +695,11 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
 
   // outputs
   const uint64_t nflag = (G + 31) / 32;
-  DevBuf gsum(K ? G * M * 8 + 8 : 8, c.s), gflag(nflag * 4 + 4, c.s), ssum(8 * 8, c.s);
+  DevBuf gsum(K ? G * M * 8 + 8 : 8, c.s), gflag(nflag * 4 + 4, c.s), ssum(8 * 8, c.s), entries(16, c.s);
   auto clear_outputs = [&] {
-    void* ptrs[3] = {ssum.p, gsum.p, gflag.p};
-    const uint64_t words[3] = {16, K ? G * M * 2 : 0, K ? nflag : 0};
-    k::zero_ranges(ptrs, words, 3, c.s);
+    void* ptrs[4] = {ssum.p, gsum.p, gflag.p, entries.p};
+    const uint64_t words[4] = {16, K ? G * M * 2 : 0, K ? nflag : 0, 4};
+    k::zero_ranges(ptrs, words, 4, c.s);
   };
   clear_outputs();
   SvmArgs a;
@@ -681,15 +709,17 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     a.iv[32 + 4 * q + 0] = keys[q].fmin;
     a.iv[32 + 4 * q + 1] = (int)keys[q].stride;
     a.iv[32 + 4 * q + 2] = (int)keys[q].n;
+    a.iv[32 + 4 * q + 3] = keys[q].shift;
     if (keys[q].shape.src == KEY_DIMROW) a.ptr[SVM_SLOT_CODE + q] = keys[q].codes.p;
   }
+  a.out[5] = entries.p;
   a.iv[44] = (int)G;
   a.iv[45] = 0;
   a.out[0] = gsum.p;
   a.out[1] = gflag.p;
   a.out[2] = ssum.p;
   const uint64_t rows = c.src->rows;
-  const bool run_scan = rows > 0 && (K == 0 || G > 0);
+  const bool run_scan = rows > 0 && (K == 0 || wide || G > 0);
   const uint64_t vec = (rows + 3) / 4;
   TuneChoice choice;
   choice.block = ks.block;
@@ -717,7 +747,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
       auto it = g_tune.find(tkey);
       if (it != g_tune.end() && it->second.done) choice = it->second.best;
     }
-  } else if (run_scan && autotune_enabled() && rows >= tune_min_rows() && opt.block == 256 && opt.unroll == 1) {
+  } else if (run_scan && !wide && autotune_enabled() && rows >= tune_min_rows() && opt.block == 256 &&
+             opt.unroll == 1) {
     const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
     std::unique_lock<std::mutex> lk(g_tune_mu);
     TuneState& st = g_tune[tkey];
@@ -790,11 +821,65 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
   const double t_launch_host = host_ms(t_host0);
   ev1.record(c.s);
-  if (run_scan) {
-    unsigned grid = scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc);
-    launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
+  // wide group spaces: the append buffer of (packed key, sums) entries.  Sized first
+  // for at most 2^24 passing rows; a scan that needs more (its entry count says so)
+  // runs again with the bound: one entry per source row plus one per LDS slot of
+  // every workgroup.
+  DevBuf ekeys, evals;
+  uint64_t ecap = 0, nentries = 0;
+  const unsigned grid = run_scan ? scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc) : 0;
+  auto size_entries = [&](uint64_t cap) {
+    ecap = cap;
+    ekeys = DevBuf(cap * 8 + 8, c.s);
+    evals = DevBuf(cap * 8 * (uint64_t)M + 8, c.s);
+    a.out[3] = ekeys.p;
+    a.out[4] = evals.p;
+    a.iv[46] = (int)(uint32_t)cap;
+    a.iv[47] = (int)(uint32_t)(cap >> 32);
+  };
+  const uint64_t entry_bound = rows + (uint64_t)grid * (1u << kHashBits);
+  if (wide && run_scan) size_entries(std::min<uint64_t>(entry_bound, (1ull << 24) + (uint64_t)grid * (1u << kHashBits)));
+  if (run_scan) launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
+  if (wide && run_scan) {
+    SVM_HIP(hipMemcpyAsync(&nentries, entries.p, 8, hipMemcpyDeviceToHost, c.s));
+    SVM_HIP(hipStreamSynchronize(c.s));
+    if (nentries > ecap) {
+      size_entries(entry_bound);
+      k::fill_u32(entries.p, 2, 0u, c.s);
+      launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
+      SVM_HIP(hipMemcpyAsync(&nentries, entries.p, 8, hipMemcpyDeviceToHost, c.s));
+      SVM_HIP(hipStreamSynchronize(c.s));
+      if (nentries > ecap) fail(SVM_ERR_DEVICE, "group entries exceed their bound");
+    }
   }
   ev2.record(c.s);
+  // wide group spaces: sort the entries by key and sum equal keys
+  uint64_t wide_rows = 0;
+  DevBuf uniq;
+  std::vector<DevBuf> wsums;
+  int64_t* wsum_ptr[8] = {};
+  if (wide) {
+    uniq = DevBuf(nentries * 8 + 8, c.s);
+    for (int m = 0; m < M; ++m) {
+      wsums.emplace_back(nentries * 8 + 8, c.s);
+      wsum_ptr[m] = wsums.back().as<int64_t>();
+    }
+    wide_rows = k::sort_reduce(ekeys.as<uint64_t>(), evals.as<int64_t>(), nentries, M, key_bits,
+                               uniq.as<uint64_t>(), wsum_ptr, c.s);
+  }
+  auto unpack_args = [&](int32_t* const* outs) {
+    k::UnpackArgs ua;
+    memset(&ua, 0, sizeof(ua));
+    ua.K = K;
+    for (int q = 0; q < K; ++q) {
+      ua.shift[q] = keys[q].shift;
+      ua.bits[q] = keys[q].bits;
+      ua.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict->dict.as<int32_t>();
+      ua.fmin[q] = keys[q].fmin;
+      ua.out[q] = outs[q];
+    }
+    return ua;
+  };
 
   // finish: ordered group emission into the output table
   std::vector<std::pair<int, int64_t>> oschema;
@@ -804,7 +889,18 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   TablePtr out;
   if (opt.noalloc) {
     // into the caller's device buffers; no output object
-    if (K == 0) {
+    if (wide) {
+      out_rows = wide_rows;
+      if (opt.out_rows) *opt.out_rows = out_rows;
+      if (out_rows > opt.out_cap)
+        fail(SVM_ERR_INVALID, "the result has " + std::to_string(out_rows) + " groups, the output buffers hold " +
+                                  std::to_string(opt.out_cap));
+      int32_t* kout[3] = {};
+      for (int q = 0; q < K; ++q) kout[q] = (int32_t*)opt.out_dev[q];
+      k::unpack_keys(uniq.as<uint64_t>(), out_rows, unpack_args(kout), c.s);
+      for (int m = 0; m < M && out_rows; ++m)
+        SVM_HIP(hipMemcpyAsync(opt.out_dev[K + m], wsum_ptr[m], out_rows * 8, hipMemcpyDeviceToDevice, c.s));
+    } else if (K == 0) {
       for (int m = 0; m < M; ++m)
         SVM_HIP(hipMemcpyAsync(opt.out_dev[m], (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));
     } else {
@@ -843,7 +939,15 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     res->types = oschema;
     return res;
   }
-  if (K == 0) {
+  if (wide) {
+    out_rows = wide_rows;
+    out = c.E.store.create(c.info.id, oschema, out_rows, nullptr, c.s);
+    int32_t* kout[3] = {};
+    for (int q = 0; q < K; ++q) kout[q] = (int32_t*)out->cols[q].dev;
+    k::unpack_keys(uniq.as<uint64_t>(), out_rows, unpack_args(kout), c.s);
+    for (int m = 0; m < M && out_rows; ++m)
+      SVM_HIP(hipMemcpyAsync(out->cols[K + m].dev, wsum_ptr[m], out_rows * 8, hipMemcpyDeviceToDevice, c.s));
+  } else if (K == 0) {
     out = c.E.store.create(c.info.id, oschema, 1, nullptr, c.s);
     for (int m = 0; m < M; ++m)
       SVM_HIP(hipMemcpyAsync(out->cols[m].dev, (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));

@@ -74,6 +74,21 @@ void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_
 // scratch); synchronises.
 bool words_differ(const void* a, const void* b, uint64_t nwords, uint32_t* flag, hipStream_t s);
 
+// Group-by over wide key spaces (groupsort.hip).  keys[0..n) are packed group keys
+// (key_bits significant bits), vals[i * M + m] their metric sums.  Sorts by key and
+// sums equal keys: uniq_keys[0..G) ascending, sums[m][0..G); returns G (synchronises).
+// Temporaries come from the stream's arena.
+uint64_t sort_reduce(const uint64_t* keys, const int64_t* vals, uint64_t n, int M, int key_bits,
+                     uint64_t* uniq_keys, int64_t* const* sums, hipStream_t s);
+struct UnpackArgs {
+  int K;
+  int shift[3], bits[3];   // field of key q in the packed key
+  const int32_t* dict[3];  // sorted dictionary of a dimension key, or null
+  int32_t fmin[3];         // source-column key: value = code + fmin
+  int32_t* out[3];         // K int32 key columns
+};
+void unpack_keys(const uint64_t* keys, uint64_t G, const UnpackArgs& a, hipStream_t s);
+
 // Zeroes words[r] 32-bit words at ptrs[r] for r < n (n <= 4), in one launch.
 void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s);
 

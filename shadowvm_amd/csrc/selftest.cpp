@@ -87,6 +87,13 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.filter = "StrCmp(((const char*)(FCOL2+(long long)i*10)),PS(0),PI(0))!=PI(1)";
     s.fact_other = {2};
     out.push_back({"scan_row_flat", gen_scan_source(s)});
+    // a group space too wide for a table: packed 64-bit keys, sort + reduce; two metrics
+    s.agg = AGG_SORT;
+    s.metrics.push_back("(f0)-(f1)");
+    s.metric_ctype.push_back("int");
+    out.push_back({"scan_sort_flat", gen_scan_source(s)});
+    s.flat = false;
+    out.push_back({"scan_sort", gen_scan_source(s)});
   }
   {  // probe joins on the reference's hash table: an Int64 key and an Int32 key, grouped
     KernelShape s;

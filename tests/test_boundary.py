@@ -159,8 +159,8 @@ def test_generated_kernels_compile_for_gfx950(tmp_path, monkeypatch):
     rc = L.svm_debug_jit_selftest(buf, len(buf))
     report = buf.value.decode()
     assert rc == 0, report
-    assert report.count(": ok") == 12
-    assert len(os.listdir(tmp_path)) == 12  # code objects cached per source
+    assert report.count(": ok") == 14
+    assert len(os.listdir(tmp_path)) == 14  # code objects cached per source
 
 
 def test_ssb_query_plans_decode_and_lower():
