@@ -55,6 +55,17 @@ int svm_shutdown(void);
 const char* svm_last_error(void);
 const char* svm_version(void);
 
+/* ---- host engine and hybrid execution (ref ResMgr::enable_hybrid_exec_, CPU_ratio_,
+ * CPU_num_, core/resource.cu:14-21; Pipeline::Run, core/pipeline.cu:139-230) ----
+ * A pipeline whose PipelineInfo.isGPU is false runs its per-row transform on host
+ * cores: code generated from the same lowered plan as the kernels, compiled by the
+ * host C++ compiler at run time, one task per bulk of the source.  With hybrid
+ * execution enabled, bulk i of a GPU ReduceBy pipeline runs on the host when
+ * i % 10 < cpu_ratio and on the device otherwise (both at once); the host's partial
+ * group tables are merged into the device's.  At most cpu_threads host tasks run at
+ * once.  Process-wide, like the reference's ResMgr statics (defaults: off, 5, 16). */
+int svm_set_hybrid(int enable, int cpu_ratio, int cpu_threads);
+
 /* ---- device column store (ref ObjectStore / RamObject, object/store.cpp:13-91) ---- */
 /* Create an empty table of `rows` rows; types/val_sizes follow ColumnInfo (com/svm.proto:73-79). */
 int svm_table_create(const char* id, int ncols, const int32_t* types, const int64_t* val_sizes,

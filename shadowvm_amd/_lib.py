@@ -46,6 +46,7 @@ SIGNATURES = {
     "svm_shutdown": (ctypes.c_int, []),
     "svm_last_error": (ctypes.c_char_p, []),
     "svm_version": (ctypes.c_char_p, []),
+    "svm_set_hybrid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "svm_table_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_i32p, c_i64p, ctypes.c_uint64]),
     "svm_table_put_column": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, ctypes.c_uint64]),
     "svm_table_get_column": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, ctypes.c_uint64]),

@@ -20,7 +20,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument", "-Wno-unused-result"]
 SOURCES = ["core.cpp", "proto.cpp", "plan.cpp", "codegen.cpp", "jit.cpp", "exec.cpp",
-           "datagen_host.cpp", "batch.cpp", "capi.cpp", "selftest.cpp", "stats.cpp", "loader.cpp", "kernels.hip", "batch_kernels.hip",
+           "datagen_host.cpp", "batch.cpp", "host_engine.cpp", "capi.cpp", "selftest.cpp", "stats.cpp", "loader.cpp", "kernels.hip", "batch_kernels.hip",
            "loader_kernels.hip", "refhash.hip", "scratch.hip", "groupsort.hip"]
 
 

@@ -283,7 +283,8 @@ struct DimLowered {
 
 DimLowered lower_dim(const PipelineInfo& info, const Plan& p, Engine& E) {
   DimLowered d;
-  if (p.is_reduce || p.select_exp.empty() || !p.join_rkey.empty() || p.has_loader) return d;
+  // a pipeline that asks for the host engine (isGPU false) is not absorbed into a shared scan
+  if (!info.isGPU || p.is_reduce || p.select_exp.empty() || !p.join_rkey.empty() || p.has_loader) return d;
   if (info.pre_pipeline_id.size() != 1) return d;
   TablePtr t = E.store.find(info.pre_pipeline_id[0]);
   if (!t) return d;
@@ -472,7 +473,7 @@ FactLowered lower_fact(const Job& job, const std::map<std::string, DimLowered>& 
   FactLowered f;
   const Plan& p = job.plan;
   const PipelineInfo& info = job.info;
-  if (!p.is_reduce || !p.select_exp.empty() || p.has_loader || info.pre_pipeline_id.empty()) return f;
+  if (!info.isGPU || !p.is_reduce || !p.select_exp.empty() || p.has_loader || info.pre_pipeline_id.empty()) return f;
   if (p.metric_exp.size() != 1 || p.reduce_key.size() > (size_t)bk::kMaxKey) return f;
   if (p.join_rkey.empty() || p.join_rkey.size() > (size_t)bk::kMaxJoin) return f;
   TablePtr src = E.store.find(info.pre_pipeline_id[0]);

@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "exec.h"
+#include "host_engine.h"
 #include "kernels.h"
 #include "stats.h"
 #include "svm_internal.h"
@@ -68,6 +69,16 @@ int svm_shutdown(void) {
   return guard([&] {
     server_stop();
     engine_shutdown();
+  });
+}
+
+int svm_set_hybrid(int enable, int cpu_ratio, int cpu_threads) {
+  return guard([&] {
+    HybridConfig h;
+    h.enabled = enable != 0;
+    h.cpu_ratio = cpu_ratio;
+    h.cpu_threads = cpu_threads;
+    set_hybrid_config(h);
   });
 }
 

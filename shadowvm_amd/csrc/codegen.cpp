@@ -19,19 +19,22 @@
 
 namespace svm {
 
-static const char* kDeviceRT = R"SVMRT(
+// The runtime the generated code calls (Hash / StrHash / StrCmp / StrCopy / Probe of
+// com/cpp/cpu-runtime.hpp:37-152 and the packed-map lookup).  The same text is
+// compiled into the device kernels (SVM_FN = __device__ __forceinline__) and into
+// the host engine's code (SVM_FN = static inline), so both evaluate alike.
+static const char* kCommonRT = R"SVMRT(
 typedef unsigned long long svm_u64;
 typedef long long svm_i64;
-typedef int svm_i32x4 __attribute__((ext_vector_type(4)));
 
 /* Hash31 over n bytes read as signed char, uint32 wrap, top bit cleared
    (reference com/cpp/cuda-runtime.cu:64-71). */
-__device__ __forceinline__ unsigned Hash(const char* p, int n) {
+SVM_FN unsigned Hash(const char* p, int n) {
   unsigned h = 0;
   for (int k = 0; k < n; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
   return h & 0x7FFFFFFFu;
 }
-__device__ __forceinline__ unsigned StrHash(const char* p, int n) {
+SVM_FN unsigned StrHash(const char* p, int n) {
   unsigned h = 0;
   for (int k = 0; k < n && p[k] != '\0'; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
   return h & 0x7FFFFFFFu;
@@ -41,7 +44,7 @@ __device__ __forceinline__ unsigned StrHash(const char* p, int n) {
    every operand is a row of a padded column or a zero-tailed literal) all bytes are
    read first and the answer is selected without branches, so the loads issue together;
    the result is the same as the early-exit loop's. */
-__device__ __forceinline__ int StrCmp(const char* a, const char* b, int n) {
+SVM_FN int StrCmp(const char* a, const char* b, int n) {
   if (n <= 64) {
     int r = 0;
     bool done = false;
@@ -63,12 +66,12 @@ __device__ __forceinline__ int StrCmp(const char* a, const char* b, int n) {
   }
   return 0;
 }
-__device__ __forceinline__ void StrCopy(char* d, const char* s, int n) {
+SVM_FN void StrCopy(char* d, const char* s, int n) {
   for (int k = 0; k < n; ++k) d[k] = s[k];
 }
 
 template <int W>
-__device__ __forceinline__ unsigned svm_map_get(const unsigned* m, unsigned off) {
+SVM_FN unsigned svm_map_get(const unsigned* m, unsigned off) {
   if (W == 32) return m[off];
   constexpr int E = 32 / W;
   constexpr int S = W == 1 ? 5 : W == 2 ? 4 : W == 4 ? 3 : W == 8 ? 2 : 1;
@@ -78,7 +81,7 @@ __device__ __forceinline__ unsigned svm_map_get(const unsigned* m, unsigned off)
 /* Probe<T> (reference com/cpp/cpu-runtime.hpp:125-138): walk at most 10 slots from
    Hash(key bytes) % slots; -1 at an empty slot, the row whose key equals, -1 after 10. */
 template <typename T>
-__device__ __forceinline__ int svm_probe(const T* key, int key_size, const T* tab, const int* ht, unsigned slots) {
+SVM_FN int svm_probe(const T* key, int key_size, const T* tab, const int* ht, unsigned slots) {
   unsigned idx = Hash((const char*)key, key_size) % slots;
   for (int t = 0; t < 10; ++t) {
     const int pos = ht[idx];
@@ -90,7 +93,7 @@ __device__ __forceinline__ int svm_probe(const T* key, int key_size, const T* ta
 }
 /* Probe<char> (:140-152): StrHash home, StrCmp (prefix) equality, and row 0 (not -1)
    after 10 slots. */
-__device__ __forceinline__ int svm_probe_str(const char* key, int key_size, const char* tab, const int* ht,
+SVM_FN int svm_probe_str(const char* key, int key_size, const char* tab, const int* ht,
                                              unsigned slots) {
   unsigned idx = StrHash(key, key_size) % slots;
   for (int t = 0; t < 10; ++t) {
@@ -102,6 +105,13 @@ __device__ __forceinline__ int svm_probe_str(const char* key, int key_size, cons
   return 0;
 }
 
+#define PI(k) (a.pi[k])
+#define PS(off) ((const char*)(a.sblob + (off)))
+)SVMRT";
+
+// Device-only helpers of the generated kernels.
+static const char* kDeviceOnlyRT = R"SVMRT(
+typedef int svm_i32x4 __attribute__((ext_vector_type(4)));
 /* Reserves one entry of an append buffer per active lane with `want` set: one atomic
    per wave on the counter, each lane's position its rank among the wanting lanes. */
 __device__ __forceinline__ svm_u64 svm_wave_reserve(svm_u64* counter, bool want) {
@@ -115,18 +125,19 @@ __device__ __forceinline__ svm_u64 svm_wave_reserve(svm_u64* counter, bool want)
   return base + (svm_u64)__popcll(m & ((1ull << lane) - 1ull));
 }
 
-#define PI(k) (a.pi[k])
-#define PS(off) ((const char*)(a.sblob + (off)))
 )SVMRT";
 
-const char* device_runtime_text() { return kDeviceRT; }
+std::string device_runtime_text() {
+  return std::string("#define SVM_FN __device__ __forceinline__\n") + kCommonRT + kDeviceOnlyRT;
+}
+std::string host_runtime_text() { return std::string("#define SVM_FN static inline\n") + kCommonRT; }
 
 namespace {
 
 std::string I(int v) { return std::to_string(v); }
 
 void emit_prologue(std::ostringstream& o, const KernelShape& s) {
-  o << SVM_KARGS_TEXT << "\n" << kDeviceRT << "\n";
+  o << SVM_KARGS_TEXT << "\n" << device_runtime_text() << "\n";
   o << "#define SVM_BLOCK " << s.block << "\n";
 }
 
@@ -226,9 +237,10 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
   }
 }
 
-// The aggregation of one passing row (group id, metrics, accumulate), emitted inside
-// `if (cond) {` with the row's f<c>, p<j> and JI<o> in scope.
-void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
+// The group key of one passing row (`gk`, packed, for AGG_SORT; else the dense group
+// id `gid`) and its metric values `mv<m>`, emitted inside `if (cond) {` with the
+// row's f<c>, p<j> and JI<o> in scope (device kernels and host engine alike).
+void emit_key_and_metrics(std::ostringstream& o, const KernelShape& s) {
   const int M = (int)s.metrics.size();
   if (s.agg == AGG_SORT) {
     // packed 64-bit key: each component's code at its bit offset
@@ -264,6 +276,12 @@ void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
   for (int m = 0; m < M; ++m)
     o << "        const svm_i64 mv" << m << " = (svm_i64)(" << s.metric_ctype[m] << ")(" << s.metrics[m]
       << ");\n";
+}
+
+// The aggregation of one passing row in a kernel (key, metrics, LDS / HBM accumulation).
+void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
+  const int M = (int)s.metrics.size();
+  emit_key_and_metrics(o, s);
   if (s.agg == AGG_SCALAR) {
     for (int m = 0; m < M; ++m) o << "        acc" << m << " += mv" << m << ";\n";
   } else if (s.agg == AGG_DENSE) {
@@ -494,6 +512,69 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       << "    }\n  }\n";
   }
   o << "}\n";
+  return o.str();
+}
+
+std::string gen_host_source(const KernelShape& s0) {
+  // The host engine's row loop over rows [r_lo, r_hi) of the source: the same filter,
+  // join and key/metric text as the kernels (maps read from host copies), each
+  // passing row accumulated into the task's SvmHostOut (or, for a select, projected
+  // into the task's output columns in row order), as HostQueryTransform's loop does
+  // (codegen/template.cpp:61-73).
+  KernelShape s = s0;
+  for (auto& J : s.joins) J.lds = false;
+  const int M = (int)s.metrics.size();
+  const bool select = !s.proj.empty();
+  std::ostringstream o;
+  o << SVM_KARGS_TEXT << "\n" << SVM_HOSTOUT_TEXT << "\n" << host_runtime_text() << "\n";
+  o << "#define SVM_M " << (M > 0 ? M : 1) << "\n";
+  o << "extern \"C\" void svm_host_task(const SvmArgs* ap, long long r_lo, long long r_hi, SvmHostOut* out) {\n"
+    << "  const SvmArgs& a = *ap;\n";
+  emit_colptrs(o, s);
+  emit_map_ptrs(o, s);
+  o << "  const unsigned G = (unsigned)a.iv[44];\n  (void)G;\n";
+  o << "  for (long long i = r_lo; i < r_hi; ++i) {\n";
+  for (int c : s.fact_i32) o << "    const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
+  o << "    bool cond = true;\n";
+  emit_row_body_joins(o, s, "    ");
+  o << "    if (!cond) continue;\n";
+  if (select) {
+    o << "    const unsigned long long at = out->nsel++;\n";
+    for (size_t q = 0; q < s.proj.size(); ++q) {
+      const int t = s.proj_type[q].first;
+      const long long size = s.proj_type[q].second;
+      const std::string dst = "(out->sel[" + I((int)q) + "] + at * " + std::to_string(size) + "ull)";
+      if (t == 7) {
+        o << "    StrCopy(" << dst << ", (const char*)(" << s.proj[q] << "), " << size << ");\n";
+      } else {
+        const char* ct = t == 0 ? "char" : t == 1 ? "short" : t == 3 ? "long long" : "int";
+        o << "    *(" << ct << "*)" << dst << " = (" << ct << ")(" << s.proj[q] << ");\n";
+      }
+    }
+  } else {
+    o << "    {\n";
+    emit_key_and_metrics(o, s);
+    if (s.agg == AGG_SCALAR) {
+      for (int m = 0; m < M; ++m) o << "        out->acc[" << m << "] += mv" << m << ";\n";
+    } else if (s.agg == AGG_SORT) {
+      o << "        const unsigned long long e = out->nentries++;\n"
+        << "        out->ekeys[e] = gk;\n";
+      for (int m = 0; m < M; ++m) o << "        out->evals[e * SVM_M + " << m << "] = mv" << m << ";\n";
+    } else {
+      o << "        if (out->shared) {\n";
+      for (int m = 0; m < M; ++m)
+        o << "          __atomic_fetch_add(&out->gsum[(svm_u64)gid * SVM_M + " << m << "], (svm_u64)mv" << m
+          << ", __ATOMIC_RELAXED);\n";
+      o << "          __atomic_fetch_or(&out->gflag[gid >> 5], 1u << (gid & 31u), __ATOMIC_RELAXED);\n"
+        << "        } else {\n";
+      for (int m = 0; m < M; ++m)
+        o << "          out->gsum[(svm_u64)gid * SVM_M + " << m << "] += (svm_u64)mv" << m << ";\n";
+      o << "          out->gflag[gid >> 5] |= 1u << (gid & 31u);\n"
+        << "        }\n";
+    }
+    o << "    }\n";
+  }
+  o << "  }\n}\n";
   return o.str();
 }
 

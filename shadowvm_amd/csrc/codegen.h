@@ -77,6 +77,11 @@ struct KernelShape {
 // so that profiles tell them from the query's own scans)
 std::string gen_scan_source(const KernelShape& s, const char* entry = "svm_scan");
 std::string gen_select_source(const KernelShape& s);
-const char* device_runtime_text();
+// C++ source of the host engine's task function for the same pipeline shape:
+//   extern "C" void svm_host_task(const SvmArgs*, long long r_lo, long long r_hi, SvmHostOut*)
+// (a select when s.proj is set, else the reduce of s.agg).
+std::string gen_host_source(const KernelShape& s);
+std::string device_runtime_text();
+std::string host_runtime_text();
 
 }  // namespace svm

@@ -17,6 +17,7 @@
 #include <set>
 
 #include "codegen.h"
+#include "host_engine.h"
 #include "jit.h"
 #include "kargs.h"
 #include "kernels.h"
@@ -558,6 +559,83 @@ bool autotune_enabled() {
   return !(e && e[0] == '0');
 }
 
+// ------------------------------------------------------------------ host engine
+// The source's bulks (its tasks, core/pipeline.cu:161-180), as row ranges relative
+// to the source: bulk 0 is the rows of its bulk 0, the rest tuple_per_bulk rows each.
+std::vector<std::pair<uint64_t, uint64_t>> source_bulks(const Table& t, uint64_t per_bulk) {
+  std::vector<std::pair<uint64_t, uint64_t>> b;
+  const uint64_t n = t.rows, b0 = t.bulk0_rows();
+  if (!n) return b;
+  b.push_back({0, b0 ? b0 : std::min(n, per_bulk)});
+  for (uint64_t lo = b.back().second; lo < n; lo += per_bulk) b.push_back({lo, std::min(n, lo + per_bulk)});
+  return b;
+}
+
+// Which bulks the host engine runs: all of them for a pipeline with isGPU = false;
+// with hybrid execution on, bulk i of a GPU pipeline when i % 10 < cpu_ratio
+// (core/pipeline.cu:172-180).
+std::vector<bool> host_placement(const PipelineInfo& info, size_t nbulks, bool noalloc) {
+  const HybridConfig hy = hybrid_config();
+  const bool all = !info.isGPU;
+  const bool hybrid = !all && hy.enabled && hy.cpu_ratio > 0;
+  if ((all || hybrid) && noalloc)
+    fail(SVM_ERR_UNSUPPORTED, "svm_pipeline_run_noalloc runs GPU pipelines (isGPU true, hybrid execution off)");
+  std::vector<bool> h(nbulks, false);
+  for (size_t b = 0; b < nbulks; ++b) h[b] = all || (hybrid && (int)(b % 10) < hy.cpu_ratio);
+  return h;
+}
+
+// The launch arguments as the host engine reads them: every device buffer the
+// generated code reads (the source's referenced columns, the join maps or hash
+// tables, dimension columns read by expressions, per-row key codes, literal strings)
+// replaced by a host copy (kept alive in `keep`).  Synchronises `c.s` (the maps are
+// built there).
+SvmArgs host_args(Ctx& c, const KernelShape& ks, const SvmArgs& a, const std::vector<JoinBuild>& joins,
+                  const std::vector<KeyBuild>& keys, const Literals& lits,
+                  std::vector<std::shared_ptr<const HostBytes>>& keep) {
+  SVM_HIP(hipStreamSynchronize(c.s));
+  SvmArgs h = a;
+  for (auto& p : h.ptr) p = nullptr;
+  for (auto& p : h.out) p = nullptr;
+  std::vector<int> fcols = ks.fact_i32;
+  fcols.insert(fcols.end(), ks.fact_other.begin(), ks.fact_other.end());
+  for (int col : fcols) {
+    auto m = host_mirror(c.src, col);
+    keep.push_back(m);
+    h.ptr[col] = m->ptr();
+  }
+  for (size_t j = 0; j < joins.size(); ++j) {
+    std::shared_ptr<HostBytes> b;
+    if (joins[j].table) b = host_copy(joins[j].table, joins[j].slots * 4, 16);
+    else if (joins[j].packed.p) b = host_copy(joins[j].packed.p, joins[j].packed.bytes, 16);
+    if (b) {
+      keep.push_back(b);
+      h.ptr[SVM_SLOT_MAP + j] = b->ptr();
+    }
+  }
+  for (auto& kv : ks.dcol_slot) {
+    const Table& D = *c.refs[kv.first.first - 1];
+    const uint64_t size = (uint64_t)D.cols[kv.first.second].size;
+    auto b = host_copy(D.col_ptr(kv.first.second), D.rows * size, kColumnPad);
+    keep.push_back(b);
+    h.ptr[kv.second] = b->ptr();
+  }
+  for (size_t q = 0; q < keys.size(); ++q)
+    if (keys[q].shape.src == KEY_DIMROW && keys[q].codes.p) {
+      auto b = host_copy(keys[q].codes.p, keys[q].codes.bytes, 16);
+      keep.push_back(b);
+      h.ptr[SVM_SLOT_CODE + q] = b->ptr();
+    }
+  auto blob = std::make_shared<HostBytes>();
+  blob->data.reset(new char[lits.blob.size() + 4096]);
+  blob->size = lits.blob.size() + 4096;
+  memset(blob->data.get(), 0, blob->size);
+  if (!lits.blob.empty()) memcpy(blob->data.get(), lits.blob.data(), lits.blob.size());
+  keep.push_back(blob);
+  h.sblob = blob->ptr();
+  return h;
+}
+
 // ------------------------------------------------------------------ reduce path
 std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   const Plan& p = c.plan;
@@ -821,10 +899,33 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
   const double t_launch_host = host_ms(t_host0);
   ev1.record(c.s);
+  // Placement of the source's bulks: the host engine's (all of them for isGPU =
+  // false, every bulk i with i % 10 < cpu_ratio under hybrid execution) and runs of
+  // consecutive device bulks, one scan launch each.
+  const auto bulks = source_bulks(*c.src, c.E.cfg.tuple_per_bulk);
+  const std::vector<bool> on_host = host_placement(c.info, bulks.size(), opt.noalloc);
+  std::vector<size_t> host_bulks;
+  std::vector<std::pair<uint64_t, uint64_t>> dev_runs;
+  for (size_t b = 0; b < bulks.size(); ++b) {
+    if (on_host[b]) host_bulks.push_back(b);
+    else if (!dev_runs.empty() && dev_runs.back().second == bulks[b].first) dev_runs.back().second = bulks[b].second;
+    else dev_runs.push_back(bulks[b]);
+  }
+  const bool dev_scan = run_scan && !dev_runs.empty();
+  const bool host_scan = run_scan && !host_bulks.empty();
+  // the host engine's arguments and code, ready before the device scan starts so that
+  // both run at once
+  std::vector<std::shared_ptr<const HostBytes>> keep;
+  SvmArgs ha;
+  HostTaskFn hfn = nullptr;
+  if (host_scan) {
+    hfn = host_task_function(gen_host_source(ks));
+    ha = host_args(c, ks, a, joins, keys, lits, keep);
+  }
   // wide group spaces: the append buffer of (packed key, sums) entries.  Sized first
   // for at most 2^24 passing rows; a scan that needs more (its entry count says so)
   // runs again with the bound: one entry per source row plus one per LDS slot of
-  // every workgroup.
+  // every workgroup of every launch.
   DevBuf ekeys, evals;
   uint64_t ecap = 0, nentries = 0;
   const unsigned grid = run_scan ? scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc) : 0;
@@ -837,22 +938,126 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     a.iv[46] = (int)(uint32_t)cap;
     a.iv[47] = (int)(uint32_t)(cap >> 32);
   };
-  const uint64_t entry_bound = rows + (uint64_t)grid * (1u << kHashBits);
-  if (wide && run_scan) size_entries(std::min<uint64_t>(entry_bound, (1ull << 24) + (uint64_t)grid * (1u << kHashBits)));
-  if (run_scan) launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
-  if (wide && run_scan) {
+  uint64_t dev_rows = 0;
+  for (auto& r : dev_runs) dev_rows += r.second - r.first;
+  const uint64_t flush_entries = (uint64_t)dev_runs.size() * grid * (1u << kHashBits);
+  const uint64_t entry_bound = dev_rows + flush_entries;
+  auto launch_device = [&] {
+    for (auto& r : dev_runs) {
+      SvmArgs a2 = a;
+      a2.row0 = (long long)(c.src->row0 + r.first);
+      a2.rows = (long long)(r.second - r.first);
+      launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a2);
+    }
+  };
+  if (wide && dev_scan) size_entries(std::min<uint64_t>(entry_bound, (1ull << 24) + flush_entries));
+  if (dev_scan) launch_device();
+  // the host bulks meanwhile, each a task of the host engine
+  const auto t_host_scan = std::chrono::steady_clock::now();
+  std::vector<int64_t> hacc(8, 0);
+  std::vector<uint64_t> hsum;
+  std::vector<uint32_t> hflag;
+  std::vector<uint64_t> hkeys;
+  std::vector<int64_t> hvals;
+  if (host_scan) {
+    const HybridConfig hy = hybrid_config();
+    // a group table shared by all tasks (atomic adds) when it is large, else one per
+    // task, summed into the result as each task ends
+    const bool shared_tab = !wide && K > 0 && G * M * 8 > (4ull << 20);
+    if (K > 0 && !wide) {
+      hsum.assign(G * M, 0);
+      hflag.assign(nflag, 0);
+    }
+    std::mutex merge_mu;
+    run_host_tasks(host_bulks.size(), hy.cpu_threads, [&](size_t t) {
+      const auto& bk = bulks[host_bulks[t]];
+      const uint64_t n = bk.second - bk.first;
+      SvmHostOut o;
+      memset(&o, 0, sizeof(o));
+      std::vector<uint64_t> tsum, tkeys;
+      std::vector<uint32_t> tflag;
+      std::vector<int64_t> tvals;
+      if (K > 0 && !wide) {
+        if (shared_tab) {
+          o.shared = 1;
+          o.gsum = (unsigned long long*)hsum.data();
+          o.gflag = hflag.data();
+        } else {
+          tsum.assign(G * M, 0);
+          tflag.assign(nflag, 0);
+          o.gsum = (unsigned long long*)tsum.data();
+          o.gflag = tflag.data();
+        }
+      } else if (wide) {
+        tkeys.resize(n);
+        tvals.resize(n * (uint64_t)M);
+        o.ekeys = (unsigned long long*)tkeys.data();
+        o.evals = (long long*)tvals.data();
+      }
+      hfn(&ha, (long long)(c.src->row0 + bk.first), (long long)(c.src->row0 + bk.second), &o);
+      std::lock_guard<std::mutex> g(merge_mu);
+      for (int m = 0; m < M && K == 0; ++m) hacc[m] += o.acc[m];
+      if (K > 0 && !wide && !shared_tab) {
+        for (uint64_t i = 0; i < G * M; ++i) hsum[i] += tsum[i];
+        for (uint64_t w = 0; w < nflag; ++w) hflag[w] |= tflag[w];
+      }
+      if (wide) {
+        hkeys.insert(hkeys.end(), tkeys.begin(), tkeys.begin() + (long)o.nentries);
+        hvals.insert(hvals.end(), tvals.begin(), tvals.begin() + (long)(o.nentries * M));
+      }
+    });
+  }
+  const double host_scan_ms = host_scan ? host_ms(t_host_scan) : 0.0;
+  if (wide && dev_scan) {
     SVM_HIP(hipMemcpyAsync(&nentries, entries.p, 8, hipMemcpyDeviceToHost, c.s));
     SVM_HIP(hipStreamSynchronize(c.s));
     if (nentries > ecap) {
       size_entries(entry_bound);
       k::fill_u32(entries.p, 2, 0u, c.s);
-      launch(fn, grid, (unsigned)ks.block, (unsigned)(run_words * 4), c.s, a);
+      launch_device();
       SVM_HIP(hipMemcpyAsync(&nentries, entries.p, 8, hipMemcpyDeviceToHost, c.s));
       SVM_HIP(hipStreamSynchronize(c.s));
       if (nentries > ecap) fail(SVM_ERR_DEVICE, "group entries exceed their bound");
     }
   }
   ev2.record(c.s);
+  // the host engine's partial results join the device's
+  if (host_scan) {
+    if (K == 0) {
+      int64_t d[8] = {0};
+      SVM_HIP(hipMemcpyAsync(d, ssum.p, 8 * M, hipMemcpyDeviceToHost, c.s));
+      SVM_HIP(hipStreamSynchronize(c.s));
+      for (int m = 0; m < M; ++m) d[m] = (int64_t)((uint64_t)d[m] + (uint64_t)hacc[m]);
+      SVM_HIP(hipMemcpyAsync(ssum.p, d, 8 * M, hipMemcpyHostToDevice, c.s));
+      SVM_HIP(hipStreamSynchronize(c.s));
+    } else if (!wide) {
+      DevBuf tmp(G * M * 8 + nflag * 4 + 8, c.s);
+      SVM_HIP(hipMemcpyAsync(tmp.p, hsum.data(), G * M * 8, hipMemcpyHostToDevice, c.s));
+      SVM_HIP(hipMemcpyAsync((char*)tmp.p + G * M * 8, hflag.data(), nflag * 4, hipMemcpyHostToDevice, c.s));
+      k::add_u64(gsum.p, tmp.p, G * M, c.s);
+      k::or_u32(gflag.p, (char*)tmp.p + G * M * 8, nflag, c.s);
+      SVM_HIP(hipStreamSynchronize(c.s));
+    } else {
+      const uint64_t nh = hkeys.size();
+      if (nentries + nh > ecap) {
+        DevBuf okeys = std::move(ekeys), ovals = std::move(evals);
+        size_entries(nentries + nh);
+        if (nentries) {
+          SVM_HIP(hipMemcpyAsync(ekeys.p, okeys.p, nentries * 8, hipMemcpyDeviceToDevice, c.s));
+          SVM_HIP(hipMemcpyAsync(evals.p, ovals.p, nentries * 8 * M, hipMemcpyDeviceToDevice, c.s));
+        }
+      }
+      if (nh) {
+        SVM_HIP(hipMemcpyAsync(ekeys.as<uint64_t>() + nentries, hkeys.data(), nh * 8, hipMemcpyHostToDevice, c.s));
+        SVM_HIP(hipMemcpyAsync(evals.as<int64_t>() + nentries * M, hvals.data(), nh * 8 * M, hipMemcpyHostToDevice,
+                               c.s));
+      }
+      SVM_HIP(hipStreamSynchronize(c.s));
+      nentries += nh;
+    }
+    res->tune += " host " + std::to_string(host_bulks.size()) + "/" + std::to_string(bulks.size()) + " bulks " +
+                 std::to_string(host_scan_ms) + " ms";
+  }
   // wide group spaces: sort the entries by key and sum equal keys
   uint64_t wide_rows = 0;
   DevBuf uniq;
@@ -1119,6 +1324,64 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
                 c.info.id.c_str(), host_ms(t_host0), c.src->id.c_str(), (unsigned long long)out->rows);
       return res;
     }
+  }
+  if (!c.info.isGPU) {
+    // The host engine (PipelineInfo.isGPU = false): each bulk of the source is a task
+    // that writes its passing rows in row order; the tasks' rows are concatenated in
+    // bulk order (sink bulk i holds what source bulk i produced), then copied to HBM.
+    (void)host_placement(c.info, 0, opt.noalloc);
+    SvmArgs a;
+    DevBuf blob;
+    fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
+    std::vector<std::shared_ptr<const HostBytes>> keep;
+    HostTaskFn hfn = host_task_function(gen_host_source(ks));
+    std::vector<KeyBuild> nokeys2;
+    const SvmArgs ha = host_args(c, ks, a, joins, nokeys2, lits, keep);
+    const auto bulks = source_bulks(*c.src, c.E.cfg.tuple_per_bulk);
+    std::vector<std::vector<std::vector<char>>> tbuf(bulks.size());
+    std::vector<uint64_t> tn(bulks.size(), 0);
+    run_host_tasks(bulks.size(), hybrid_config().cpu_threads, [&](size_t t) {
+      const uint64_t n = bulks[t].second - bulks[t].first;
+      SvmHostOut o;
+      memset(&o, 0, sizeof(o));
+      tbuf[t].resize(P);
+      for (int q = 0; q < P; ++q) {
+        tbuf[t][q].resize(n * (uint64_t)oschema[q].second + 16);
+        o.sel[q] = tbuf[t][q].data();
+      }
+      hfn(&ha, (long long)(c.src->row0 + bulks[t].first), (long long)(c.src->row0 + bulks[t].second), &o);
+      tn[t] = o.nsel;
+    });
+    uint64_t total = 0;
+    for (uint64_t n : tn) total += n;
+    TablePtr out = c.E.store.create(c.info.id, oschema, total, &mat);
+    out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+    out->bulk0 = tn.empty() ? 0 : tn[0];
+    res->kind = 0;
+    res->rows = (int64_t)total;
+    for (int q = 0; q < P; ++q) {
+      res->types.push_back(oschema[q]);
+      const uint64_t size = (uint64_t)oschema[q].second;
+      std::vector<uint8_t> host(total * size);
+      uint64_t off = 0;
+      for (size_t t = 0; t < bulks.size(); ++t) {
+        if (tn[t]) memcpy(host.data() + off * size, tbuf[t][q].data(), tn[t] * size);
+        off += tn[t];
+      }
+      if (total) SVM_HIP(hipMemcpy(out->cols[q].dev, host.data(), host.size(), hipMemcpyHostToDevice));
+      const int col = bare_source_col(p.select_exp[q], c.schemas, oschema[q].first, oschema[q].second);
+      if (col >= 0) {
+        out->cols[q].origin = c.src;
+        out->cols[q].origin_col = col;
+        out->cols[q].origin_version = c.src->data_version();
+      }
+      if (opt.want_host) res->cols.push_back(std::move(host));
+    }
+    c.E.store.put(c.info.id, out);
+    if (c.E.cfg.verbose)
+      fprintf(stderr, "[svm] %s: select on the host engine, %zu bulks, %.3f ms; rows %llu\n", c.info.id.c_str(),
+              bulks.size(), host_ms(t_host0), (unsigned long long)total);
+    return res;
   }
   // A source of one bulk is compacted into an output sized for every row (trimmed
   // to the passing rows once counted) when that capacity is modest: in one launch up

@@ -27,9 +27,33 @@
 
 SVM_KARGS_DEF
 
+// What one host task of the CPU engine writes (the generated host code and libsvm
+// share this text, as they share SvmArgs):
+//   acc        scalar sums (K = 0)
+//   gsum/gflag dense group table (G x M int64) and its flags; `shared`: the table is
+//              updated by concurrent tasks (atomic adds), else private to the task
+//   ekeys/evals/nentries  packed group keys and metrics of a wide group space, one
+//              entry per passing row
+//   sel/nsel   select: the task's projected columns and its row count
+#define SVM_HOSTOUT_DEF                                                                   \
+  struct SvmHostOut {                                                                     \
+    long long acc[8];                                                                     \
+    unsigned long long* gsum;                                                             \
+    unsigned* gflag;                                                                      \
+    int shared;                                                                           \
+    unsigned long long* ekeys;                                                            \
+    long long* evals;                                                                     \
+    unsigned long long nentries;                                                          \
+    char* sel[10];                                                                        \
+    unsigned long long nsel;                                                              \
+  };
+
+SVM_HOSTOUT_DEF
+
 #define SVM_KSTR2(x) #x
 #define SVM_KSTR(x) SVM_KSTR2(x)
 #define SVM_KARGS_TEXT SVM_KSTR(SVM_KARGS_DEF)
+#define SVM_HOSTOUT_TEXT SVM_KSTR(SVM_HOSTOUT_DEF)
 
 #define SVM_SLOT_FACT 0
 #define SVM_SLOT_MAP 24

@@ -70,6 +70,11 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scra
                      unsigned long long* scratch_offsets, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
 
+// dst[i] += src[i] for n 64-bit words; dst[i] |= src[i] for n 32-bit words (merging
+// a host engine's partial group table into the device's).
+void add_u64(void* dst, const void* src, uint64_t n, hipStream_t s);
+void or_u32(void* dst, const void* src, uint64_t n, hipStream_t s);
+
 // True when the nwords 32-bit words at a and b differ (flag: one device word of
 // scratch); synchronises.
 bool words_differ(const void* a, const void* b, uint64_t nwords, uint32_t* flag, hipStream_t s);

@@ -274,6 +274,15 @@ __global__ void k_words_differ(const uint32_t* __restrict__ a, const uint32_t* _
   if (__any(d != 0u) && (threadIdx.x & 63u) == 0u) atomicOr(flag, 1u);
 }
 
+__global__ void k_add_u64(unsigned long long* __restrict__ dst, const unsigned long long* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] += src[i];
+}
+__global__ void k_or_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] |= src[i];
+}
+
 // Zeroes kColumnPad bytes at each pointer (one block per pointer).
 __global__ void k_zero_pads(PadList l) {
   if (blockIdx.x < (unsigned)l.n && threadIdx.x < kPadBytes) l.p[blockIdx.x][threadIdx.x] = 0;
@@ -429,6 +438,18 @@ void zero_ranges(void* const* ptrs, const uint64_t* words, int n, hipStream_t s)
   if (!most) return;
   k_zero_ranges<<<grid_for(most), kBlock, 0, s>>>(z);
   LAUNCH_CHECK("zero_ranges");
+}
+
+void add_u64(void* dst, const void* src, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_add_u64<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>((unsigned long long*)dst, (const unsigned long long*)src, n);
+  LAUNCH_CHECK("add_u64");
+}
+
+void or_u32(void* dst, const void* src, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_or_u32<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>((uint32_t*)dst, (const uint32_t*)src, n);
+  LAUNCH_CHECK("or_u32");
 }
 
 bool words_differ(const void* a, const void* b, uint64_t nwords, uint32_t* flag, hipStream_t s) {

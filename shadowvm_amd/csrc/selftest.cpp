@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "codegen.h"
+#include "host_engine.h"
 #include "jit.h"
 #include "svm_internal.h"
 
@@ -144,6 +145,75 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
   return out;
 }
 
+// Host-engine sources of representative shapes: scalar, dense, large dense (shared
+// table), wide (packed keys), deferred maps, row-id payloads, a source-column key,
+// probe joins and a select with a string-key probe.
+std::vector<std::pair<std::string, std::string>> selftest_host_sources() {
+  std::vector<std::pair<std::string, std::string>> out;
+  {
+    KernelShape s;
+    s.fact_i32 = {0, 1, 2, 3};
+    s.filter = "(f2)>=PI(0)&&(f2)<=PI(1)&&(f1)<PI(2)";
+    JoinShape j; j.robj = 1; j.lcol = 0; j.payload = PAY_EXIST; j.wbits = 1; j.lds = true;
+    s.joins = {j};
+    s.metrics = {"(f3)*(f2)"};
+    s.metric_ctype = {"int"};
+    s.agg = AGG_SCALAR;
+    out.push_back({"host_scalar", gen_host_source(s)});
+  }
+  {
+    KernelShape s;
+    s.fact_i32 = {0, 1, 2, 3, 4, 5};
+    JoinShape a; a.robj = 1; a.lcol = 0; a.payload = PAY_CODE; a.wbits = 2; a.lds = true;
+    JoinShape b; b.robj = 2; b.lcol = 1; b.payload = PAY_CODE; b.wbits = 8; b.defer = true;
+    JoinShape c; c.robj = 3; c.lcol = 2; c.payload = PAY_ROW; c.wbits = 32;
+    s.joins = {a, b, c};
+    s.dcol_slot[{3, 1}] = 32;
+    KeyShape k0; k0.src = KEY_DIMCODE; k0.join = 0; k0.obj = 1;
+    KeyShape k1; k1.src = KEY_DIMCODE; k1.join = 1; k1.obj = 2;
+    KeyShape k2; k2.src = KEY_DIMROW; k2.join = 2; k2.obj = 3; k2.col = 2;
+    s.keys = {k0, k1, k2};
+    s.metrics = {"(f4)-(f5)", "(*(const int*)(DCOL3_1+(long long)JI3*4))"};
+    s.metric_ctype = {"int", "long long"};
+    s.agg = AGG_HASH;
+    out.push_back({"host_hash", gen_host_source(s)});
+    s.agg = AGG_SORT;
+    s.keys[2].src = KEY_FACT;
+    s.keys[2].col = 3;
+    out.push_back({"host_sort", gen_host_source(s)});
+  }
+  {
+    KernelShape s;
+    s.fact_i32 = {1, 3};
+    s.fact_other = {0, 2};
+    s.filter = "StrCmp(((const char*)(FCOL2+(long long)i*10)),PS(0),10)!=PI(1)";
+    JoinShape a; a.robj = 1; a.lcol = 0; a.payload = PAY_ROW; a.wbits = 32;
+    a.kind = JOIN_PROBE; a.rcol = 0; a.ktype = Int64; a.ksize = 8;
+    s.joins = {a};
+    s.dcol_slot[{1, 0}] = 32;
+    KeyShape k0; k0.src = KEY_FACT; k0.col = 1;
+    s.keys = {k0};
+    s.metrics = {"(f3)"};
+    s.metric_ctype = {"int"};
+    s.agg = AGG_DENSE;
+    out.push_back({"host_probe", gen_host_source(s)});
+  }
+  {
+    KernelShape s;
+    s.fact_other = {2, 4};
+    s.filter = "StrCmp(((const char*)(FCOL4+(long long)i*10)),PS(0),PI(0))==PI(1)";
+    JoinShape a; a.robj = 1; a.lcol = 2; a.payload = PAY_ROW; a.wbits = 32;
+    a.kind = JOIN_PROBE_STR; a.rcol = 0; a.ktype = String; a.ksize = 40;
+    s.joins = {a};
+    s.dcol_slot[{1, 0}] = 32;
+    s.dcol_slot[{1, 1}] = 33;
+    s.proj = {"((const char*)(DCOL1_1+(long long)JI1*40))", "((const char*)(FCOL2+(long long)i*40))", "(PI(2))"};
+    s.proj_type = {{String, 40}, {String, 40}, {Int64, 8}};
+    out.push_back({"host_select", gen_host_source(s)});
+  }
+  return out;
+}
+
 }  // namespace svm
 
 extern "C" int svm_debug_jit_selftest(char* buf, size_t cap) {
@@ -155,6 +225,16 @@ extern "C" int svm_debug_jit_selftest(char* buf, size_t cap) {
       report += kv.first + ": ok (" + std::to_string(co.size()) + " bytes)\n";
     } catch (const std::exception& e) {
       report += kv.first + ": FAILED\n" + e.what() + "\n";
+      rc = -1;
+    }
+  }
+  // the host engine's code for the same shapes, compiled by the host compiler and loaded
+  for (auto& kv : svm::selftest_host_sources()) {
+    try {
+      (void)svm::host_task_function(kv.second);
+      report += kv.first + ": host ok\n";
+    } catch (const std::exception& e) {
+      report += kv.first + ": host FAILED\n" + e.what() + "\n";
       rc = -1;
     }
   }

@@ -43,6 +43,12 @@ def shutdown():
     _lib.check(_lib.lib().svm_shutdown())
 
 
+def set_hybrid(enabled, cpu_ratio=5, cpu_threads=16):
+    """Hybrid execution (ResMgr::enable_hybrid_exec_ / CPU_ratio_ / CPU_num_): bulk i of
+    a GPU ReduceBy pipeline runs on host cores when i % 10 < cpu_ratio."""
+    _lib.check(_lib.lib().svm_set_hybrid(1 if enabled else 0, cpu_ratio, cpu_threads))
+
+
 def table_create(id, types, sizes, rows):
     n = len(types)
     t = (ctypes.c_int32 * n)(*types)

@@ -160,7 +160,10 @@ def test_generated_kernels_compile_for_gfx950(tmp_path, monkeypatch):
     report = buf.value.decode()
     assert rc == 0, report
     assert report.count(": ok") == 14
-    assert len(os.listdir(tmp_path)) == 14  # code objects cached per source
+    assert len([f for f in os.listdir(tmp_path) if f.endswith(".co")]) == 14  # code objects cached per source
+    # the host engine's code of five shapes, compiled by the host compiler and loaded
+    assert report.count(": host ok") == 5
+    assert len([f for f in os.listdir(tmp_path) if f.startswith("host_") and f.endswith(".so")]) == 5
 
 
 def test_ssb_query_plans_decode_and_lower():
