@@ -32,6 +32,13 @@ int svm_debug_jit_selftest(char* buf, size_t cap);
  * A kernel of known traffic (read n, write n bytes) to calibrate HBM counters. */
 int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, int src_col);
 
+/* What HBM gives a plain read of a scan's columns: Int32 columns cols[0..ncols) (<= 8)
+ * of table `id` streamed with non-temporal 128-bit loads and nothing else
+ * (svm_read_probe), blocks_per_cu workgroups of `block` threads per CU; *best_ms
+ * receives the fastest of `reps` launches (bytes = rows / 4 * 16 * ncols). */
+int svm_debug_read_probe(const char* id, int ncols, const int* cols, int block, int blocks_per_cu, int reps,
+                         double* best_ms);
+
 /* Inserts the reference's join hash table would drop (BuildHashTable,
  * com/cpp/com.hpp:168-174: Hash of the key bytes % slots, at most 10 linear probes,
  * silent drop) over every row of Int32 column `col` of table `id`, inserted in row

@@ -88,6 +88,7 @@ class OTable:
         if self.bulk0_rows < 0:
             self.bulk0_rows = min(self.rows, self.max_tuple_count) if self.max_tuple_count else self.rows
         self.bulk0_rows = min(self.bulk0_rows, self.rows)
+        self._row0 = 0
         self._bufs = []
         for t, s, c in zip(self.types, self.sizes, self.columns):
             if c is None:
@@ -102,7 +103,23 @@ class OTable:
 
     def ptr(self, c):
         b = self._bufs[c]
-        return None if b is None else b.ctypes.data
+        return None if b is None else b.ctypes.data + self._row0 * int(self.sizes[c])
+
+    def view(self, row0, rows):
+        """Rows [row0, row0+rows) of this table without a copy (a fact slice, as the
+        device's zero-copy views are): reads past its last row see the following
+        rows of this table, as on the device."""
+        if row0 < 0 or row0 + rows > self.rows:
+            raise OracleError("view rows exceed the table")
+        v = OTable.__new__(OTable)
+        v.types, v.sizes = self.types, self.sizes
+        v.columns = [None if c is None else c[row0:row0 + rows] for c in self.columns]
+        v.max_tuple_count = self.max_tuple_count
+        v.rows = rows
+        v.bulk0_rows = min(rows, self.max_tuple_count) if self.max_tuple_count else rows
+        v._bufs = self._bufs
+        v._row0 = self._row0 + row0
+        return v
 
 
 def strings(values, width):

@@ -377,6 +377,44 @@ int svm_debug_copy_column(const char* dst_id, int dst_col, const char* src_id, i
   });
 }
 
+// Debug: the read-only HBM bound of a scan's columns (svm_read_probe): columns
+// cols[0..ncols) of table `id` streamed with non-temporal 128-bit loads at
+// `blocks_per_cu` workgroups of `block` threads per CU; best of `reps` launches.
+int svm_debug_read_probe(const char* id, int ncols, const int* cols, int block, int blocks_per_cu, int reps,
+                         double* best_ms) {
+  return guard([&] {
+    need_id(id);
+    if (!cols || !best_ms || reps < 1) fail(SVM_ERR_INVALID, "bad arguments");
+    Engine& E = engine();
+    TablePtr t = E.store.get(id);
+    std::vector<const void*> ptrs;
+    for (int k = 0; k < ncols; ++k) {
+      if (cols[k] < 0 || cols[k] >= (int)t->cols.size() || t->cols[cols[k]].type != Int32 || !t->cols[cols[k]].dev)
+        fail(SVM_ERR_INVALID, "not a materialised Int32 column");
+      ptrs.push_back(t->col_ptr(cols[k]));
+    }
+    std::lock_guard<std::mutex> g(E.run_mu);
+    DevBuf out(64);
+    hipEvent_t a, b;
+    SVM_HIP(hipEventCreate(&a));
+    SVM_HIP(hipEventCreate(&b));
+    const unsigned grid = (unsigned)(E.num_cus * std::max(1, blocks_per_cu));
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+      SVM_HIP(hipEventRecord(a, E.stream));
+      k::read_probe(ptrs.data(), ncols, t->rows, block, grid, out.p, E.stream);
+      SVM_HIP(hipEventRecord(b, E.stream));
+      SVM_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      SVM_HIP(hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *best_ms = best;
+  });
+}
+
 // Debug: inserts the reference's join hash table drops over every row of an Int32
 // column inserted in row order (slots = 0: HashRate x the table's bulk capacity).
 int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* dropped) {

@@ -106,6 +106,11 @@ struct PadList {
 };
 void zero_pads(char* const* ptrs, int n, hipStream_t s);
 
+// Non-temporal 128-bit reads of rows/4 groups of each of `ncols` (<= 8) int32 columns
+// and nothing else (svm_read_probe), on `grid` workgroups of `block` threads; out: 16
+// bytes of device scratch.
+void read_probe(const void* const* cols, int ncols, uint64_t rows, int block, unsigned grid, void* out,
+                hipStream_t s);
 // Streaming 16-byte copy of `bytes` (a multiple of 16) — counter calibration.
 void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 // Synthetic table rows [row0, row0+n) of `kind` into device columns.

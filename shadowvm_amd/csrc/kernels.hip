@@ -294,6 +294,26 @@ __global__ void svm_calib_copy(const uint4* __restrict__ src, uint4* __restrict_
     dst[i] = src[i];
 }
 
+// Streams `n16` 16-byte groups of each of `ncols` columns with non-temporal 128-bit
+// loads (the access pattern of the scans) and nothing else: what HBM gives a read of
+// those columns at a launch shape.  The XOR of everything read goes to out[0] only in
+// a case that cannot happen for the data here, so no load is optimised away.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+struct ReadCols {
+  const i32x4* c[8];
+};
+__global__ void __launch_bounds__(1024) svm_read_probe(ReadCols cols, int ncols, uint64_t n16, i32x4* out) {
+  i32x4 acc = {0, 0, 0, 0};
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n16; v += stride) {
+    for (int k = 0; k < 8; ++k) {
+      if (k >= ncols) break;
+      acc ^= __builtin_nontemporal_load(cols.c[k] + v);
+    }
+  }
+  if (acc.x == 0x7A3B9C1D && acc.y == 0x5E2F8A47 && acc.z == 0x1C6D3E9B && acc.w == 0x4F8B2A6C) out[0] = acc;
+}
+
 struct GenCols {
   void* c[SVM_GEN_MAX_COLS];
 };
@@ -472,6 +492,19 @@ void zero_pads(char* const* ptrs, int n, hipStream_t s) {
     k_zero_pads<<<(unsigned)l.n, kPadBytes, 0, s>>>(l);
     LAUNCH_CHECK("zero_pads");
   }
+}
+
+void read_probe(const void* const* cols, int ncols, uint64_t rows, int block, unsigned grid, void* out,
+                hipStream_t s) {
+  if (ncols < 1 || ncols > 8) fail(SVM_ERR_INVALID, "read_probe: 1..8 columns");
+  if (block < 64 || block > 1024 || block % 64) fail(SVM_ERR_INVALID, "read_probe: block of 64..1024 threads");
+  ReadCols c;
+  memset(&c, 0, sizeof(c));
+  for (int k = 0; k < ncols; ++k) c.c[k] = (const i32x4*)cols[k];
+  const uint64_t n16 = rows / 4;
+  if (!n16) return;
+  svm_read_probe<<<grid ? grid : 1, block, 0, s>>>(c, ncols, n16, (i32x4*)out);
+  LAUNCH_CHECK("read_probe");
 }
 
 void calib_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {

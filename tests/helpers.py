@@ -83,8 +83,7 @@ def oracle_slice_query(q, tabs, fact, threads=4):
     """The oracle's answer to a batch-template query: its dimension pipelines over
     the dimension tables, then its fact pipeline over its slice of `fact`."""
     vid, base, row0, rows = q.view
-    view = O.OTable(fact.types, fact.sizes, [c[row0:row0 + rows] for c in fact.columns],
-                    max_tuple_count=3_000_000)
+    view = fact.view(row0, rows)
     dims = {}
     for pt, src, refs in q.dims:
         dims[pt.pipe_info["id"]] = O.run_pipeline(pt.plan(src, refs), tabs[src], [], threads=threads)
@@ -92,7 +91,7 @@ def oracle_slice_query(q, tabs, fact, threads=4):
     return O.run_pipeline(fq.plan(src, refs), view, [dims[r].table for r in refs], threads=threads)
 
 
-def oracle_slice_queries(qs, tabs, fact, workers=8):
+def oracle_slice_queries(qs, tabs, fact, workers=8, threads=4):
     """oracle_slice_query for every query, several at once (the plans compile in parallel)."""
     with cf.ThreadPoolExecutor(workers) as ex:
-        return list(ex.map(lambda q: oracle_slice_query(q, tabs, fact), qs))
+        return list(ex.map(lambda q: oracle_slice_query(q, tabs, fact, threads=threads), qs))

@@ -193,3 +193,21 @@ def test_int32_metric_wraps():
     assert got.rows == len(want[0])
     for g, w in zip(got.columns, want):
         assert np.array_equal(g, w)
+
+
+def test_zero_copy_view_equals_copied_slice():
+    """OTable.view (a slice sharing its table's buffers) answers exactly like a copied
+    slice, for a batch-template query over an odd row range."""
+    wl = ssb.four_way(seed=3, fact_rows=200_003, n_cust=20_000, n_supp=2_000, n_part=14_000, prefix="ov")
+    tabs = host_tables(wl, threads=4)
+    q = ssb.batch_queries(wl, n=1, seed=2, window=200_003, min_rows=50_000, prefix="ovq")[0]
+    fact = tabs[wl.fact[1]]
+    _, _, row0, rows = q.view
+    copied = O.OTable(fact.types, fact.sizes, [c[row0:row0 + rows] for c in fact.columns])
+    dims = {pt.pipe_info["id"]: O.run_pipeline(pt.plan(src, refs), tabs[src], []) for pt, src, refs in q.dims}
+    fq, src, refs = q.fact
+    a = O.run_pipeline(fq.plan(src, refs), fact.view(row0, rows), [dims[r].table for r in refs])
+    b = O.run_pipeline(fq.plan(src, refs), copied, [dims[r].table for r in refs])
+    assert a.rows == b.rows
+    for x, y in zip(a.columns, b.columns):
+        assert np.array_equal(x, y)
