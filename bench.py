@@ -58,8 +58,10 @@ def cpu_baseline(wl, sample_rows, threads):
     n = tabs[src].rows
     what = "all" if n == wl.fact_rows else "the first"
     return {"value": n / (best / 1e3), "unit": "fact rows/s", "cores": threads, "kind": "port",
+            "host_cores": os.cpu_count(),
             "sample": f"{what} {n} fact rows of the same table, full dimensions, same query; hash build + "
-                      f"scan + per-bulk reduce + merge (best of 2 runs)"}
+                      f"scan + per-bulk reduce + merge (best of 2 runs) on {threads} threads of a "
+                      f"{os.cpu_count()}-core host"}, r
 
 
 def _pct(xs, p):
@@ -75,7 +77,7 @@ def _same(a, b):
     return a.rows == b.rows and all(np.array_equal(x, y) for x, y in zip(a.columns, b.columns))
 
 
-def bench_batched(wl, nq, seed, reps=3):
+def bench_batched(wl, nq, seed, reps=3, batch_runs=30):
     """64 seeded template queries over the four-way tables (fact slices of 1M..60M
     rows inside the first 60M rows): one svm_batch_run (shared scan) against the
     same queries run one after another through the plain per-query kernels."""
@@ -103,11 +105,15 @@ def bench_batched(wl, nq, seed, reps=3):
 
     run_plain()
     P.batch_run(plans)  # warm: caches and code objects
-    plain_t, batch_t = [], []
+    plain_t, batch_t, lat = [], [], []
     for _ in range(reps):
         t0 = time.perf_counter()
-        lat, pres = run_plain()
+        lt, pres = run_plain()
+        lat += lt
         plain_t.append(time.perf_counter() - t0)
+    # a batch's queries complete together: its latency is the batch's wall time,
+    # sampled over batch_runs submissions
+    for _ in range(batch_runs):
         t0 = time.perf_counter()
         bres, fused = P.batch_run(plans)
         batch_t.append(time.perf_counter() - t0)
@@ -116,12 +122,13 @@ def bench_batched(wl, nq, seed, reps=3):
     log(f"[batch] {nq} queries: plain {tp*1e3:.1f} ms, batched {tb*1e3:.1f} ms ({fused} fused), bit-exact={exact}")
     return {"queries": nq, "fused": fused, "bit_exact_vs_plain": exact,
             "batched_qps": nq / tb, "plain_qps": nq / tp, "speedup": tp / tb,
-            "batched_ms": tb * 1e3, "plain_ms": tp * 1e3,
-            "batched_p50_ms": tb * 1e3, "batched_p99_ms": tb * 1e3,
+            "batched_ms": tb * 1e3, "plain_ms": tp * 1e3, "batch_runs": batch_runs, "plain_runs": reps,
+            "batched_p50_ms": _pct(batch_t, 50) * 1e3, "batched_p99_ms": _pct(batch_t, 99) * 1e3,
             "plain_p50_ms": _pct(lat, 50) * 1e3, "plain_p99_ms": _pct(lat, 99) * 1e3,
             "workload": "seeded template generator: 1-4 joins over the four-way dimensions, selectivity "
                         "~1e-4..0.5, 1..100K groups, fact slices U[1M,60M] rows in the first 60M rows; "
-                        "all queries submitted at once as PipelineInfo bytes (latency = batch completion)"}
+                        "all queries submitted at once as PipelineInfo bytes (batched latency = batch "
+                        "completion, over batch_runs runs; plain latency = each query's own time)"}
 
 
 def _token(i, width):
@@ -158,9 +165,16 @@ def bench_served(wl, rate, seconds, seed, mode, workers, pool):
         t += rng.expovariate(rate)
         arrivals.append(t)
     P.server_start(workers=workers, budget=32 << 30, max_batch=64, mode=mode)
-    # warm pass (untimed): code objects, dimension profiles and autotuned shapes
+    # warm passes (untimed): code objects, dimension profiles and autotuned shapes; each
+    # query once alone (the plain kernels a lone query runs on), then all at once (the
+    # shared scans)
+    nwarm = min(len(qs), 200)
+    log(f"[serve] mode {mode}: warming with {nwarm} queries alone, then together")
+    for i in range(nwarm):
+        tk = P.server_submit(_renamed(plans[i], qs[i].qid, _token(9_000_000 + i, width)))
+        P.server_wait_timed(tk, timeout_ms=-1, want_result=False)
     warm = [P.server_submit(_renamed(plans[i % len(qs)], qs[i % len(qs)].qid, _token(10_000_000 + i, width)))
-            for i in range(min(len(qs), 200))]
+            for i in range(nwarm)]
     for tk in warm:
         P.server_wait_timed(tk, timeout_ms=-1, want_result=False)
     lat, qlat = [0.0] * n, [0.0] * n
@@ -213,14 +227,20 @@ def serve_sweep(wl, rates, seconds, seed, workers_batched, workers_plain, p99_bo
             runs.append(o)
             if o["achieved_qps"] < 0.9 * r or o["p99_ms"] > 10 * p99_bound_ms:
                 break
-        ok = [o["offered_qps"] for o in runs if o["p99_ms"] <= p99_bound_ms and o["achieved_qps"] >= 0.95 * o["offered_qps"]]
-        res["batching" if mode == 0 else "plain"] = {"runs": runs, "workers": workers,
-                                                     "capacity_qps": max(ok) if ok else 0.0}
+        def capacity(bound):
+            ok = [o["offered_qps"] for o in runs if o["p99_ms"] <= bound and o["achieved_qps"] >= 0.95 * o["offered_qps"]]
+            return max(ok) if ok else 0.0
+        res["batching" if mode == 0 else "plain"] = {
+            "runs": runs, "workers": workers, "capacity_qps": capacity(p99_bound_ms),
+            "capacity_at_p99_ms": {str(int(b)): capacity(b) for b in (10.0, 20.0, 50.0)}}
     cb, cp = res["batching"]["capacity_qps"], res["plain"]["capacity_qps"]
     res["p99_bound_ms"] = p99_bound_ms
     res["capacity_ratio"] = cb / cp if cp else None
+    res["capacity_ratio_at_p99_ms"] = {
+        b: (res["batching"]["capacity_at_p99_ms"][b] / res["plain"]["capacity_at_p99_ms"][b]
+            if res["plain"]["capacity_at_p99_ms"][b] else None) for b in ("10", "20", "50")}
     log(f"[serve] capacity at p99 <= {p99_bound_ms:.0f} ms: continuous batching {cb:.0f} q/s, "
-        f"plain {cp:.0f} q/s ({res['capacity_ratio'] or 0:.2f}x)")
+        f"plain {cp:.0f} q/s ({res['capacity_ratio'] or 0:.2f}x); by bound {res['capacity_ratio_at_p99_ms']}")
     return res
 
 
@@ -280,11 +300,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
     ap.add_argument("--batch-queries", type=int, default=64)
-    ap.add_argument("--serve-rates", default="500,1000,2000,3000,4000,6000,8000,12000,16000",
+    ap.add_argument("--serve-rates", default="50,200,500,1000,2000,3000,4000,6000,8000,12000,16000",
                     help="offered query rates (q/s), swept upward per mode")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-p99-ms", type=float, default=50.0, help="latency bound of the capacity comparison")
-    ap.add_argument("--serve-workers", default="2,4", help="worker streams: continuous batching, plain")
+    ap.add_argument("--serve-workers", default="3,4", help="worker streams: continuous batching, plain")
+    ap.add_argument("--hybrid-ratio", type=int, default=3,
+                    help="also time the query with this many of every 10 bulks on host cores (0: skip)")
     ap.add_argument("--load-rows", type=int, default=4_000_000, help="Loader benchmark size (0: skip)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -344,6 +366,47 @@ def main():
         _lib.check(L.svm_debug_copy_column(b"calib_dst", 0, wl.fact[1].encode(), 4))
         log(f"[calib] svm_calib_copy moved {rows * 4} bytes each way")
 
+    # what HBM gives a plain read of the scan's six columns (svm_read_probe) at the best
+    # shape of a small sweep: the achievable bound beside the 8 TB/s peak
+    read_probe = None
+    if rows == 600_000_000:
+        import ctypes
+        from shadowvm_amd import _lib
+        L = ctypes.CDLL(_lib.LIB_PATH)
+        L.svm_debug_read_probe.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        cols = (ctypes.c_int * 6)(0, 1, 2, 3, 4, 5)
+        shapes = []
+        for block, bpc in ((256, 1), (256, 2), (512, 1), (512, 2), (1024, 1), (1024, 3)):
+            ms = ctypes.c_double()
+            _lib.check(L.svm_debug_read_probe(wl.fact[1].encode(), 6, cols, block, bpc, 5, ctypes.byref(ms)))
+            shapes.append({"block": block, "blocks_per_cu": bpc, "ms": ms.value,
+                           "gb_per_s": (rows // 4) * 16 * 6 / (ms.value / 1e3) / 1e9})
+        best = max(shapes, key=lambda x: x["gb_per_s"])
+        read_probe = {"kernel": "svm_read_probe", "bytes": (rows // 4) * 16 * 6, "best": best, "shapes": shapes,
+                      "scan_frac_of_read": achieved / best["gb_per_s"]}
+        log(f"[probe] plain read of the six columns: {best['gb_per_s']:.0f} GB/s at {best['block']} threads x "
+            f"{best['blocks_per_cu']}/CU ({best['ms']:.3f} ms); svm_scan at {achieved / best['gb_per_s']:.1%} of it")
+
+    # hybrid execution: `hybrid_ratio` of every 10 bulks on host cores, the rest on the
+    # GPU at once, merged (after one untimed run that mirrors the columns to the host)
+    hybrid = None
+    if args.hybrid_ratio > 0:
+        threads = min(16, os.cpu_count() or 1)
+        P.set_hybrid(True, args.hybrid_ratio, threads)
+        try:
+            step()
+            t0 = time.perf_counter()
+            rh = step()
+            hms = (time.perf_counter() - t0) * 1e3
+        finally:
+            P.set_hybrid(False)
+        hybrid = {"cpu_ratio": args.hybrid_ratio, "cpu_threads": threads, "ms": hms,
+                  "fact_rows_per_s": rows / (hms / 1e3), "bit_exact_vs_gpu": _same(rh, r),
+                  "vs_gpu_only": ms_per_step / hms}
+        log(f"[hybrid] {args.hybrid_ratio}/10 bulks on {threads} host threads: {hms:.1f} ms per query "
+            f"({rows / (hms / 1e3):.3e} fact rows/s), bit-exact vs GPU: {hybrid['bit_exact_vs_gpu']}")
+
     batched = None if args.no_batch else bench_batched(wl, args.batch_queries, args.seed)
     served = None
     if not args.no_serve:
@@ -360,8 +423,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, min(args.cpu_sample_rows, rows), threads)
-        log(f"[cpu] {cpu['value']:.3e} fact rows/s on {threads} threads")
+        cpu, cres = cpu_baseline(wl, min(args.cpu_sample_rows, rows), threads)
+        # the GPU's answer equals the CPU path's (when the baseline scanned the whole table)
+        if cpu["sample"].startswith("all"):
+            import numpy as np
+            cpu["bit_exact_vs_gpu"] = r.rows == cres.rows and all(
+                np.array_equal(np.asarray(g), np.asarray(w)) for g, w in zip(r.columns, cres.columns))
+        log(f"[cpu] {cpu['value']:.3e} fact rows/s on {threads} threads; bit-exact vs GPU: "
+            f"{cpu.get('bit_exact_vs_gpu')}")
     P.shutdown()
 
     # HBM bytes per svm_scan launch from the committed FETCH_SIZE / WRITE_SIZE counter
@@ -391,6 +460,9 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "svm_scan", "kernel_ms": scan_avg},
         "cpu_baseline": cpu,
+        "bit_exact_vs_cpu": cpu.get("bit_exact_vs_gpu") if cpu else None,
+        "read_probe": read_probe,
+        "hybrid": hybrid,
         "batched": batched,
         "served": served,
         "loaded": loaded,

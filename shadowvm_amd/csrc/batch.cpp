@@ -1417,6 +1417,8 @@ struct Ticket {
   FactLowered fact;
   bool fused = false;
   uint64_t need = 0;
+  uint64_t rows = 0;   // fact rows the query scans
+  bool light = true;   // served in the light lane (not is_heavy)
   bool done = false;
   int status = SVM_OK;
   std::string error;
@@ -1427,7 +1429,10 @@ struct Ticket {
 struct Server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<std::shared_ptr<Ticket>> pending;
+  // waiting queries: the light lane (small fact slices) and the heavy lane (large
+  // scans); in mode 1 every query waits in `light`, served in arrival order
+  std::deque<std::shared_ptr<Ticket>> light, heavy;
+  int nworkers = 1;
   std::map<uint64_t, std::shared_ptr<Ticket>> tickets;  // submitted and not yet claimed by a wait
   uint64_t bytes_in_use = 0, budget = 0, running = 0, completed = 0;
   int max_batch = 64, mode = 0;
@@ -1446,7 +1451,119 @@ std::vector<std::shared_ptr<Server>> g_stopped;
 std::mutex g_server_mu;
 // Ticket numbers are unique in the process, across servers.
 std::atomic<uint64_t> g_next_ticket{1};
-constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a query on the plain path
+constexpr uint64_t kPlainNeed = 64ull << 20;  // HBM assumed for a plain query whose plans say too little
+// A query is heavy when it scans more than half of its fact table and at least
+// kHeavyMinRows rows (the serving mix's whole-table star joins); heavy queries wait in
+// their own lane, so light ones never queue behind a whole-table pass.
+constexpr uint64_t kHeavyMinRows = 4ull << 20;
+bool is_heavy(uint64_t rows, uint64_t table_rows) { return rows >= kHeavyMinRows && rows > table_rows / 2; }
+// Light queries waiting from which a heavy pass takes them along.
+constexpr size_t kLightBacklog = 4;
+
+// Whether one shared scan over `batch` beats running its queries one by one on the
+// plain kernels, by a cost model fitted to measured passes on the 600M-row tables:
+// a pass costs ~4.1 ps per row of the union of the slices (the issue-bound shared
+// scan) + ~0.45 ps per slice row (each query's share) + ~0.2 ms (profile masks,
+// emission, synchronisation); a plain query ~3.7 ps per slice row (the fused scan at
+// its HBM rate) + ~0.15 ms (its dimension select, map builds, emission).  Queries
+// whose slices barely overlap are cheaper alone.  A fused pass answers all its
+// queries at its end, while queries run one by one finish in turn, so a pass is
+// chosen only for a clear gain (under 0.7 of the plain time): latency stays the plain
+// kernels' until sharing the scan saves real work.
+bool fusing_pays(const std::vector<std::shared_ptr<Ticket>>& batch) {
+  if (batch.size() < 2) return false;
+  std::vector<std::pair<long long, long long>> r;
+  double sum = 0;
+  for (auto& t : batch) {
+    r.push_back({t->fact.row0, t->fact.row1});
+    sum += (double)(t->fact.row1 - t->fact.row0);
+  }
+  std::sort(r.begin(), r.end());
+  double uni = 0;
+  long long lo = r[0].first, hi = r[0].second;
+  for (auto& x : r) {
+    if (x.first > hi) {
+      uni += (double)(hi - lo);
+      lo = x.first;
+      hi = x.second;
+    } else {
+      hi = std::max(hi, x.second);
+    }
+  }
+  uni += (double)(hi - lo);
+  const double fused = 4.1e-12 * uni + 0.45e-12 * sum + 0.2e-3;
+  const double plain = 3.7e-12 * sum + 0.15e-3 * (double)batch.size();
+  return fused < 0.7 * plain;
+}
+
+// HBM a query on the plain path holds while it runs, estimated from its plans and
+// the resident tables' cached key ranges and dictionaries: each dimension pipeline's
+// output (its source's rows x the projected widths), each join's dense map (4 bytes
+// per key of the dimension key column's range, plus existence bits; a key column too
+// wide for a map probes the reference's hash table, 5 slots x 4 bytes per row), and
+// the group table (8 bytes per group and metric over the product of the group key
+// columns' distinct values, plus flag bits; beyond 2^27 groups the sorted entries,
+// 8 + 8 x metrics bytes per row for at most 2^24 rows before they are resized).
+uint64_t plain_need(const std::vector<Job>& jobs, Engine& E, hipStream_t s) {
+  try {
+    std::map<std::string, size_t> made;
+    for (size_t i = 0; i + 1 < jobs.size(); ++i) made[jobs[i].info.id] = i;
+    uint64_t need = 4096;
+    // the column a ref object's column `col` holds: (resident table, its column)
+    auto origin = [&](const std::string& id, int col) -> std::pair<TablePtr, int> {
+      auto it = made.find(id);
+      if (it == made.end()) return {E.store.find(id), col};
+      const Job& d = jobs[it->second];
+      if (d.info.pre_pipeline_id.empty() || col < 0 || col >= (int)d.plan.select_exp.size()) return {nullptr, -1};
+      int obj = -1, c = -1;
+      if (!parse_operand(d.plan.select_exp[col], obj, c) || obj != 0) return {nullptr, -1};
+      return {E.store.find(d.info.pre_pipeline_id[0]), c};
+    };
+    for (size_t i = 0; i + 1 < jobs.size(); ++i) {
+      TablePtr src = jobs[i].info.pre_pipeline_id.empty() ? nullptr : E.store.find(jobs[i].info.pre_pipeline_id[0]);
+      uint64_t width = 0;
+      for (auto& c : jobs[i].plan.output_info.column_list) width += (uint64_t)std::max<int64_t>(c.val_size, 4);
+      need += (src ? src->rows : 0) * width;
+    }
+    const Job& f = jobs.back();
+    TablePtr fsrc = f.info.pre_pipeline_id.empty() ? nullptr : E.store.find(f.info.pre_pipeline_id[0]);
+    if (!fsrc) return kPlainNeed;
+    const uint64_t rows = fsrc->rows;
+    for (size_t j = 0; j < f.plan.join_rkey.size(); ++j) {
+      const KeyIndex R = f.plan.join_rkey[j];
+      if (R.obj < 1 || R.obj >= (int)f.info.pre_pipeline_id.size()) return kPlainNeed;
+      auto o = origin(f.info.pre_pipeline_id[R.obj], R.col);
+      if (!o.first || o.second < 0 || o.second >= (int)o.first->cols.size()) return kPlainNeed;
+      uint64_t range = 0;
+      if (o.first->cols[o.second].type == Int32 && o.first->rows) {
+        KeyEntry ke = table_key(o.first, o.second, s);
+        range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
+      }
+      need += range && range <= (1ull << 28) ? range * 4 + range / 8 : o.first->rows * 20;
+    }
+    uint64_t G = 1;
+    for (const KeyIndex& k : f.plan.reduce_key) {
+      uint64_t card = 0;
+      if (k.obj == 0) {
+        if (k.col < 0 || k.col >= (int)fsrc->cols.size() || fsrc->cols[k.col].type != Int32) return kPlainNeed;
+        KeyEntry ke = table_key(fsrc, k.col, s);
+        card = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
+      } else {
+        if (k.obj >= (int)f.info.pre_pipeline_id.size()) return kPlainNeed;
+        auto o = origin(f.info.pre_pipeline_id[k.obj], k.col);
+        if (!o.first || o.second < 0 || o.second >= (int)o.first->cols.size()) return kPlainNeed;
+        card = table_dict(o.first, o.second, s)->n;
+      }
+      G = card && G > (1ull << 40) / card ? (1ull << 40) : G * std::max<uint64_t>(card, 1);
+    }
+    const uint64_t M = std::max<size_t>(f.plan.metric_exp.size(), 1);
+    if (G <= (1ull << 27)) need += G * M * 8 + G / 8 + 64;
+    else need += std::min<uint64_t>(rows, 1ull << 24) * (8 + 8 * M);
+    return need;
+  } catch (const Error&) {
+    return kPlainNeed;
+  }
+}
 
 std::shared_ptr<Server> running_server() {
   std::lock_guard<std::mutex> g(g_server_mu);
@@ -1515,25 +1632,49 @@ void worker_main(Server* S, int index) {
   for (;;) {
     std::vector<std::shared_ptr<Ticket>> batch;
     uint64_t need = 0;
+    bool alone = true;
     {
+      // Worker 0 of a batching server with two or more workers serves the light lane
+      // only, so light queries never wait behind a heavy pass on every worker; the
+      // others take heavy queries first.
+      const bool light_only = S->mode == 0 && S->nworkers >= 2 && index == 0;
+      auto fits = [&](const std::deque<std::shared_ptr<Ticket>>& q) {
+        return !q.empty() && S->bytes_in_use + q.front()->need <= S->budget;
+      };
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv_work.wait(lk, [&] {
-        if (S->stop) return true;
-        return !S->pending.empty() && S->bytes_in_use + S->pending.front()->need <= S->budget;
-      });
+      S->cv_work.wait(lk, [&] { return S->stop || fits(S->light) || (!light_only && fits(S->heavy)); });
       if (S->stop) break;
-      // admit in arrival order while memory and the batch size allow
+      const bool heavy_lane = !light_only && fits(S->heavy);
+      // candidates in arrival order while memory and the batch size allow (a batch is
+      // one shared scan over one fact table); they run as one pass when sharing the
+      // scan pays, else the oldest runs alone on the plain kernels and the rest wait
+      // for the next free worker
       const int cap = S->mode == 1 ? 1 : S->max_batch;
-      while (!S->pending.empty() && (int)batch.size() < cap &&
-             S->bytes_in_use + need + S->pending.front()->need <= S->budget) {
-        auto t = S->pending.front();
-        if (S->mode == 0 && !batch.empty() && (!t->fused || !batch[0]->fused ||
-                                               t->fact.base != batch[0]->fact.base))
-          break;  // a batch is one shared scan over one fact table
-        need += t->need;
-        batch.push_back(t);
-        S->pending.pop_front();
-        if (!t->fused) break;
+      auto take = [&](std::deque<std::shared_ptr<Ticket>>& q) {
+        while (!q.empty() && (int)batch.size() < cap && S->bytes_in_use + need + q.front()->need <= S->budget) {
+          auto t = q.front();
+          if (!batch.empty() && (!t->fused || !batch[0]->fused || t->fact.base != batch[0]->fact.base)) break;
+          need += t->need;
+          batch.push_back(t);
+          q.pop_front();
+          if (!t->fused) break;
+        }
+      };
+      take(heavy_lane ? S->heavy : S->light);
+      // a heavy pass reads the whole table anyway: when the light lane has a backlog,
+      // waiting light queries ride along for their share of the pass
+      if (heavy_lane && S->mode == 0 && batch[0]->fused && S->light.size() >= kLightBacklog) take(S->light);
+      alone = S->mode == 1 || !batch[0]->fused || !fusing_pays(batch);
+      if (alone && batch.size() > 1) {
+        // back to the front of their lanes, in arrival order
+        while (batch.size() > 1) {
+          need -= batch.back()->need;
+          (batch.back()->light ? S->light : S->heavy).push_front(batch.back());
+          batch.pop_back();
+        }
+        // the queries put back are another worker's to take (every worker re-checks: a
+        // light-lane worker cannot take a heavy query)
+        S->cv_work.notify_all();
       }
       S->bytes_in_use += need;
       S->running += batch.size();
@@ -1541,7 +1682,7 @@ void worker_main(Server* S, int index) {
       for (auto& t : batch) t->admitted = now;
     }
     try {
-      if (batch.size() == 1 && (!batch[0]->fused || S->mode == 1)) {
+      if (alone) {
         run_plain_query(*batch[0], s);
       } else {
         std::vector<FactLowered*> fq;
@@ -1591,6 +1732,7 @@ int server_start(int workers, uint64_t budget, int max_batch, int mode) {
   S->budget = budget ? budget : E.cfg.hbm_budget;
   S->max_batch = max_batch > 0 ? max_batch : 64;
   S->mode = mode;
+  S->nworkers = workers;
   SVM_HIP(hipStreamCreateWithFlags(&S->submit_stream, hipStreamNonBlocking));
   Server* raw = S.get();
   for (int w = 0; w < workers; ++w) S->workers.emplace_back(worker_main, raw, w);
@@ -1634,17 +1776,29 @@ int server_submit(const uint8_t* const* plans, const size_t* lens, int n, uint64
     // every dimension pipeline of the query must be absorbed by the fused fact pipeline
     t->fused = t->fact.ok && t->fact.deferred.size() == (size_t)(n - 1);
   }
-  t->need = t->fused ? fused_bytes(t->fact) : kPlainNeed;
+  uint64_t table_rows = 0;
+  if (t->fused) {
+    t->rows = (uint64_t)(t->fact.row1 - t->fact.row0);
+    table_rows = t->fact.base ? t->fact.base->rows : t->rows;
+  } else if (!t->jobs.back().info.pre_pipeline_id.empty()) {
+    TablePtr src = E.store.find(t->jobs.back().info.pre_pipeline_id[0]);
+    t->rows = src ? src->rows : 0;
+    table_rows = src ? (src->base ? src->base->rows : src->rows) : 0;
+  }
+  t->light = S->mode == 1 || !is_heavy(t->rows, table_rows);
+  t->need = t->fused ? fused_bytes(t->fact) : plain_need(t->jobs, E, S->submit_stream);
   if (t->need > S->budget) fail(SVM_ERR_BUSY, "query needs more HBM than the serving budget");
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) fail(SVM_ERR_STATE, "server stopping");
     t->submitted = std::chrono::steady_clock::now();
     S->tickets[t->id] = t;
-    S->pending.push_back(t);
+    (t->light ? S->light : S->heavy).push_back(t);
     if (ticket) *ticket = t->id;
   }
-  S->cv_work.notify_one();
+  // every worker re-checks its lanes: waking one that cannot serve this query's lane
+  // (the light-only worker, for a heavy query) would leave the query waiting
+  S->cv_work.notify_all();
   return SVM_OK;
 }
 
@@ -1684,7 +1838,7 @@ int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* 
   std::shared_ptr<Server> S = running_server();
   if (!S) fail(SVM_ERR_STATE, "server not running");
   std::lock_guard<std::mutex> lk(S->mu);
-  if (queued) *queued = S->pending.size();
+  if (queued) *queued = S->light.size() + S->heavy.size();
   if (running) *running = S->running;
   if (done) *done = S->completed;
   if (bytes) *bytes = S->bytes_in_use;
@@ -1712,13 +1866,15 @@ int server_stop() {
     // before the submission stream goes away
     std::unique_lock<std::mutex> lk(S->mu);
     const auto now = std::chrono::steady_clock::now();
-    for (auto& t : S->pending) {
-      t->status = SVM_ERR_STATE;
-      t->error = "server stopped before the query was admitted";
-      t->finished = now;
-      t->done = true;
+    for (auto* q : {&S->light, &S->heavy}) {
+      for (auto& t : *q) {
+        t->status = SVM_ERR_STATE;
+        t->error = "server stopped before the query was admitted";
+        t->finished = now;
+        t->done = true;
+      }
+      q->clear();
     }
-    S->pending.clear();
     S->cv_done.notify_all();
     S->cv_done.wait(lk, [&] { return S->submitting == 0; });
   }

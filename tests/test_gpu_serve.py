@@ -92,3 +92,23 @@ def test_query_larger_than_budget_is_refused(serve_tables):
         assert e.value.code == P._lib.ERR_BUSY
     finally:
         P.server_stop()
+
+
+def test_sequential_submissions_reach_a_worker_that_serves_their_lane(serve_tables):
+    """One query at a time (each waited for before the next), heavy and light, to a
+    batching server whose worker 0 serves the light lane only: every query is taken by
+    a worker of its lane (a wake-up that reached only the light-lane worker would leave
+    a heavy query waiting; the bounded wait turns that into a failure)."""
+    wl = serve_tables
+    qs = ssb.serve_queries(wl, 12, seed=8, light_frac=0.5, light_rows=4_000_000, prefix="sw")
+    ssb.make_views(qs)
+    assert any(q.shape["kind"] == "heavy" for q in qs) and any(q.shape["kind"] == "light" for q in qs)
+    want = _serve(qs, mode=1)
+    P.server_start(workers=3, budget=0, max_batch=64, mode=0)
+    try:
+        for q, w in zip(qs, want):
+            got = P.server_wait(P.server_submit(q.plans()), timeout_ms=20_000)
+            assert got.rows == w.rows
+            assert_same_columns(got.columns, w.columns)
+    finally:
+        P.server_stop()
