@@ -377,7 +377,7 @@ def main():
                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         cols = (ctypes.c_int * 6)(0, 1, 2, 3, 4, 5)
         shapes = []
-        for block, bpc in ((256, 1), (256, 2), (512, 1), (512, 2), (1024, 1), (1024, 3)):
+        for block, bpc in ((256, 1), (256, 2), (256, 4), (512, 1), (512, 2), (1024, 1), (1024, 2), (1024, 3)):
             ms = ctypes.c_double()
             _lib.check(L.svm_debug_read_probe(wl.fact[1].encode(), 6, cols, block, bpc, 5, ctypes.byref(ms)))
             shapes.append({"block": block, "blocks_per_cu": bpc, "ms": ms.value,

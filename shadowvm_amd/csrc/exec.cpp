@@ -478,13 +478,35 @@ std::map<std::pair<int, int>, int> assign_dcols(const std::set<std::pair<int, in
   return slots;
 }
 
-unsigned scan_grid(Engine& E, uint64_t vec, int block, int unroll, uint64_t lds_bytes, int bpc) {
+// Workgroups of the persistent scan grid: per CU, `bpc` when given, else as many as LDS
+// allows (at most 8), never more than the CU keeps resident at once for this kernel
+// (its registers, LDS and workgroup size): a grid larger than that runs its last
+// workgroups as a second, partial wave, which streams slower than one full wave.
+std::mutex g_occ_mu;
+std::map<std::pair<hipFunction_t, uint64_t>, int> g_occ;
+int resident_blocks(hipFunction_t fn, int block, uint64_t lds_bytes) {
+  std::lock_guard<std::mutex> g(g_occ_mu);
+  const auto key = std::make_pair(fn, ((uint64_t)block << 32) | lds_bytes);
+  auto it = g_occ.find(key);
+  if (it != g_occ.end()) return it->second;
+  int n = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, (size_t)lds_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  g_occ[key] = n;
+  return n;
+}
+
+unsigned scan_grid(Engine& E, hipFunction_t fn, uint64_t vec, int block, int unroll, uint64_t lds_bytes, int bpc) {
   if (bpc <= 0) {
     uint64_t lds_cu = 160 * 1024;
     bpc = lds_bytes ? (int)(lds_cu / (lds_bytes + 512)) : 8;
     if (bpc > 8) bpc = 8;
     if (bpc < 1) bpc = 1;
   }
+  const int resident = fn ? resident_blocks(fn, block, lds_bytes) : 0;
+  if (resident > 0 && bpc > resident) bpc = resident;
   uint64_t need = (vec + (uint64_t)block * unroll - 1) / ((uint64_t)block * unroll);
   uint64_t cap = (uint64_t)E.num_cus * bpc;
   uint64_t g = need < cap ? need : cap;
@@ -523,7 +545,9 @@ const TuneChoice kTuneCandidates[] = {
     // few waves, a scan with dependent map lookups needs more to hide them
     {512, 1, 2, true, false, false},  {512, 2, 2, true, false, false},
     // the flat form with the next groups' loads in flight during this group's lookups
-    {1024, 1, 0, true, true, true},   {512, 1, 0, true, true, true},    {1024, 2, 0, true, false, true}};
+    {1024, 1, 0, true, true, true},   {512, 1, 0, true, true, true},    {1024, 2, 0, true, false, true},
+    // 1,024 threads, one or two workgroups per CU
+    {1024, 1, 1, true, false, true},  {1024, 1, 2, true, false, true}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -846,7 +870,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         uint64_t w2 = 0;
         shape_for(cand, k2, w2);
         hipFunction_t f2 = jit_function(gen_scan_source(k2, "svm_scan_tune"), "svm_scan_tune");
-        unsigned g2 = scan_grid(c.E, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
+        unsigned g2 = scan_grid(c.E, f2, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
         clear_outputs();
         Event ta, tb;
         SVM_HIP(hipEventRecord(ta.e, c.s));
@@ -928,7 +952,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   // every workgroup of every launch.
   DevBuf ekeys, evals;
   uint64_t ecap = 0, nentries = 0;
-  const unsigned grid = run_scan ? scan_grid(c.E, vec, ks.block, ks.unroll, run_words * 4, choice.bpc) : 0;
+  const unsigned grid = run_scan ? scan_grid(c.E, fn, vec, ks.block, ks.unroll, run_words * 4, choice.bpc) : 0;
   auto size_entries = [&](uint64_t cap) {
     ecap = cap;
     ekeys = DevBuf(cap * 8 + 8, c.s);
