@@ -13,11 +13,16 @@
 //   * select pipelines compact passing rows in row order with wave64 ballots.
 #include "codegen.h"
 
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 
 #include "kargs.h"
 
 namespace svm {
+
+uint64_t source_hash(const std::string& s);
 
 // The runtime the generated code calls (Hash / StrHash / StrCmp / StrCopy / Probe of
 // com/cpp/cpu-runtime.hpp:37-152 and the packed-map lookup).  The same text is
@@ -513,6 +518,73 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
   }
   o << "}\n";
   return o.str();
+}
+
+// ------------------------------------------------------------------ source memo
+// Generating a kernel's source text takes ~20 us; a pipeline shape's text is the
+// same every run (constants travel as launch parameters), so it is memoised under a
+// compact description of the shape, with its hash.
+namespace {
+
+// Every field of the shape, strings length-prefixed, so that two shapes share a key
+// only when they are equal.
+void put(std::ostringstream& k, const std::string& v) { k << v.size() << ':' << v; }
+
+std::string shape_key(const KernelShape& s, char kind, const char* entry) {
+  static_assert(sizeof(KernelShape) == 304 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
+                "a field was added to the kernel shape: add it to shape_key");
+  std::ostringstream k;
+  k << kind << entry << '|' << s.block << ',' << s.unroll << ',' << s.prefetch << s.flat << ',' << s.agg << ','
+    << s.hash_bits << ',' << s.hash_probes << '|';
+  put(k, s.filter);
+  for (int c : s.fact_i32) k << c << ',';
+  k << '|';
+  for (int c : s.fact_other) k << c << ',';
+  k << '|';
+  for (auto& J : s.joins)
+    k << J.robj << ',' << J.lcol << ',' << J.payload << ',' << J.wbits << ',' << J.lds << J.defer << ',' << J.kind << ','
+      << J.rcol << ',' << J.ktype << ',' << J.ksize << ';';
+  k << '|';
+  for (auto& kv : s.dcol_slot) k << kv.first.first << '.' << kv.first.second << '=' << kv.second << ',';
+  k << '|';
+  for (auto& K : s.keys) k << K.src << ',' << K.join << ',' << K.obj << ',' << K.col << ';';
+  k << '|' << s.metrics.size() << ':';
+  for (auto& m : s.metrics) put(k, m);
+  k << '|' << s.metric_ctype.size() << ':';
+  for (auto& m : s.metric_ctype) put(k, m);
+  k << '|' << s.proj.size() << ':';
+  for (auto& q : s.proj) put(k, q);
+  k << '|';
+  for (auto& t : s.proj_type) k << t.first << ',' << t.second << ';';
+  return k.str();
+}
+
+std::mutex g_memo_mu;
+std::map<std::string, std::shared_ptr<const GeneratedSource>> g_memo;
+
+template <typename Gen>
+std::shared_ptr<const GeneratedSource> memo(const std::string& key, Gen&& gen) {
+  {
+    std::lock_guard<std::mutex> g(g_memo_mu);
+    auto it = g_memo.find(key);
+    if (it != g_memo.end()) return it->second;
+  }
+  auto src = std::make_shared<GeneratedSource>();
+  src->text = gen();
+  src->hash = source_hash(src->text);
+  std::lock_guard<std::mutex> g(g_memo_mu);
+  if (g_memo.size() > 4096) g_memo.clear();  // bounded: shapes are few in practice
+  return g_memo.emplace(key, src).first->second;
+}
+
+}  // namespace
+
+std::shared_ptr<const GeneratedSource> scan_source(const KernelShape& s, const char* entry) {
+  return memo(shape_key(s, 'r', entry), [&] { return gen_scan_source(s, entry); });
+}
+
+std::shared_ptr<const GeneratedSource> select_source(const KernelShape& s) {
+  return memo(shape_key(s, 's', ""), [&] { return gen_select_source(s); });
 }
 
 std::string gen_host_source(const KernelShape& s0) {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,16 @@ struct KernelShape {
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs
 };
+
+// Generated source text and its source_hash.
+struct GeneratedSource {
+  std::string text;
+  uint64_t hash = 0;
+};
+// gen_scan_source / gen_select_source memoised per kernel shape (the text of a shape
+// never changes; constants are launch parameters).
+std::shared_ptr<const GeneratedSource> scan_source(const KernelShape& s, const char* entry = "svm_scan");
+std::shared_ptr<const GeneratedSource> select_source(const KernelShape& s);
 
 // entry: the kernel's symbol (the autotuner's trial launches use "svm_scan_tune",
 // so that profiles tell them from the query's own scans)

@@ -844,14 +844,14 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   } else if (opt.noalloc) {
     // no trials: the configuration an earlier run of this shape chose, else the default
     if (run_scan && autotune_enabled() && opt.block == 256 && opt.unroll == 1) {
-      const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
+      const uint64_t tkey = scan_source(ks)->hash ^ (uint64_t)(63 - __builtin_clzll(rows));
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune.find(tkey);
       if (it != g_tune.end() && it->second.done) choice = it->second.best;
     }
   } else if (run_scan && !wide && autotune_enabled() && rows >= tune_min_rows() && opt.block == 256 &&
              opt.unroll == 1) {
-    const uint64_t tkey = source_hash(gen_scan_source(ks)) ^ (uint64_t)(63 - __builtin_clzll(rows));
+    const uint64_t tkey = scan_source(ks)->hash ^ (uint64_t)(63 - __builtin_clzll(rows));
     std::unique_lock<std::mutex> lk(g_tune_mu);
     TuneState& st = g_tune[tkey];
     if (!st.done) {
@@ -869,7 +869,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         KernelShape k2;
         uint64_t w2 = 0;
         shape_for(cand, k2, w2);
-        hipFunction_t f2 = jit_function(gen_scan_source(k2, "svm_scan_tune"), "svm_scan_tune");
+        const auto g2src = scan_source(k2, "svm_scan_tune");
+        hipFunction_t f2 = jit_function(g2src->text, g2src->hash, "svm_scan_tune");
         unsigned g2 = scan_grid(c.E, f2, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
         clear_outputs();
         Event ta, tb;
@@ -920,7 +921,10 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
               (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
   hipFunction_t fn = nullptr;
-  if (run_scan) fn = jit_function(gen_scan_source(ks), "svm_scan");
+  if (run_scan) {
+    const auto gsrc = scan_source(ks);
+    fn = jit_function(gsrc->text, gsrc->hash, "svm_scan");
+  }
   const double t_launch_host = host_ms(t_host0);
   ev1.record(c.s);
   // Placement of the source's bulks: the host engine's (all of them for isGPU =
@@ -1420,7 +1424,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   fill_common_args(c, a, joins, ks.dcol_slot, lits, blob);
   ev1.record(c.s);
   uint64_t total = 0;
-  std::string src;
+  std::shared_ptr<const GeneratedSource> src;
   const HostSlots slots = host_slots(c.s);
   TablePtr out;
   // projected columns into the kernel arguments; a bare source column projected
@@ -1438,8 +1442,8 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     }
   };
   if (single) {
-    src = gen_select_source(ks);
-    hipFunction_t fsingle = jit_function(src, "svm_sel_single");
+    src = select_source(ks);
+    hipFunction_t fsingle = jit_function(src->text, src->hash, "svm_sel_single");
     out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
     bind_outputs();
     const ScanState st = scan_state(c.s, tiles);
@@ -1461,16 +1465,16 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     // count, scan and write back to back into an output sized for every source row,
     // then one synchronisation for the row count (the write's last tile ends the
     // output with its padding)
-    src = gen_select_source(ks);
+    src = select_source(ks);
     DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
     a.out[6] = offs.p;
     a.out[7] = counts.p;
     out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
     bind_outputs();
-    launch(jit_function(src, "svm_sel_count"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    launch(jit_function(src->text, src->hash, "svm_sel_count"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
                           offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
-    launch(jit_function(src, "svm_sel_write"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
+    launch(jit_function(src->text, src->hash, "svm_sel_write"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     SVM_HIP(hipStreamSynchronize(c.s));
     total = slots.host[0];
     out->rows = total;
@@ -1480,8 +1484,8 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     a.out[6] = offs.p;
     a.out[7] = counts.p;
     if (tiles) {
-      src = gen_select_source(ks);
-      hipFunction_t fcount = jit_function(src, "svm_sel_count");
+      src = select_source(ks);
+      hipFunction_t fcount = jit_function(src->text, src->hash, "svm_sel_count");
       launch(fcount, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
       k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
                             offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
@@ -1501,7 +1505,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
         a0.rows = (long long)src_bulk0;
         a0.out[6] = offs0.p;
         a0.out[7] = counts0.p;
-        launch(jit_function(src, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4),
+        launch(jit_function(src->text, src->hash, "svm_sel_count"), (unsigned)tiles0, (unsigned)ks.block, (unsigned)(lds_words * 4),
                c.s, a0);
         k::exclusive_scan_u32(counts0.as<uint32_t>(), tiles0, offs0.as<unsigned long long>(),
                               offs0.as<unsigned long long>() + tiles0, c.s, slots.dev + 1);
@@ -1513,7 +1517,7 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
     out->bulk0 = out_bulk0;
     bind_outputs();
     if (tiles && total) {
-      hipFunction_t fwrite = jit_function(src, "svm_sel_write");
+      hipFunction_t fwrite = jit_function(src->text, src->hash, "svm_sel_write");
       launch(fwrite, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     }
   }

@@ -187,7 +187,10 @@ std::string jit_code_object(const std::string& src) {
 }
 
 hipFunction_t jit_function(const std::string& src, const char* name) {
-  uint64_t h = source_hash(src);
+  return jit_function(src, source_hash(src), name);
+}
+
+hipFunction_t jit_function(const std::string& src, uint64_t h, const char* name) {
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_mods.find(h);
   if (it != g_mods.end() && it->second.src != src) it = g_mods.end();  // a 64-bit hash collision

@@ -9,6 +9,24 @@
 
 namespace svm {
 
+namespace {
+// The freshly generated source, checked against the memoised one for the same shape
+// (the shapes below differ from each other in one field at a time, so a memo key that
+// missed a field would return another shape's text here).
+std::string scan_checked(const KernelShape& s) {
+  std::string t = gen_scan_source(s);
+  if (scan_source(s)->text != t || scan_source(s)->hash != source_hash(t))
+    fail(SVM_ERR_JIT, "memoised scan source differs from the generated one");
+  return t;
+}
+std::string select_checked(const KernelShape& s) {
+  std::string t = gen_select_source(s);
+  if (select_source(s)->text != t || select_source(s)->hash != source_hash(t))
+    fail(SVM_ERR_JIT, "memoised select source differs from the generated one");
+  return t;
+}
+}  // namespace
+
 std::vector<std::pair<std::string, std::string>> selftest_sources() {
   std::vector<std::pair<std::string, std::string>> out;
   {  // single join, scalar sum, map in LDS
@@ -20,7 +38,7 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metrics = {"(f3)*(f2)"};
     s.metric_ctype = {"int"};
     s.agg = AGG_SCALAR;
-    out.push_back({"scan_scalar", gen_scan_source(s)});
+    out.push_back({"scan_scalar", scan_checked(s)});
   }
   {  // three joins, dense LDS group table
     KernelShape s;
@@ -35,7 +53,7 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metrics = {"(f3)"};
     s.metric_ctype = {"int"};
     s.agg = AGG_DENSE;
-    out.push_back({"scan_dense", gen_scan_source(s)});
+    out.push_back({"scan_dense", scan_checked(s)});
   }
   {  // four joins, LDS hash group table, two int4 groups per iteration
     KernelShape s;
@@ -54,22 +72,22 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metric_ctype = {"int"};
     s.agg = AGG_HASH;
     s.hash_bits = 10;
-    out.push_back({"scan_hash", gen_scan_source(s)});
+    out.push_back({"scan_hash", scan_checked(s)});
     s.unroll = 1;
     s.prefetch = true;
     s.block = 512;
-    out.push_back({"scan_hash_prefetch", gen_scan_source(s)});
+    out.push_back({"scan_hash_prefetch", scan_checked(s)});
     s.prefetch = false;
     s.unroll = 2;
     s.flat = true;
-    out.push_back({"scan_hash_flat", gen_scan_source(s)});
+    out.push_back({"scan_hash_flat", scan_checked(s)});
     // the large code maps probed through their existence bits
     s.joins[1].defer = true;
     s.joins[2].defer = true;
-    out.push_back({"scan_hash_flat_deferred", gen_scan_source(s)});
+    out.push_back({"scan_hash_flat_deferred", scan_checked(s)});
     s.flat = false;
     s.unroll = 1;
-    out.push_back({"scan_hash_deferred", gen_scan_source(s)});
+    out.push_back({"scan_hash_deferred", scan_checked(s)});
   }
   {  // row-id payload, dimension attribute in the metric, source-column key
     KernelShape s;
@@ -83,18 +101,18 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metrics = {"(*(const int*)(DCOL1_1+(long long)JI1*4))*(f1)"};
     s.metric_ctype = {"long long"};
     s.agg = AGG_HASH;
-    out.push_back({"scan_row", gen_scan_source(s)});
+    out.push_back({"scan_row", scan_checked(s)});
     s.flat = true;
     s.filter = "StrCmp(((const char*)(FCOL2+(long long)i*10)),PS(0),PI(0))!=PI(1)";
     s.fact_other = {2};
-    out.push_back({"scan_row_flat", gen_scan_source(s)});
+    out.push_back({"scan_row_flat", scan_checked(s)});
     // a group space too wide for a table: packed 64-bit keys, sort + reduce; two metrics
     s.agg = AGG_SORT;
     s.metrics.push_back("(f0)-(f1)");
     s.metric_ctype.push_back("int");
-    out.push_back({"scan_sort_flat", gen_scan_source(s)});
+    out.push_back({"scan_sort_flat", scan_checked(s)});
     s.flat = false;
-    out.push_back({"scan_sort", gen_scan_source(s)});
+    out.push_back({"scan_sort", scan_checked(s)});
   }
   {  // probe joins on the reference's hash table: an Int64 key and an Int32 key, grouped
     KernelShape s;
@@ -113,7 +131,7 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.metric_ctype = {"int"};
     s.agg = AGG_HASH;
     s.flat = true;  // falls back to the per-row form
-    out.push_back({"scan_probe", gen_scan_source(s)});
+    out.push_back({"scan_probe", scan_checked(s)});
   }
   {  // select with a string-key probe join (Probe<char>) projecting dimension strings
     KernelShape s;
@@ -127,7 +145,7 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.proj = {"((const char*)(DCOL1_1+(long long)JI1*40))", "((const char*)(DCOL1_0+(long long)JI1*40))",
               "((const char*)(FCOL2+(long long)i*40))"};
     s.proj_type = {{String, 40}, {String, 40}, {String, 40}};
-    out.push_back({"select_probe_str", gen_select_source(s)});
+    out.push_back({"select_probe_str", select_checked(s)});
   }
   {  // select with a string filter and projections, plus a row-id join
     KernelShape s;
@@ -140,7 +158,7 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.proj = {"(f0)", "((const char*)(FCOL1+(long long)i*10))",
               "(*(const int*)(DCOL1_1+(long long)JI1*4))"};
     s.proj_type = {{Int32, 4}, {String, 10}, {Int32, 4}};
-    out.push_back({"select", gen_select_source(s)});
+    out.push_back({"select", select_checked(s)});
   }
   return out;
 }
