@@ -13,6 +13,9 @@
 //   * select pipelines compact passing rows in row order with wave64 ballots.
 #include "codegen.h"
 
+#include <algorithm>
+#include <cctype>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -340,6 +343,133 @@ void emit_aggregate(std::ostringstream& o, const KernelShape& s) {
 
 }  // namespace
 
+namespace {
+
+// One dense-map join test of the flat form for row e of group u: `cond` becomes
+// false when the row's key has no slot (rows already failing read slot 0), and the
+// payload (or, for a deferred map, the slot offset) is kept in pp<j>_<u>[e].
+void emit_flat_join_test(std::ostringstream& o, const KernelShape& s, size_t j, int u) {
+  const JoinShape& J = s.joins[j];
+  if (J.defer) {
+    // the existence bit decides; the slot offset is kept and the payload read
+    // only for rows that pass every join
+    o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+      << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
+      << "        const unsigned b = svm_map_get<1>(m" << j << " + a.iv[" << 4 * j + 2 << "], in ? off : 0u);\n"
+      << "        cond = in && b != 0u;\n"
+      << "        pp" << j << "_" << u << "[e] = off; }\n";
+    return;
+  }
+  o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+    << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
+    << "        unsigned p = svm_map_get<" << J.wbits << ">(m" << j << ", in ? off : 0u);\n"
+    << "        p = in ? p : 0u;\n"
+    << "        cond = p != 0u;\n"
+    << "        pp" << j << "_" << u << "[e] = p; }\n";
+}
+
+// The flat form's second sweep: each passing row of group u is aggregated.
+void emit_flat_aggregate(std::ostringstream& o, const KernelShape& s, int u) {
+  o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+    << "      if (!cnd" << u << "[e]) continue;\n"
+    << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
+  for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+  for (size_t j = 0; j < s.joins.size(); ++j) {
+    if (s.joins[j].defer)
+      o << "      const unsigned p" << j << " = svm_map_get<" << s.joins[j].wbits << ">(m" << j << ", pp" << j
+        << "_" << u << "[e]);\n      (void)p" << j << ";\n";
+    else
+      o << "      const unsigned p" << j << " = pp" << j << "_" << u << "[e];\n      (void)p" << j << ";\n";
+    if (s.joins[j].payload == PAY_ROW)
+      o << "      const long long JI" << s.joins[j].robj << " = (long long)p" << j << " - 1;\n";
+  }
+  o << "      {\n";
+  emit_aggregate(o, s);
+  o << "      }\n    }\n";
+}
+
+// Load tiers of the staged form: the tier of each source column and of the filter.
+struct Tiers {
+  bool on = false;
+  int last = 0;               // the tier after every join tier (metric, key, filter columns)
+  std::map<int, int> col;     // fact_i32 column -> tier
+  int filter = 0;             // tier at which the filter is applied
+};
+
+// The Int32 source columns an expression names as f<c>; false when it reads anything
+// else of a row (other source columns, dimension columns through a join).
+bool int_columns_of(const std::string& e, std::vector<int>& cols) {
+  if (e.find("FCOL") != std::string::npos || e.find("DCOL") != std::string::npos ||
+      e.find("JI") != std::string::npos)
+    return false;
+  for (size_t i = 0; i < e.size(); ++i) {
+    if (e[i] != 'f' || (i > 0 && (isalnum((unsigned char)e[i - 1]) || e[i - 1] == '_'))) continue;
+    size_t k = i + 1;
+    while (k < e.size() && isdigit((unsigned char)e[k])) ++k;
+    if (k == i + 1 || (k < e.size() && (isalnum((unsigned char)e[k]) || e[k] == '_'))) continue;
+    cols.push_back(atoi(e.c_str() + i + 1));
+  }
+  return true;
+}
+
+Tiers column_tiers(const KernelShape& s, bool flat) {
+  Tiers t;
+  if (!flat || s.join_tier.size() != s.joins.size()) return t;
+  for (int jt : s.join_tier) t.last = std::max(t.last, jt + 1);
+  if (t.last == 0) return t;  // no join in a tier
+  t.on = true;
+  for (int c : s.fact_i32) t.col[c] = t.last;
+  for (size_t j = 0; j < s.joins.size(); ++j)
+    if (s.join_tier[j] >= 0) t.col[s.joins[j].lcol] = std::min(t.col[s.joins[j].lcol], s.join_tier[j]);
+  // the filter goes with the latest of its columns (after every tier when it reads more
+  // than Int32 source columns)
+  t.filter = 0;
+  std::vector<int> fc;
+  if (s.filter != "true") {
+    if (!int_columns_of(s.filter, fc)) t.filter = t.last;
+    for (int c : fc) t.filter = std::max(t.filter, t.col.count(c) ? t.col[c] : t.last);
+  }
+  return t;
+}
+
+// The staged form's first sweeps for group u: tier by tier, a lane loads the tier's
+// columns (tier 0: loaded with the iteration) only if one of its rows still passes,
+// then applies the tier's joins (and the filter at its tier) to its four rows.
+void emit_tiered_tests(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int u, const std::string& idx) {
+  o << "    bool cnd" << u << "[4];\n";
+  for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+  for (int t = 0; t <= tz.last; ++t) {
+    std::vector<int> load;
+    for (int c : s.fact_i32)
+      if (t > 0 && tz.col.at(c) == t) load.push_back(c);
+    std::vector<size_t> js;
+    for (size_t j = 0; j < s.joins.size(); ++j)
+      if (s.join_tier[j] == t || (s.join_tier[j] < 0 && t == tz.last)) js.push_back(j);
+    const bool filt = s.filter != "true" && tz.filter == t;
+    if (!load.empty()) {
+      o << "    if (cnd" << u << "[0] | cnd" << u << "[1] | cnd" << u << "[2] | cnd" << u << "[3]) {\n";
+      for (int c : load)
+        o << "      x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + "
+          << idx << ");\n";
+      o << "    }\n";
+    }
+    if (t > 0 && js.empty() && !filt) continue;
+    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
+    for (int c : s.fact_i32)
+      if (tz.col.at(c) <= t) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
+    if (t == 0)
+      o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
+    else
+      o << "      bool cond = cnd" << u << "[e];\n";
+    if (filt) o << "      cond = cond && (" << s.filter << ");\n";
+    for (size_t j : js) emit_flat_join_test(o, s, j, u);
+    o << "      cnd" << u << "[e] = cond;\n    }\n";
+  }
+}
+
+}  // namespace
+
 std::string gen_scan_source(const KernelShape& s, const char* entry) {
   std::ostringstream o;
   emit_prologue(o, s);
@@ -382,20 +512,33 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
     << "  const long long v_lo = r_lo >> 2, v_hi = (r_hi + 3) >> 2;\n"
     << "  const long long GT = (long long)gridDim.x * SVM_BLOCK;\n";
   const bool pf = s.prefetch && U == 1;
+  // the flat form holds dense-map lookups only; probe joins use the per-row form
+  bool flat = s.flat;
+  for (auto& J : s.joins) flat = flat && J.kind == JOIN_MAP;
+  const Tiers tz = column_tiers(s, flat);
+  // columns loaded at the top of an iteration (all of them without staged loads)
+  auto up_front = [&](int c) { return !tz.on || tz.col.at(c) == 0; };
   if (pf) {
     // software pipelining: the next iteration's 128-bit groups are requested before
     // this iteration's rows are probed and aggregated (out-of-range indexes are
     // clamped to v_lo, a valid group whose rows fail the range test)
     o << "  long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x;\n";
     for (int c : s.fact_i32)
-      o << "  svm_i32x4 n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
-        << "] + (v0 < v_hi ? v0 : v_lo));\n";
+      if (up_front(c))
+        o << "  svm_i32x4 n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
+          << "] + (v0 < v_hi ? v0 : v_lo));\n";
     o << "  for (; v0 < v_hi; v0 += GT) {\n"
       << "    const long long v0_ = v0;\n";
-    for (int c : s.fact_i32) o << "    const svm_i32x4 x" << c << "_0 = n" << c << ";\n";
+    for (int c : s.fact_i32) {
+      if (up_front(c))
+        o << "    const svm_i32x4 x" << c << "_0 = n" << c << ";\n";
+      else
+        o << "    svm_i32x4 x" << c << "_0 = {0, 0, 0, 0};\n";
+    }
     o << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
     for (int c : s.fact_i32)
-      o << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
+      if (up_front(c))
+        o << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
     o << "    }\n";
   } else {
     o << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
@@ -403,15 +546,21 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
     for (int u = 0; u < U; ++u) {
       o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
         << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
-      for (int c : s.fact_i32)
-        o << "    const svm_i32x4 x" << c << "_" << u
-          << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+      for (int c : s.fact_i32) {
+        if (up_front(c))
+          o << "    const svm_i32x4 x" << c << "_" << u
+            << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+        else
+          o << "    svm_i32x4 x" << c << "_" << u << " = {0, 0, 0, 0};\n";
+      }
     }
   }
-  // the flat form holds dense-map lookups only; probe joins use the per-row form
-  bool flat = s.flat;
-  for (auto& J : s.joins) flat = flat && J.kind == JOIN_MAP;
   for (int u = 0; u < U; ++u) {
+    if (tz.on) {
+      emit_tiered_tests(o, s, tz, u, pf ? std::string("v0_") : "vc" + I(u));
+      emit_flat_aggregate(o, s, u);
+      continue;
+    }
     if (!flat) {
     o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
       << "      const long long i = v" << u << "_ * 4 + e;\n";
@@ -432,42 +581,9 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
     for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
     o << "      bool cond = (i >= r_lo) & (i < r_hi);\n"
       << "      cond = cond && (" << s.filter << ");\n";
-    for (size_t j = 0; j < s.joins.size(); ++j) {
-      const JoinShape& J = s.joins[j];
-      if (J.defer) {
-        // the existence bit decides; the slot offset is kept and the payload read
-        // only for rows that pass every join
-        o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
-          << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
-          << "        const unsigned b = svm_map_get<1>(m" << j << " + a.iv[" << 4 * j + 2 << "], in ? off : 0u);\n"
-          << "        cond = in && b != 0u;\n"
-          << "        pp" << j << "_" << u << "[e] = off; }\n";
-        continue;
-      }
-      o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
-        << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
-        << "        unsigned p = svm_map_get<" << J.wbits << ">(m" << j << ", in ? off : 0u);\n"
-        << "        p = in ? p : 0u;\n"
-        << "        cond = p != 0u;\n"
-        << "        pp" << j << "_" << u << "[e] = p; }\n";
-    }
+    for (size_t j = 0; j < s.joins.size(); ++j) emit_flat_join_test(o, s, j, u);
     o << "      cnd" << u << "[e] = cond;\n    }\n";
-    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
-      << "      if (!cnd" << u << "[e]) continue;\n"
-      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
-    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
-    for (size_t j = 0; j < s.joins.size(); ++j) {
-      if (s.joins[j].defer)
-        o << "      const unsigned p" << j << " = svm_map_get<" << s.joins[j].wbits << ">(m" << j << ", pp" << j
-          << "_" << u << "[e]);\n      (void)p" << j << ";\n";
-      else
-        o << "      const unsigned p" << j << " = pp" << j << "_" << u << "[e];\n      (void)p" << j << ";\n";
-      if (s.joins[j].payload == PAY_ROW)
-        o << "      const long long JI" << s.joins[j].robj << " = (long long)p" << j << " - 1;\n";
-    }
-    o << "      {\n";
-    emit_aggregate(o, s);
-    o << "      }\n    }\n";
+    emit_flat_aggregate(o, s, u);
     }
   }
   o << "  }\n";
@@ -531,7 +647,7 @@ namespace {
 void put(std::ostringstream& k, const std::string& v) { k << v.size() << ':' << v; }
 
 std::string shape_key(const KernelShape& s, char kind, const char* entry) {
-  static_assert(sizeof(KernelShape) == 304 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
+  static_assert(sizeof(KernelShape) == 328 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
                 "a field was added to the kernel shape: add it to shape_key");
   std::ostringstream k;
   k << kind << entry << '|' << s.block << ',' << s.unroll << ',' << s.prefetch << s.flat << ',' << s.agg << ','
@@ -556,6 +672,8 @@ std::string shape_key(const KernelShape& s, char kind, const char* entry) {
   for (auto& q : s.proj) put(k, q);
   k << '|';
   for (auto& t : s.proj_type) k << t.first << ',' << t.second << ';';
+  k << '|';
+  for (int t : s.join_tier) k << t << ',';
   return k.str();
 }
 

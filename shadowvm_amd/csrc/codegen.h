@@ -69,6 +69,12 @@ struct KernelShape {
   int unroll = 1;                        // int4 groups per thread per iteration
   bool prefetch = false;                 // load the next iteration's groups before processing
   bool flat = false;                     // branch-free filter + joins for the four rows, then aggregate
+  // Staged column loads (flat form; empty: every column is loaded up front).  The key
+  // column of join j is loaded in tier join_tier[j] (-1: with the last columns); a
+  // lane loads the columns of tier t > 0 only when one of its four rows passed every
+  // join of the tiers before, and the columns of no tier (metrics, group keys, filter
+  // columns, the keys of untiered joins) after the last tier.
+  std::vector<int> join_tier;
   // select
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs

@@ -12,7 +12,9 @@
 
 #include "loader.h"
 
+#include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <set>
 
@@ -158,6 +160,10 @@ struct JoinBuild {
   uint64_t dropped = 0;  // inserts the reference's hash table drops
   const int32_t* table = nullptr;  // probe joins: the reference's hash table (slots entries)
   uint64_t slots = 0;
+  // estimated fraction of source rows whose key the map holds: the dimension object's
+  // rows over its key column's root table's (foreign keys assumed spread over the
+  // root's keys); 1 when the object is not drawn from another table
+  double sel = 1.0;
 };
 
 struct KeyBuild {
@@ -277,6 +283,8 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
       range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
     }
     const bool probe = rt.first != Int32 || range > kMaxDenseRange;
+    if ((root.first != Dp || root.second != R.col) && root.first->bulk0_rows() > 0)
+      jb.sel = std::min(1.0, (double)n0 / (double)root.first->bulk0_rows());
     // payload selection
     std::vector<int> comps;
     for (size_t q = 0; q < keys.size(); ++q)
@@ -389,6 +397,52 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     joins.push_back(std::move(jb));
   }
   return joins;
+}
+
+// Staged column loads: the tier of each join's key column (KernelShape::join_tier), or
+// none.  A column is read in 32-byte sectors (8 rows); a sector of a column loaded
+// after tiers that pass an estimated fraction r of rows is read with probability
+// 1 - (1 - r)^8.  Dense-map joins that filter (estimated pass fraction below 1) are
+// taken most selective first: mode 1 puts the first k in one tier, mode 2 gives each
+// its own tier; k is chosen to minimise the estimated column bytes (plus a small
+// charge per tier for the dependent load), and staging is used only when that is at
+// most 85% of reading every column.
+std::vector<int> plan_tiers(const std::vector<JoinBuild>& joins, const KernelShape& ks, int mode) {
+  std::vector<size_t> order;
+  for (size_t j = 0; j < joins.size(); ++j)
+    if (joins[j].shape.kind == JOIN_MAP && joins[j].sel < 1.0) order.push_back(j);
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return joins[x].sel < joins[y].sel; });
+  const double ncols = (double)ks.fact_i32.size();
+  auto sector = [](double r) { return 1.0 - std::pow(1.0 - r, 8.0); };
+  double best = ncols * 0.85;
+  std::vector<int> plan;
+  for (size_t k = 1; k <= order.size(); ++k) {
+    std::vector<int> tier(joins.size(), -1);
+    for (size_t q = 0; q < k; ++q) tier[order[q]] = mode == 2 ? (int)q : 0;
+    // each column at its earliest tier; the rest after the last
+    std::map<int, int> ct;
+    for (size_t q = 0; q < k; ++q) {
+      const int c = joins[order[q]].shape.lcol, t = tier[order[q]];
+      ct[c] = ct.count(c) ? std::min(ct[c], t) : t;
+    }
+    const int ntiers = mode == 2 ? (int)k : 1;
+    std::vector<double> pass(ntiers + 1, 1.0);  // rows passing the tiers before t
+    for (int t = 1; t <= ntiers; ++t) {
+      pass[t] = pass[t - 1];
+      for (size_t q = 0; q < k; ++q)
+        if (tier[order[q]] == t - 1) pass[t] *= joins[order[q]].sel;
+    }
+    double cost = 0.05 * ntiers;
+    for (int c : ks.fact_i32) {
+      const int t = ct.count(c) ? ct[c] : ntiers;
+      cost += t == 0 ? 1.0 : sector(pass[t]);
+    }
+    if (cost < best) {
+      best = cost;
+      plan = tier;
+    }
+  }
+  return plan;
 }
 
 void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t& lds_words) {
@@ -523,6 +577,7 @@ struct TuneChoice {
   bool stage = true;                      // small join maps copied into LDS
   bool prefetch = false;                  // software-pipelined column loads
   bool flat = false;                      // branch-free filter + joins for a lane's four rows
+  int late = 0;                           // staged column loads (flat form): 1 one tier, 2 chained tiers
 };
 struct TuneState {
   bool done = false;
@@ -547,7 +602,12 @@ const TuneChoice kTuneCandidates[] = {
     // the flat form with the next groups' loads in flight during this group's lookups
     {1024, 1, 0, true, true, true},   {512, 1, 0, true, true, true},    {1024, 2, 0, true, false, true},
     // 1,024 threads, one or two workgroups per CU
-    {1024, 1, 1, true, false, true},  {1024, 1, 2, true, false, true}};
+    {1024, 1, 1, true, false, true},  {1024, 1, 2, true, false, true},
+    // staged column loads: the selective joins' key columns first (one tier, or one
+    // tier per join), the other columns only for lanes with a row still passing
+    {1024, 1, 0, true, false, true, 1}, {1024, 1, 0, true, true, true, 1}, {512, 1, 0, true, true, true, 1},
+    {1024, 1, 2, true, false, true, 1}, {1024, 1, 0, true, false, true, 2}, {1024, 1, 0, true, true, true, 2},
+    {512, 1, 0, true, true, true, 2},   {1024, 1, 2, true, false, true, 2}, {1024, 2, 0, true, false, true, 2}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -835,6 +895,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2.unroll = t.unroll;
     k2.prefetch = t.prefetch;
     k2.flat = t.flat;
+    k2.join_tier.clear();
+    if (t.late && t.flat) k2.join_tier = plan_tiers(joins, ks, t.late);
     for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
     words = t.stage ? lds_words : agg_words;
   };
@@ -890,9 +952,9 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
                  !k::words_differ(ref_flag.p, gflag.p, flag_words, diff.as<uint32_t>(), c.s);
         }
         if (!same) {
-          fprintf(stderr, "[svm] autotune %s: candidate %d (%d,%d,%d,%d,%d,%d) gave a different group table; "
+          fprintf(stderr, "[svm] autotune %s: candidate %d (%d,%d,%d,%d,%d,%d,%d) gave a different group table; "
                           "not chosen\n", c.info.id.c_str(), ci, cand.block, cand.unroll, cand.bpc, (int)cand.stage,
-                  (int)cand.prefetch, (int)cand.flat);
+                  (int)cand.prefetch, (int)cand.flat, cand.late);
           ms = -1.0f;
         }
         st.trials.push_back({cand, ms});
@@ -903,8 +965,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
       if (c.E.cfg.verbose) {
         fprintf(stderr, "[svm] autotune %s rows=%llu:", c.info.id.c_str(), (unsigned long long)rows);
         for (auto& t : st.trials)
-          fprintf(stderr, " (%d,%d,%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc,
-                  (int)t.first.stage, (int)t.first.prefetch, (int)t.first.flat, t.second);
+          fprintf(stderr, " (%d,%d,%d,%d,%d,%d,%d)=%.3fms", t.first.block, t.first.unroll, t.first.bpc,
+                  (int)t.first.stage, (int)t.first.prefetch, (int)t.first.flat, t.first.late, t.second);
         fprintf(stderr, "\n");
       }
     }
@@ -920,6 +982,11 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   }
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
               (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
+  if (!ks.join_tier.empty()) {
+    res->tune += choice.late == 2 ? " chained-tiers(" : " staged(";
+    for (size_t j = 0; j < ks.join_tier.size(); ++j) res->tune += (j ? "," : "") + std::to_string(ks.join_tier[j]);
+    res->tune += ")";
+  }
   hipFunction_t fn = nullptr;
   if (run_scan) {
     const auto gsrc = scan_source(ks);
@@ -1562,7 +1629,7 @@ std::string scan_tune_report(bool reset) {
     for (int ci = 0; ci < kNumTuneCandidates; ++ci) {
       const TuneChoice& t = kTuneCandidates[ci];
       if (t.block == st.best.block && t.unroll == st.best.unroll && t.bpc == st.best.bpc && t.stage == st.best.stage &&
-          t.prefetch == st.best.prefetch && t.flat == st.best.flat)
+          t.prefetch == st.best.prefetch && t.flat == st.best.flat && t.late == st.best.late)
         best = ci;
     }
     std::string id;

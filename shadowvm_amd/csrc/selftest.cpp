@@ -85,8 +85,19 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.joins[1].defer = true;
     s.joins[2].defer = true;
     out.push_back({"scan_hash_flat_deferred", scan_checked(s)});
-    s.flat = false;
+    // staged loads: the key columns of joins 1 and 2 first, the rest for lanes with a
+    // passing row (one tier; then a tier each, with the first tier prefetched)
+    s.join_tier = {-1, 0, 0, -1};
+    out.push_back({"scan_staged", scan_checked(s)});
+    s.join_tier = {2, 0, 1, -1};
     s.unroll = 1;
+    s.prefetch = true;
+    s.filter = "(f4)>PI(0)&&(f1)!=PI(1)";
+    out.push_back({"scan_staged_chain", scan_checked(s)});
+    s.join_tier.clear();
+    s.prefetch = false;
+    s.filter = "true";
+    s.flat = false;
     out.push_back({"scan_hash_deferred", scan_checked(s)});
   }
   {  // row-id payload, dimension attribute in the metric, source-column key
