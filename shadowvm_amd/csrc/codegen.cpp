@@ -190,12 +190,27 @@ void emit_map_ptrs(std::ostringstream& o, const KernelShape& s) {
       o << "  const unsigned* m" << j << " = svm_lds + a.iv[" << 4 * j + 3 << "];\n";
     else
       o << "  const unsigned* m" << j << " = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "];\n";
+    // a deferred map's existence bits (after its packed words, or staged in LDS)
+    if (J.kind == JOIN_MAP && J.defer) {
+      if (J.bits_lds)
+        o << "  const unsigned* mb" << j << " = svm_lds + a.iv[" << 4 * j + 3 << "];\n";
+      else
+        o << "  const unsigned* mb" << j << " = m" << j << " + a.iv[" << 4 * j + 2 << "];\n";
+    }
   }
 }
 
 void emit_stage_maps(std::ostringstream& o, const KernelShape& s) {
   bool any = false;
   for (size_t j = 0; j < s.joins.size(); ++j) {
+    if (s.joins[j].bits_lds) {
+      // the existence bits: ceil(range / 32) words after the packed map
+      o << "  { const unsigned* g = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "] + a.iv[" << 4 * j + 2 << "];\n"
+        << "    unsigned* d = svm_lds + a.iv[" << 4 * j + 3 << "];\n"
+        << "    const int nw = (int)(((unsigned)a.iv[" << 4 * j + 1 << "] + 31u) >> 5);\n"
+        << "    for (int w = threadIdx.x; w < nw; w += SVM_BLOCK) d[w] = g[w]; }\n";
+      continue;
+    }
     if (!s.joins[j].lds) continue;
     any = true;
     o << "  { const unsigned* g = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "];\n"
@@ -227,8 +242,8 @@ void emit_row_body_joins(std::ostringstream& o, const KernelShape& s, const std:
       o << ind << "unsigned p" << j << " = 0u;\n"
         << ind << "if (cond) {\n"
         << ind << "  const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
-        << ind << "  if (off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<1>(m" << j << " + a.iv["
-        << 4 * j + 2 << "], off)) p" << j << " = svm_map_get<" << J.wbits << ">(m" << j << ", off);\n"
+        << ind << "  if (off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<1>(mb" << j
+        << ", off)) p" << j << " = svm_map_get<" << J.wbits << ">(m" << j << ", off);\n"
         << ind << "  cond = p" << j << " != 0u;\n"
         << ind << "}\n";
       continue;
@@ -355,7 +370,7 @@ void emit_flat_join_test(std::ostringstream& o, const KernelShape& s, size_t j, 
     // only for rows that pass every join
     o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
       << "        const bool in = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
-      << "        const unsigned b = svm_map_get<1>(m" << j << " + a.iv[" << 4 * j + 2 << "], in ? off : 0u);\n"
+      << "        const unsigned b = svm_map_get<1>(mb" << j << ", in ? off : 0u);\n"
       << "        cond = in && b != 0u;\n"
       << "        pp" << j << "_" << u << "[e] = off; }\n";
     return;
@@ -432,12 +447,18 @@ Tiers column_tiers(const KernelShape& s, bool flat) {
   return t;
 }
 
-// The staged form's first sweeps for group u: tier by tier, a lane loads the tier's
-// columns (tier 0: loaded with the iteration) only if one of its rows still passes,
-// then applies the tier's joins (and the filter at its tier) to its four rows.
-void emit_tiered_tests(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int u, const std::string& idx) {
-  o << "    bool cnd" << u << "[4];\n";
-  for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+// The staged form for all U groups: tier by tier (so that the groups' loads and lookups
+// of a tier are in flight together) a lane loads the tier's columns (tier 0: with the
+// iteration) only if one of its rows still passes, then applies the tier's joins (and
+// the filter at its tier) to its four rows; then each group's passing rows are
+// aggregated.  `pf_next` (the next iteration's tier-0 loads) goes after the tier-0
+// lookups.
+void emit_tiered(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int U, bool pf,
+                 const std::string& pf_next) {
+  for (int u = 0; u < U; ++u) {
+    o << "    bool cnd" << u << "[4];\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+  }
   for (int t = 0; t <= tz.last; ++t) {
     std::vector<int> load;
     for (int c : s.fact_i32)
@@ -446,26 +467,36 @@ void emit_tiered_tests(std::ostringstream& o, const KernelShape& s, const Tiers&
     for (size_t j = 0; j < s.joins.size(); ++j)
       if (s.join_tier[j] == t || (s.join_tier[j] < 0 && t == tz.last)) js.push_back(j);
     const bool filt = s.filter != "true" && tz.filter == t;
-    if (!load.empty()) {
+    const bool tests = t == 0 || !js.empty() || filt;
+    auto gated_loads = [&](int u) {
+      if (load.empty()) return;
+      const std::string idx = pf ? std::string("v0_") : "vc" + I(u);
       o << "    if (cnd" << u << "[0] | cnd" << u << "[1] | cnd" << u << "[2] | cnd" << u << "[3]) {\n";
       for (int c : load)
         o << "      x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + "
           << idx << ");\n";
       o << "    }\n";
-    }
-    if (t > 0 && js.empty() && !filt) continue;
-    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
-      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
-    for (int c : s.fact_i32)
-      if (tz.col.at(c) <= t) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
-    if (t == 0)
-      o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
-    else
-      o << "      bool cond = cnd" << u << "[e];\n";
-    if (filt) o << "      cond = cond && (" << s.filter << ");\n";
-    for (size_t j : js) emit_flat_join_test(o, s, j, u);
-    o << "      cnd" << u << "[e] = cond;\n    }\n";
+    };
+    auto row_tests = [&](int u) {
+      if (!tests) return;
+      o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+        << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
+      for (int c : s.fact_i32)
+        if (tz.col.at(c) <= t)
+          o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
+      if (t == 0)
+        o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
+      else
+        o << "      bool cond = cnd" << u << "[e];\n";
+      if (filt) o << "      cond = cond && (" << s.filter << ");\n";
+      for (size_t j : js) emit_flat_join_test(o, s, j, u);
+      o << "      cnd" << u << "[e] = cond;\n    }\n";
+    };
+    for (int u = 0; u < U; ++u) gated_loads(u);
+    for (int u = 0; u < U; ++u) row_tests(u);
+    if (t == 0 && !pf_next.empty()) o << "    asm volatile(\"\" ::: \"memory\");\n" << pf_next;
   }
+  for (int u = 0; u < U; ++u) emit_flat_aggregate(o, s, u);
 }
 
 }  // namespace
@@ -518,6 +549,7 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
   const Tiers tz = column_tiers(s, flat);
   // columns loaded at the top of an iteration (all of them without staged loads)
   auto up_front = [&](int c) { return !tz.on || tz.col.at(c) == 0; };
+  std::string pf_next;
   if (pf) {
     // software pipelining: the next iteration's 128-bit groups are requested before
     // this iteration's rows are probed and aggregated (out-of-range indexes are
@@ -535,11 +567,18 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       else
         o << "    svm_i32x4 x" << c << "_0 = {0, 0, 0, 0};\n";
     }
-    o << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
+    std::ostringstream nx;
+    nx << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
     for (int c : s.fact_i32)
       if (up_front(c))
-        o << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
-    o << "    }\n";
+        nx << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
+    nx << "    }\n";
+    // staged: requested after this iteration's first map lookups (loads return in
+    // order, so a lookup issued after the prefetch would wait for it)
+    if (tz.on)
+      pf_next = nx.str();
+    else
+      o << nx.str();
   } else {
     o << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
       << U << " * GT) {\n";
@@ -555,12 +594,8 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
       }
     }
   }
-  for (int u = 0; u < U; ++u) {
-    if (tz.on) {
-      emit_tiered_tests(o, s, tz, u, pf ? std::string("v0_") : "vc" + I(u));
-      emit_flat_aggregate(o, s, u);
-      continue;
-    }
+  if (tz.on) emit_tiered(o, s, tz, U, pf, pf_next);
+  for (int u = 0; u < U && !tz.on; ++u) {
     if (!flat) {
     o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
       << "      const long long i = v" << u << "_ * 4 + e;\n";
@@ -658,7 +693,7 @@ std::string shape_key(const KernelShape& s, char kind, const char* entry) {
   for (int c : s.fact_other) k << c << ',';
   k << '|';
   for (auto& J : s.joins)
-    k << J.robj << ',' << J.lcol << ',' << J.payload << ',' << J.wbits << ',' << J.lds << J.defer << ',' << J.kind << ','
+    k << J.robj << ',' << J.lcol << ',' << J.payload << ',' << J.wbits << ',' << J.lds << J.defer << J.bits_lds << ',' << J.kind << ','
       << J.rcol << ',' << J.ktype << ',' << J.ksize << ';';
   k << '|';
   for (auto& kv : s.dcol_slot) k << kv.first.first << '.' << kv.first.second << '=' << kv.second << ',';
@@ -712,7 +747,7 @@ std::string gen_host_source(const KernelShape& s0) {
   // into the task's output columns in row order), as HostQueryTransform's loop does
   // (codegen/template.cpp:61-73).
   KernelShape s = s0;
-  for (auto& J : s.joins) J.lds = false;
+  for (auto& J : s.joins) J.lds = J.bits_lds = false;
   const int M = (int)s.metrics.size();
   const bool select = !s.proj.empty();
   std::ostringstream o;

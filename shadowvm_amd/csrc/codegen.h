@@ -32,6 +32,8 @@ struct JoinShape {
   // word a.iv[4j+2]) and reads the packed payload only for rows that pass every
   // join: a large code map then stays out of the probe chain.
   bool defer = false;
+  // a deferred map's existence bits staged in LDS at word a.iv[4j+3]
+  bool bits_lds = false;
   int kind = JOIN_MAP;
   int rcol = 0;      // probes: the dimension's key column (read as DCOL<robj>_<rcol>)
   int ktype = 2;     // probes: key type (ValType) and width in bytes

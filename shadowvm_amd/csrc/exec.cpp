@@ -157,6 +157,10 @@ struct JoinBuild {
   uint64_t words = 0;
   DevBuf packed;
   uint32_t lds_off = 0;
+  // LDS placement for staged scans (assign_lds): whether the map, or a deferred map's
+  // existence bits, is staged and at which word
+  bool staged_lds = false, staged_bits = false;
+  uint32_t staged_off = 0;
   uint64_t dropped = 0;  // inserts the reference's hash table drops
   const int32_t* table = nullptr;  // probe joins: the reference's hash table (slots entries)
   uint64_t slots = 0;
@@ -445,16 +449,36 @@ std::vector<int> plan_tiers(const std::vector<JoinBuild>& joins, const KernelSha
   return plan;
 }
 
-void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t& lds_words) {
-  uint64_t used = (agg_bytes + 7) / 8 * 2;  // agg table first, 8-byte aligned, in words
+// LDS layouts, the group table first in both.  Plain kernels (jb.shape.lds, lds_off):
+// small dense maps in plan order while they fit.  Staged scans (staged_*): maps and the
+// existence bits of deferred maps, the most selective join first (a staged scan tests
+// it on every row, the others on the few rows that pass it; among equal estimates,
+// plan order).  lds_words[0] / [1]: the words each layout uses.
+void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t lds_words[2]) {
+  const uint64_t agg_words = (agg_bytes + 7) / 8 * 2;  // agg table first, 8-byte aligned, in words
+  uint64_t used = agg_words;
   for (auto& jb : joins) {
-    if (jb.words && !jb.shape.defer && (used + jb.words) * 4 <= kLdsBudget) {
+    if (jb.words && !jb.table && !jb.shape.defer && (used + jb.words) * 4 <= kLdsBudget) {
       jb.shape.lds = true;
       jb.lds_off = (uint32_t)used;
       used += jb.words;
     }
   }
-  lds_words = used;
+  lds_words[0] = used;
+  std::vector<size_t> order(joins.size());
+  for (size_t j = 0; j < order.size(); ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return joins[x].sel < joins[y].sel; });
+  used = agg_words;
+  for (size_t j : order) {
+    JoinBuild& jb = joins[j];
+    if (!jb.words || jb.table) continue;
+    const uint64_t need = jb.shape.defer ? ((uint64_t)jb.range + 31) / 32 : jb.words;
+    if ((used + need) * 4 > kLdsBudget) continue;
+    (jb.shape.defer ? jb.staged_bits : jb.staged_lds) = true;
+    jb.staged_off = (uint32_t)used;
+    used += need;
+  }
+  lds_words[1] = used;
 }
 
 void fill_common_args(Ctx& c, SvmArgs& a, const std::vector<JoinBuild>& joins,
@@ -605,9 +629,10 @@ const TuneChoice kTuneCandidates[] = {
     {1024, 1, 1, true, false, true},  {1024, 1, 2, true, false, true},
     // staged column loads: the selective joins' key columns first (one tier, or one
     // tier per join), the other columns only for lanes with a row still passing
-    {1024, 1, 0, true, false, true, 1}, {1024, 1, 0, true, true, true, 1}, {512, 1, 0, true, true, true, 1},
-    {1024, 1, 2, true, false, true, 1}, {1024, 1, 0, true, false, true, 2}, {1024, 1, 0, true, true, true, 2},
-    {512, 1, 0, true, true, true, 2},   {1024, 1, 2, true, false, true, 2}, {1024, 2, 0, true, false, true, 2}};
+    {1024, 1, 0, true, true, true, 1},  {1024, 2, 0, true, false, true, 1}, {1024, 1, 0, true, false, true, 2},
+    {1024, 1, 0, true, true, true, 2},  {512, 1, 0, true, true, true, 2},   {1024, 1, 2, true, false, true, 2},
+    {1024, 2, 0, true, false, true, 2}, {512, 2, 0, true, false, true, 2},  {512, 4, 0, true, false, true, 2},
+    {256, 4, 0, true, false, true, 2}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -851,8 +876,9 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     ks.hash_bits = kHashBits;
     agg_bytes = (uint64_t)(1 << kHashBits) * (4 + 8 * M);
   }
-  uint64_t lds_words = 0;
-  assign_lds(joins, agg_bytes, lds_words);
+  uint64_t layout_words[2] = {0, 0};
+  assign_lds(joins, agg_bytes, layout_words);
+  const uint64_t lds_words = layout_words[0];
   for (size_t j = 0; j < joins.size(); ++j) ks.joins[j].lds = joins[j].shape.lds;
 
   // outputs
@@ -889,6 +915,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   choice.bpc = opt.blocks_per_cu;
   // the kernel shape and dynamic LDS of a launch configuration (maps staged or not)
   const uint64_t agg_words = (agg_bytes + 7) / 8 * 2;
+  // (a staged scan uses the staged LDS layout; `layout_args` puts its offsets into a
+  // copy of the launch arguments)
   auto shape_for = [&](const TuneChoice& t, KernelShape& k2, uint64_t& words) {
     k2 = ks;
     k2.block = t.block;
@@ -897,8 +925,19 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2.flat = t.flat;
     k2.join_tier.clear();
     if (t.late && t.flat) k2.join_tier = plan_tiers(joins, ks, t.late);
-    for (size_t j = 0; j < joins.size(); ++j) k2.joins[j].lds = t.stage && joins[j].shape.lds;
-    words = t.stage ? lds_words : agg_words;
+    const bool staged = !k2.join_tier.empty();
+    for (size_t j = 0; j < joins.size(); ++j) {
+      k2.joins[j].lds = t.stage && (staged ? joins[j].staged_lds : joins[j].shape.lds);
+      k2.joins[j].bits_lds = t.stage && staged && joins[j].staged_bits;
+    }
+    words = t.stage ? layout_words[staged ? 1 : 0] : agg_words;
+  };
+  auto layout_args = [&](const KernelShape& k2) {
+    SvmArgs a2 = a;
+    if (!k2.join_tier.empty())
+      for (size_t j = 0; j < joins.size(); ++j)
+        if (!joins[j].table) a2.iv[4 * j + 3] = (int)joins[j].staged_off;
+    return a2;
   };
   const int forced = forced_scan_variant();
   if (forced >= 0) {
@@ -937,7 +976,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         clear_outputs();
         Event ta, tb;
         SVM_HIP(hipEventRecord(ta.e, c.s));
-        launch(f2, g2, (unsigned)k2.block, (unsigned)(w2 * 4), c.s, a);
+        launch(f2, g2, (unsigned)k2.block, (unsigned)(w2 * 4), c.s, layout_args(k2));
         SVM_HIP(hipEventRecord(tb.e, c.s));
         SVM_HIP(hipEventSynchronize(tb.e));
         float ms = elapsed(ta, tb);
@@ -979,6 +1018,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     KernelShape k2;
     shape_for(choice, k2, run_words);
     ks = k2;
+    a = layout_args(ks);
   }
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
               (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
@@ -1351,8 +1391,9 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   for (auto& jb : joins) ks.joins.push_back(jb.shape);
   ks.dcol_slot = assign_dcols(proj_ops);
   classify_fact_columns(c, ops, ks);
-  uint64_t lds_words = 0;
-  assign_lds(joins, 0, lds_words);
+  uint64_t layout_words[2] = {0, 0};
+  assign_lds(joins, 0, layout_words);
+  const uint64_t lds_words = layout_words[0];
   for (size_t j = 0; j < joins.size(); ++j) ks.joins[j].lds = joins[j].shape.lds;
 
   const uint64_t rows = c.src->rows;

@@ -90,11 +90,13 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.join_tier = {-1, 0, 0, -1};
     out.push_back({"scan_staged", scan_checked(s)});
     s.join_tier = {2, 0, 1, -1};
+    s.joins[1].bits_lds = true;  // existence bits of a deferred map in LDS
     s.unroll = 1;
     s.prefetch = true;
     s.filter = "(f4)>PI(0)&&(f1)!=PI(1)";
     out.push_back({"scan_staged_chain", scan_checked(s)});
     s.join_tier.clear();
+    s.joins[1].bits_lds = false;
     s.prefetch = false;
     s.filter = "true";
     s.flat = false;
