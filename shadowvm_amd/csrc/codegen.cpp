@@ -499,6 +499,87 @@ void emit_tiered(std::ostringstream& o, const KernelShape& s, const Tiers& tz, i
   for (int u = 0; u < U; ++u) emit_flat_aggregate(o, s, u);
 }
 
+// The compacting staged form.  Phase 1: each lane loads U int4 groups of the tier-0
+// columns and tests its rows against the tier-0 joins (and the filter when it reads
+// only tier-0 columns); the passing rows are marked in a 4U-bit mask.  Phase 2: the
+// wave's passing rows are numbered (prefix sum of the mask counts), lane l takes row
+// number kb + l (its owner lane found by a binary search over the exclusive prefixes
+// and the row by the owner's mask), gathers the key columns of tiers 1.. one tier at
+// a time while the row passes, and finally the whole row (filter and every join again,
+// for the payloads), which it aggregates.
+void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int U) {
+  o << "  const unsigned svm_lane = threadIdx.x & 63u;\n"
+    << "  const long long CH = (long long)SVM_BLOCK * " << U << ";\n"
+    << "  for (long long vb = v_lo + (long long)blockIdx.x * CH; vb < v_hi; vb += (long long)gridDim.x * CH) {\n";
+  for (int u = 0; u < U; ++u) {
+    o << "    const long long v" << u << "_ = vb + " << u << "ll * SVM_BLOCK + threadIdx.x;\n"
+      << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+    for (int c : s.fact_i32)
+      if (tz.col.at(c) == 0)
+        o << "    const svm_i32x4 x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
+          << "] + vc" << u << ");\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+  }
+  o << "    unsigned svm_m = 0u;\n";
+  const bool filt0 = s.filter != "true" && tz.filter == 0;
+  for (int u = 0; u < U; ++u) {
+    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
+    for (int c : s.fact_i32)
+      if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
+    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
+    if (filt0) o << "      cond = cond && (" << s.filter << ");\n";
+    for (size_t j = 0; j < s.joins.size(); ++j)
+      if (s.join_tier[j] == 0) emit_flat_join_test(o, s, j, u);
+    o << "      svm_m |= (cond ? 1u : 0u) << (" << 4 * u << " + e);\n    }\n";
+  }
+  o << "    const unsigned svm_c = (unsigned)__popc(svm_m);\n"
+    << "    unsigned svm_in = svm_c;\n"
+    << "    for (int d = 1; d < 64; d <<= 1) {\n"
+    << "      const unsigned t = __shfl_up(svm_in, d, 64);\n"
+    << "      if (svm_lane >= (unsigned)d) svm_in += t;\n"
+    << "    }\n"
+    << "    const unsigned svm_ex = svm_in - svm_c;\n"
+    << "    const unsigned svm_tot = __shfl(svm_in, 63, 64);\n"
+    << "    for (unsigned kb = 0; kb < svm_tot; kb += 64u) {\n"
+    << "      const unsigned k = kb + svm_lane;\n"
+    << "      int L = 0;\n"
+    << "      for (int st = 32; st > 0; st >>= 1) {\n"
+    << "        const unsigned ex = __shfl(svm_ex, L + st, 64);\n"
+    << "        if (ex <= k) L += st;\n"
+    << "      }\n"
+    << "      const unsigned mo = __shfl(svm_m, L, 64);\n"
+    << "      const unsigned nth = k - __shfl(svm_ex, L, 64);\n"
+    << "      if (k < svm_tot) {\n"
+    << "        unsigned mm = mo;\n"
+    << "        for (unsigned t = 0; t < nth; ++t) mm &= mm - 1u;\n"
+    << "        const unsigned bit = (unsigned)__builtin_ctz(mm);\n"
+    << "        const long long i = (vb + (long long)(bit >> 2) * SVM_BLOCK + (long long)((threadIdx.x & ~63u) + (unsigned)L)) * 4"
+       " + (long long)(bit & 3u);\n"
+    << "        bool cond = true;\n";
+  for (int t = 1; t < tz.last; ++t) {
+    o << "        if (cond) {\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) {
+      if (s.join_tier[j] != t) continue;
+      const JoinShape& J = s.joins[j];
+      o << "          { const unsigned off = (unsigned)(((const int*)a.ptr[" << J.lcol << "])[i] - a.iv[" << 4 * j << "]);\n";
+      if (J.defer)
+        o << "            cond = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<1>(mb" << j
+          << ", off) != 0u; }\n";
+      else
+        o << "            cond = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<" << J.wbits << ">(m"
+          << j << ", off) != 0u; }\n";
+    }
+    o << "        }\n";
+  }
+  o << "        if (cond) {\n";
+  for (int c : s.fact_i32) o << "          const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
+  emit_row_body_joins(o, s, "          ");
+  o << "          if (cond) {\n";
+  emit_aggregate(o, s);
+  o << "          }\n        }\n      }\n    }\n  }\n";
+}
+
 }  // namespace
 
 std::string gen_scan_source(const KernelShape& s, const char* entry) {
@@ -547,81 +628,85 @@ std::string gen_scan_source(const KernelShape& s, const char* entry) {
   bool flat = s.flat;
   for (auto& J : s.joins) flat = flat && J.kind == JOIN_MAP;
   const Tiers tz = column_tiers(s, flat);
-  // columns loaded at the top of an iteration (all of them without staged loads)
-  auto up_front = [&](int c) { return !tz.on || tz.col.at(c) == 0; };
-  std::string pf_next;
-  if (pf) {
-    // software pipelining: the next iteration's 128-bit groups are requested before
-    // this iteration's rows are probed and aggregated (out-of-range indexes are
-    // clamped to v_lo, a valid group whose rows fail the range test)
-    o << "  long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x;\n";
-    for (int c : s.fact_i32)
-      if (up_front(c))
-        o << "  svm_i32x4 n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
-          << "] + (v0 < v_hi ? v0 : v_lo));\n";
-    o << "  for (; v0 < v_hi; v0 += GT) {\n"
-      << "    const long long v0_ = v0;\n";
-    for (int c : s.fact_i32) {
-      if (up_front(c))
-        o << "    const svm_i32x4 x" << c << "_0 = n" << c << ";\n";
-      else
-        o << "    svm_i32x4 x" << c << "_0 = {0, 0, 0, 0};\n";
-    }
-    std::ostringstream nx;
-    nx << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
-    for (int c : s.fact_i32)
-      if (up_front(c))
-        nx << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
-    nx << "    }\n";
-    // staged: requested after this iteration's first map lookups (loads return in
-    // order, so a lookup issued after the prefetch would wait for it)
-    if (tz.on)
-      pf_next = nx.str();
-    else
-      o << nx.str();
-  } else {
-    o << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
-      << U << " * GT) {\n";
-    for (int u = 0; u < U; ++u) {
-      o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
-        << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+  const bool compact = tz.on && s.compact;
+  if (compact) emit_compact(o, s, tz, U > 8 ? 8 : U);
+  if (!compact) {
+    // columns loaded at the top of an iteration (all of them without staged loads)
+    auto up_front = [&](int c) { return !tz.on || tz.col.at(c) == 0; };
+    std::string pf_next;
+    if (pf) {
+      // software pipelining: the next iteration's 128-bit groups are requested before
+      // this iteration's rows are probed and aggregated (out-of-range indexes are
+      // clamped to v_lo, a valid group whose rows fail the range test)
+      o << "  long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x;\n";
+      for (int c : s.fact_i32)
+        if (up_front(c))
+          o << "  svm_i32x4 n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
+            << "] + (v0 < v_hi ? v0 : v_lo));\n";
+      o << "  for (; v0 < v_hi; v0 += GT) {\n"
+        << "    const long long v0_ = v0;\n";
       for (int c : s.fact_i32) {
         if (up_front(c))
-          o << "    const svm_i32x4 x" << c << "_" << u
-            << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+          o << "    const svm_i32x4 x" << c << "_0 = n" << c << ";\n";
         else
-          o << "    svm_i32x4 x" << c << "_" << u << " = {0, 0, 0, 0};\n";
+          o << "    svm_i32x4 x" << c << "_0 = {0, 0, 0, 0};\n";
+      }
+      std::ostringstream nx;
+      nx << "    { const long long vn = v0 + GT; const long long vnc = vn < v_hi ? vn : v_lo;\n";
+      for (int c : s.fact_i32)
+        if (up_front(c))
+          nx << "      n" << c << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vnc);\n";
+      nx << "    }\n";
+      // staged: requested after this iteration's first map lookups (loads return in
+      // order, so a lookup issued after the prefetch would wait for it)
+      if (tz.on)
+        pf_next = nx.str();
+      else
+        o << nx.str();
+    } else {
+      o << "  for (long long v0 = v_lo + (long long)blockIdx.x * SVM_BLOCK + threadIdx.x; v0 < v_hi; v0 += "
+        << U << " * GT) {\n";
+      for (int u = 0; u < U; ++u) {
+        o << "    const long long v" << u << "_ = v0 + " << u << " * GT;\n"
+          << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
+        for (int c : s.fact_i32) {
+          if (up_front(c))
+            o << "    const svm_i32x4 x" << c << "_" << u
+              << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c << "] + vc" << u << ");\n";
+          else
+            o << "    svm_i32x4 x" << c << "_" << u << " = {0, 0, 0, 0};\n";
+        }
       }
     }
-  }
-  if (tz.on) emit_tiered(o, s, tz, U, pf, pf_next);
-  for (int u = 0; u < U && !tz.on; ++u) {
-    if (!flat) {
-    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
-      << "      const long long i = v" << u << "_ * 4 + e;\n";
-    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
-    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
-    emit_row_body_joins(o, s, "      ");
-    o << "      if (cond) {\n";
-    emit_aggregate(o, s);
-    o << "      }\n    }\n";
-    } else {
-    // flat: the four rows' filters and map lookups are straight-line code (a row
-    // that needs no lookup reads entry 0), so their loads are in flight together;
-    // the passing rows are then aggregated one by one
-    o << "    bool cnd" << u << "[4];\n";
-    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
-    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
-      << "      const long long i = v" << u << "_ * 4 + e;\n";
-    for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
-    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n"
-      << "      cond = cond && (" << s.filter << ");\n";
-    for (size_t j = 0; j < s.joins.size(); ++j) emit_flat_join_test(o, s, j, u);
-    o << "      cnd" << u << "[e] = cond;\n    }\n";
-    emit_flat_aggregate(o, s, u);
+    if (tz.on) emit_tiered(o, s, tz, U, pf, pf_next);
+    for (int u = 0; u < U && !tz.on; ++u) {
+      if (!flat) {
+      o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+        << "      const long long i = v" << u << "_ * 4 + e;\n";
+      for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+      o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
+      emit_row_body_joins(o, s, "      ");
+      o << "      if (cond) {\n";
+      emit_aggregate(o, s);
+      o << "      }\n    }\n";
+      } else {
+      // flat: the four rows' filters and map lookups are straight-line code (a row
+      // that needs no lookup reads entry 0), so their loads are in flight together;
+      // the passing rows are then aggregated one by one
+      o << "    bool cnd" << u << "[4];\n";
+      for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+      o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
+        << "      const long long i = v" << u << "_ * 4 + e;\n";
+      for (int c : s.fact_i32) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n";
+      o << "      bool cond = (i >= r_lo) & (i < r_hi);\n"
+        << "      cond = cond && (" << s.filter << ");\n";
+      for (size_t j = 0; j < s.joins.size(); ++j) emit_flat_join_test(o, s, j, u);
+      o << "      cnd" << u << "[e] = cond;\n    }\n";
+      emit_flat_aggregate(o, s, u);
+      }
     }
+    o << "  }\n";
   }
-  o << "  }\n";
   // flush
   if (s.agg == AGG_SCALAR) {
     o << "  __shared__ svm_i64 red[SVM_BLOCK / 64][" << (M > 0 ? M : 1) << "];\n";
@@ -682,7 +767,7 @@ namespace {
 void put(std::ostringstream& k, const std::string& v) { k << v.size() << ':' << v; }
 
 std::string shape_key(const KernelShape& s, char kind, const char* entry) {
-  static_assert(sizeof(KernelShape) == 328 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
+  static_assert(sizeof(KernelShape) == 336 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
                 "a field was added to the kernel shape: add it to shape_key");
   std::ostringstream k;
   k << kind << entry << '|' << s.block << ',' << s.unroll << ',' << s.prefetch << s.flat << ',' << s.agg << ','
@@ -709,6 +794,7 @@ std::string shape_key(const KernelShape& s, char kind, const char* entry) {
   for (auto& t : s.proj_type) k << t.first << ',' << t.second << ';';
   k << '|';
   for (int t : s.join_tier) k << t << ',';
+  k << '|' << s.compact;
   return k.str();
 }
 

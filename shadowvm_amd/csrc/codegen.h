@@ -77,6 +77,10 @@ struct KernelShape {
   // join of the tiers before, and the columns of no tier (metrics, group keys, filter
   // columns, the keys of untiered joins) after the last tier.
   std::vector<int> join_tier;
+  // staged, compacting form: tier 0 is streamed by every lane; each wave then hands
+  // the rows that pass it out one per lane, and those rows' later tiers, filter and
+  // aggregation run row by row from 4-byte gathers
+  bool compact = false;
   // select
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs

@@ -601,7 +601,8 @@ struct TuneChoice {
   bool stage = true;                      // small join maps copied into LDS
   bool prefetch = false;                  // software-pipelined column loads
   bool flat = false;                      // branch-free filter + joins for a lane's four rows
-  int late = 0;                           // staged column loads (flat form): 1 one tier, 2 chained tiers
+  int late = 0;                           // staged column loads (flat form): 1 one tier, 2 chained tiers,
+                                          // 3 chained tiers with the passing rows compacted per wave
 };
 struct TuneState {
   bool done = false;
@@ -632,7 +633,10 @@ const TuneChoice kTuneCandidates[] = {
     {1024, 1, 0, true, true, true, 1},  {1024, 2, 0, true, false, true, 1}, {1024, 1, 0, true, false, true, 2},
     {1024, 1, 0, true, true, true, 2},  {512, 1, 0, true, true, true, 2},   {1024, 1, 2, true, false, true, 2},
     {1024, 2, 0, true, false, true, 2}, {512, 2, 0, true, false, true, 2},  {512, 4, 0, true, false, true, 2},
-    {256, 4, 0, true, false, true, 2}};
+    {256, 4, 0, true, false, true, 2},
+    // the same with the rows that pass tier 0 handed out one per lane
+    {512, 2, 0, true, false, true, 3},  {512, 4, 0, true, false, true, 3},  {1024, 4, 0, true, false, true, 3},
+    {256, 8, 0, true, false, true, 3},  {512, 8, 0, true, false, true, 3},  {1024, 2, 0, true, false, true, 3}};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -924,7 +928,8 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2.prefetch = t.prefetch;
     k2.flat = t.flat;
     k2.join_tier.clear();
-    if (t.late && t.flat) k2.join_tier = plan_tiers(joins, ks, t.late);
+    if (t.late && t.flat) k2.join_tier = plan_tiers(joins, ks, t.late == 1 ? 1 : 2);
+    k2.compact = t.late == 3 && !k2.join_tier.empty();
     const bool staged = !k2.join_tier.empty();
     for (size_t j = 0; j < joins.size(); ++j) {
       k2.joins[j].lds = t.stage && (staged ? joins[j].staged_lds : joins[j].shape.lds);
@@ -1023,7 +1028,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
               (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
   if (!ks.join_tier.empty()) {
-    res->tune += choice.late == 2 ? " chained-tiers(" : " staged(";
+    res->tune += choice.late == 3 ? " compacted-tiers(" : choice.late == 2 ? " chained-tiers(" : " staged(";
     for (size_t j = 0; j < ks.join_tier.size(); ++j) res->tune += (j ? "," : "") + std::to_string(ks.join_tier[j]);
     res->tune += ")";
   }

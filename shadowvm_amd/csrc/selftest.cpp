@@ -95,6 +95,11 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.prefetch = true;
     s.filter = "(f4)>PI(0)&&(f1)!=PI(1)";
     out.push_back({"scan_staged_chain", scan_checked(s)});
+    s.compact = true;  // the rows passing tier 0 compacted per wave
+    s.prefetch = false;
+    s.unroll = 4;
+    out.push_back({"scan_compacted", scan_checked(s)});
+    s.compact = false;
     s.join_tier.clear();
     s.joins[1].bits_lds = false;
     s.prefetch = false;

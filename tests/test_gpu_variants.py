@@ -23,7 +23,7 @@ from .helpers import (assert_same_columns, edge_query, host_tables, oracle_query
 
 pytestmark = pytest.mark.gpu
 
-N_SCAN_CANDIDATES = 31   # kTuneCandidates in shadowvm_amd/csrc/exec.cpp
+N_SCAN_CANDIDATES = 37   # kTuneCandidates in shadowvm_amd/csrc/exec.cpp
 N_BATCH_VARIANTS = 8     # kVariants in shadowvm_amd/csrc/batch_kernels.hip
 
 
