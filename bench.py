@@ -31,14 +31,17 @@ def log(*a):
 
 def cpu_baseline(wl, sample_rows, threads):
     """The oracle (the reference's generated CPU path restated in C++) on the first
-    `sample_rows` fact rows (by default the whole table), on this host's cores."""
+    `sample_rows` fact rows (by default the whole table), on this host's cores.
+    Returns (baseline record, fact result, dimension results, host fact columns)."""
     from oracle import oracle as O
     from shadowvm_amd import ssb
 
-    tabs = {}
+    tabs, fact_cols = {}, None
     for t in wl.tables:
         n = min(t.rows, sample_rows) if t.id == wl.fact[1] else t.rows
         types, sizes, cols = ssb.host_table(wl, t, threads=threads, n=n)
+        if t.id == wl.fact[1]:
+            fact_cols = cols
         tabs[t.id] = O.OTable(types, sizes, cols, max_tuple_count=3_000_000)
     dims = {}
     for pt, src, refs in wl.dims:
@@ -61,7 +64,67 @@ def cpu_baseline(wl, sample_rows, threads):
             "host_cores": os.cpu_count(),
             "sample": f"{what} {n} fact rows of the same table, full dimensions, same query; hash build + "
                       f"scan + per-bulk reduce + merge (best of 2 runs) on {threads} threads of a "
-                      f"{os.cpu_count()}-core host"}, r
+                      f"{os.cpu_count()}-core host"}, r, dims, fact_cols
+
+
+def scan_bytes_model(config, fact_cols, dims, wl, line=128):
+    """HBM bytes the fact scan has to read, from its launch configuration and the data.
+
+    Without staged loads every referenced column is streamed: 4 B x rows x 6.  With
+    staged loads (the configuration lists each join's load tier, plan order), a
+    column of tier 0 is streamed whole, and a column of a later tier (the key of a
+    later join, or, after the last tier, the metric and remaining key columns) is read
+    in the `line`-byte L2 lines that hold a row passing every join of the earlier
+    tiers: HBM returns whole lines, so one passing row costs its line.  The pass
+    masks come from the host copy of the tables and the dimension results."""
+    import re
+    import numpy as np
+    rows = len(fact_cols[0])
+    per_line = line // 4
+    nlines = (rows + per_line - 1) // per_line
+    m = re.search(r"tiers\(([-0-9,]+)\)|staged\(([-0-9,]+)\)", config or "")
+    ncols = 6
+    if not m:
+        return nlines * line * ncols, {"model": "every column streamed", "line_bytes": line}
+    tiers = [int(x) for x in (m.group(1) or m.group(2)).split(",")]
+    fq, src, refs = wl.fact
+    last = max(tiers) + 1
+    # join j probes fact column j (orderdate, custkey, suppkey, partkey) against the
+    # key column of its dimension result
+    passes = []
+    for j, ref in enumerate(refs):
+        keys = np.asarray(dims[ref].columns[0])
+        fk = np.asarray(fact_cols[j])
+        lut = np.zeros(int(max(fk.max(), keys.max())) + 2, dtype=bool)
+        lut[keys] = True
+        passes.append(lut[fk])
+    col_tier = {c: last for c in range(ncols)}
+    for j, t in enumerate(tiers):
+        if t >= 0:
+            col_tier[j] = min(col_tier[j], t)
+
+    def lines_passing(t):
+        if t == 0:
+            return nlines
+        mask = np.ones(rows, dtype=bool)
+        for j, tj in enumerate(tiers):
+            if 0 <= tj < t:
+                mask &= passes[j]
+        pad = nlines * per_line - rows
+        if pad:
+            mask = np.concatenate([mask, np.zeros(pad, dtype=bool)])
+        return int(mask.reshape(nlines, per_line).any(axis=1).sum())
+
+    cache, total, detail = {}, 0, []
+    for c in range(ncols):
+        t = col_tier[c]
+        if t not in cache:
+            cache[t] = lines_passing(t)
+        total += cache[t] * line
+        detail.append({"column": c, "tier": t, "lines": cache[t]})
+    return total, {"model": "tier-0 columns streamed; later columns by the 128-byte lines holding a row "
+                            "that passed the earlier tiers", "line_bytes": line, "tiers": tiers,
+                   "columns": detail}
 
 
 def _pct(xs, p):
@@ -349,11 +412,22 @@ def main():
     rows = wl.fact_rows
     ms_per_step = total * 1e3 / args.steps
     scan_avg = sum(scan_ms) / len(scan_ms)
-    scan_bytes = rows * 24  # six int32 fact columns referenced
-    achieved = scan_bytes / (scan_avg / 1e3) / 1e9
+    full_bytes = rows * 24  # six int32 fact columns referenced
+    full_equiv = full_bytes / (scan_avg / 1e3) / 1e9
+    achieved = None  # from the bytes the scan has to read (needs the host tables, below)
     log(f"[gpu] step {ms_per_step:.3f} ms  fact build {sum(build_ms)/len(build_ms):.3f} ms  "
-        f"scan {scan_avg:.3f} ms ({achieved:.0f} GB/s)  finish {sum(finish_ms)/len(finish_ms):.3f} ms  "
-        f"groups {r.rows}")
+        f"scan {scan_avg:.3f} ms ({full_equiv:.0f} GB/s of all six columns)  finish "
+        f"{sum(finish_ms)/len(finish_ms):.3f} ms  groups {r.rows}")
+
+    # the scan's launch configuration (staged loads change what it must read)
+    import ctypes
+    from shadowvm_amd import _lib
+    Ld = ctypes.CDLL(_lib.LIB_PATH)
+    Ld.svm_debug_scan_config.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    cbuf = ctypes.create_string_buffer(512)
+    _lib.check(Ld.svm_debug_scan_config(wl.fact[0].pipe_info["id"].encode(), cbuf, len(cbuf)))
+    scan_config = cbuf.value.decode()
+    log(f"[gpu] svm_scan configuration: {scan_config}")
 
     if args.calib:
         # a streaming copy of one fact column (read n, write n bytes): the reference
@@ -383,10 +457,9 @@ def main():
             shapes.append({"block": block, "blocks_per_cu": bpc, "ms": ms.value,
                            "gb_per_s": (rows // 4) * 16 * 6 / (ms.value / 1e3) / 1e9})
         best = max(shapes, key=lambda x: x["gb_per_s"])
-        read_probe = {"kernel": "svm_read_probe", "bytes": (rows // 4) * 16 * 6, "best": best, "shapes": shapes,
-                      "scan_frac_of_read": achieved / best["gb_per_s"]}
+        read_probe = {"kernel": "svm_read_probe", "bytes": (rows // 4) * 16 * 6, "best": best, "shapes": shapes}
         log(f"[probe] plain read of the six columns: {best['gb_per_s']:.0f} GB/s at {best['block']} threads x "
-            f"{best['blocks_per_cu']}/CU ({best['ms']:.3f} ms); svm_scan at {achieved / best['gb_per_s']:.1%} of it")
+            f"{best['blocks_per_cu']}/CU ({best['ms']:.3f} ms)")
 
     # hybrid execution: `hybrid_ratio` of every 10 bulks on host cores, the rest on the
     # GPU at once, merged (after one untimed run that mirrors the columns to the host)
@@ -421,9 +494,16 @@ def main():
     loaded = bench_load(args.load_rows, args.seed, "/tmp") if args.load_rows else None
 
     cpu = None
+    bytes_model = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu, cres = cpu_baseline(wl, min(args.cpu_sample_rows, rows), threads)
+        cpu, cres, cdims, fact_cols = cpu_baseline(wl, min(args.cpu_sample_rows, rows), threads)
+        if cpu["sample"].startswith("all"):
+            alg_bytes, bytes_model = scan_bytes_model(scan_config, fact_cols, cdims, wl)
+            bytes_model["bytes"] = alg_bytes
+            achieved = alg_bytes / (scan_avg / 1e3) / 1e9
+            log(f"[gpu] svm_scan reads {alg_bytes / 1e9:.3f} GB ({bytes_model['model']}): {achieved:.0f} GB/s")
+        del fact_cols
         # the GPU's answer equals the CPU path's (when the baseline scanned the whole table)
         if cpu["sample"].startswith("all"):
             import numpy as np
@@ -457,8 +537,11 @@ def main():
                                "(d_year, c_city, s_city), SUM(revenue-supplycost)",
                    "fact_rows": rows, "groups": r.rows, "constants": wl.constants},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "svm_scan", "kernel_ms": scan_avg},
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": traffic,
+                     "kernel": "svm_scan", "kernel_ms": scan_avg, "configuration": scan_config,
+                     "bytes_model": bytes_model,
+                     "frac_of_read_probe": achieved / read_probe["best"]["gb_per_s"] if achieved and read_probe else None,
+                     "all_columns_bytes": full_bytes, "all_columns_equivalent_gb_s": full_equiv},
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": cpu.get("bit_exact_vs_gpu") if cpu else None,
         "read_probe": read_probe,

@@ -56,6 +56,14 @@ int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* drop
  * SVM_AUTOTUNE=0 turns tuning off. */
 int svm_debug_tune_report(char* buf, size_t cap, int reset);
 
+/* The launch configuration of the last fact scan of pipeline `pipeline_id`, as text:
+ * "<threads>x<groups per thread> lds-maps|l2-maps [prefetch] [flat]" and, for a scan
+ * with staged column loads, " staged(...)", " chained-tiers(...)" or
+ * " compacted-tiers(...)" listing each join's load tier in plan order (-1: loaded
+ * with the metric and key columns after the last tier).  SVM_ERR_NOT_FOUND when no
+ * scan of that pipeline has run. */
+int svm_debug_scan_config(const char* pipeline_id, char* buf, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

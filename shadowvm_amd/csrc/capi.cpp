@@ -15,6 +15,7 @@ int server_wait(uint64_t ticket, int timeout_ms, svm_result** out, double* queue
 int server_stats(uint64_t* queued, uint64_t* running, uint64_t* done, uint64_t* bytes);
 int server_stop();
 std::string scan_tune_report(bool reset);
+std::string last_scan_config(const std::string& pipeline_id);
 std::string batch_tune_report(bool reset);
 }  // namespace svm
 
@@ -440,6 +441,15 @@ int svm_debug_tune_report(char* buf, size_t cap, int reset) {
     const std::string j = "{\"scan\":" + scan_tune_report(reset != 0) + ",\"batch\":" + batch_tune_report(reset != 0) + "}";
     if (j.size() + 1 > cap) fail(SVM_ERR_INVALID, "output buffer too small (" + std::to_string(j.size() + 1) + " bytes needed)");
     memcpy(buf, j.c_str(), j.size() + 1);
+  });
+}
+
+int svm_debug_scan_config(const char* pipeline_id, char* buf, size_t cap) {
+  return guard([&] {
+    if (!pipeline_id || !buf || cap == 0) fail(SVM_ERR_INVALID, "null pipeline id or output buffer");
+    const std::string t = last_scan_config(pipeline_id);
+    if (t.size() + 1 > cap) fail(SVM_ERR_INVALID, "output buffer too small (" + std::to_string(t.size() + 1) + " bytes needed)");
+    memcpy(buf, t.c_str(), t.size() + 1);
   });
 }
 

@@ -615,6 +615,7 @@ struct TuneState {
 };
 std::mutex g_tune_mu;
 std::map<uint64_t, TuneState> g_tune;
+std::map<std::string, std::string> g_last_config;  // pipeline id -> its last scan's configuration
 const TuneChoice kTuneCandidates[] = {
     {256, 1, 0, true, false, false},  {256, 2, 0, true, false, false},  {512, 1, 0, true, false, false},
     {512, 1, 0, false, false, false}, {1024, 1, 0, false, false, false}, {512, 1, 0, true, true, false},
@@ -637,6 +638,7 @@ const TuneChoice kTuneCandidates[] = {
     // the same with the rows that pass tier 0 handed out one per lane
     {512, 2, 0, true, false, true, 3},  {512, 4, 0, true, false, true, 3},  {1024, 4, 0, true, false, true, 3},
     {256, 8, 0, true, false, true, 3},  {512, 8, 0, true, false, true, 3},  {1024, 2, 0, true, false, true, 3}};
+constexpr TuneChoice kCompactDefault = {512, 8, 0, true, false, true, 3};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
 constexpr uint64_t kTuneMinRows = 1ull << 26;
@@ -944,6 +946,14 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         if (!joins[j].table) a2.iv[4 * j + 3] = (int)joins[j].staged_off;
     return a2;
   };
+  // Untuned default: when the joins are dense maps and staging is estimated to save
+  // bytes, the compacting staged form (fastest on every table measured); otherwise the
+  // plain form of the requested shape.
+  if (run_scan && opt.block == 256 && opt.unroll == 1) {
+    bool maps = true;
+    for (auto& jb : joins) maps = maps && jb.shape.kind == JOIN_MAP;
+    if (maps && !plan_tiers(joins, ks, 2).empty()) choice = kCompactDefault;
+  }
   const int forced = forced_scan_variant();
   if (forced >= 0) {
     choice = kTuneCandidates[forced];
@@ -1031,6 +1041,11 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     res->tune += choice.late == 3 ? " compacted-tiers(" : choice.late == 2 ? " chained-tiers(" : " staged(";
     for (size_t j = 0; j < ks.join_tier.size(); ++j) res->tune += (j ? "," : "") + std::to_string(ks.join_tier[j]);
     res->tune += ")";
+  }
+  if (run_scan && !opt.noalloc) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_last_config.size() > 4096) g_last_config.clear();
+    g_last_config[c.info.id] = res->tune;
   }
   hipFunction_t fn = nullptr;
   if (run_scan) {
@@ -1654,6 +1669,13 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
 }
 
 }  // namespace
+
+std::string last_scan_config(const std::string& pipeline_id) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_last_config.find(pipeline_id);
+  if (it == g_last_config.end()) fail(SVM_ERR_NOT_FOUND, "no scan of pipeline '" + pipeline_id + "' has run");
+  return it->second;
+}
 
 std::string scan_tune_report(bool reset) {
   std::lock_guard<std::mutex> g(g_tune_mu);
