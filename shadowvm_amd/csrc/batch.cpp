@@ -1460,12 +1460,32 @@ bool is_heavy(uint64_t rows, uint64_t table_rows) { return rows >= kHeavyMinRows
 // Light queries waiting from which a heavy pass takes them along.
 constexpr size_t kLightBacklog = 4;
 
+// Seconds per fact row of a plain query's fact pipeline (map builds, scan, emission),
+// learned from the served queries that scanned at least kRateMinRows rows, where the
+// per-row part dominates (an exponential average; 3.7 ps per row, the rate of a scan
+// that reads every column, until one has run).  The staged scans read far fewer bytes
+// on selective queries, so the rate depends on the queries served.
+constexpr uint64_t kRateMinRows = 64ull << 20;
+std::mutex g_rate_mu;
+double g_plain_rate = 3.7e-12;
+double plain_rate() {
+  std::lock_guard<std::mutex> g(g_rate_mu);
+  return g_plain_rate;
+}
+void learn_plain_rate(const svm_result& r, uint64_t rows) {
+  if (rows < kRateMinRows) return;
+  const double sec = (r.build_ms + r.scan_ms + r.finish_ms) * 1e-3;
+  if (!(sec > 0)) return;
+  std::lock_guard<std::mutex> g(g_rate_mu);
+  g_plain_rate = 0.75 * g_plain_rate + 0.25 * (sec / (double)rows);
+}
+
 // Whether one shared scan over `batch` beats running its queries one by one on the
 // plain kernels, by a cost model fitted to measured passes on the 600M-row tables:
 // a pass costs ~4.1 ps per row of the union of the slices (the issue-bound shared
 // scan) + ~0.45 ps per slice row (each query's share) + ~0.2 ms (profile masks,
-// emission, synchronisation); a plain query ~3.7 ps per slice row (the fused scan at
-// its HBM rate) + ~0.15 ms (its dimension select, map builds, emission).  Queries
+// emission, synchronisation); a plain query plain_rate() per slice row (its fact
+// pipeline, as measured on the queries served) + ~0.15 ms (its dimension selects).  Queries
 // whose slices barely overlap are cheaper alone.  A fused pass answers all its
 // queries at its end, while queries run one by one finish in turn, so a pass is
 // chosen only for a clear gain (under 0.7 of the plain time): latency stays the plain
@@ -1492,7 +1512,7 @@ bool fusing_pays(const std::vector<std::shared_ptr<Ticket>>& batch) {
   }
   uni += (double)(hi - lo);
   const double fused = 4.1e-12 * uni + 0.45e-12 * sum + 0.2e-3;
-  const double plain = 3.7e-12 * sum + 0.15e-3 * (double)batch.size();
+  const double plain = plain_rate() * sum + 0.15e-3 * (double)batch.size();
   return fused < 0.7 * plain;
 }
 
@@ -1619,6 +1639,7 @@ void run_plain_query(Ticket& t, hipStream_t s) {
     for (auto& j : t.jobs) E.store.drop(j.info.id);
     throw;
   }
+  if (last) learn_plain_rate(*last, t.rows);
   t.result = std::move(last);
   for (auto& j : t.jobs) E.store.drop(j.info.id);
 }
