@@ -504,12 +504,32 @@ void emit_tiered(std::ostringstream& o, const KernelShape& s, const Tiers& tz, i
 // only tier-0 columns); the passing rows are marked in a 4U-bit mask.  Phase 2: the
 // wave's passing rows are numbered (prefix sum of the mask counts), lane l takes row
 // number kb + l (its owner lane found by a binary search over the exclusive prefixes
-// and the row by the owner's mask), gathers the key columns of tiers 1.. one tier at
-// a time while the row passes, and finally the whole row (filter and every join again,
-// for the payloads), which it aggregates.
+// and the row by the owner's mask) and gathers the key columns of tiers 1.. one tier
+// at a time while the row passes.  Phase 3: the rows that pass every tier are queued
+// per wave; every 64 of them (and the rest at the end) the wave gathers their whole
+// rows (filter and every join again, for the payloads) and aggregates them, one per
+// lane.
 void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int U) {
   o << "  const unsigned svm_lane = threadIdx.x & 63u;\n"
-    << "  const long long CH = (long long)SVM_BLOCK * " << U << ";\n"
+    // rows that passed every tier wait in a 128-entry ring per wave until 64 have
+    // gathered; the wave then evaluates and aggregates them together, one per lane
+    << "  __shared__ svm_u64 svm_q[SVM_BLOCK * 2];\n"
+    << "  svm_u64* const svm_qw = svm_q + (threadIdx.x >> 6) * 128u;\n"
+    << "  unsigned svm_qh = 0u, svm_qn = 0u;\n"
+    << "  auto svm_drain = [&](unsigned cnt) {\n"
+    << "    if (svm_lane < cnt) {\n"
+    << "      const long long i = (long long)svm_qw[(svm_qh + svm_lane) & 127u];\n"
+    << "      bool cond = true;\n";
+  for (int c : s.fact_i32) o << "      const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
+  emit_row_body_joins(o, s, "      ");
+  o << "      if (cond) {\n";
+  emit_aggregate(o, s);
+  o << "      }\n"
+    << "    }\n"
+    << "    svm_qh = (svm_qh + cnt) & 127u;\n"
+    << "    svm_qn -= cnt;\n"
+    << "  };\n";
+  o << "  const long long CH = (long long)SVM_BLOCK * " << U << ";\n"
     << "  for (long long vb = v_lo + (long long)blockIdx.x * CH; vb < v_hi; vb += (long long)gridDim.x * CH) {\n";
   for (int u = 0; u < U; ++u) {
     o << "    const long long v" << u << "_ = vb + " << u << "ll * SVM_BLOCK + threadIdx.x;\n"
@@ -550,6 +570,8 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     << "      }\n"
     << "      const unsigned mo = __shfl(svm_m, L, 64);\n"
     << "      const unsigned nth = k - __shfl(svm_ex, L, 64);\n"
+    << "      bool fin = false;\n"
+    << "      long long fi = 0;\n"
     << "      if (k < svm_tot) {\n"
     << "        unsigned mm = mo;\n"
     << "        for (unsigned t = 0; t < nth; ++t) mm &= mm - 1u;\n"
@@ -572,12 +594,20 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     }
     o << "        }\n";
   }
-  o << "        if (cond) {\n";
-  for (int c : s.fact_i32) o << "          const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
-  emit_row_body_joins(o, s, "          ");
-  o << "          if (cond) {\n";
-  emit_aggregate(o, s);
-  o << "          }\n        }\n      }\n    }\n  }\n";
+  o << "        fin = cond;\n"
+    << "        fi = i;\n"
+    << "      }\n"
+    << "      const unsigned long long fb = __ballot(fin);\n"
+    << "      if (fb) {\n"
+    << "        const unsigned p = svm_qn + (unsigned)__popcll(fb & ((1ull << svm_lane) - 1ull));\n"
+    << "        if (fin) svm_qw[(svm_qh + p) & 127u] = (svm_u64)fi;\n"
+    << "        svm_qn += (unsigned)__popcll(fb);\n"
+    << "        __builtin_amdgcn_wave_barrier();\n"
+    << "        if (svm_qn >= 64u) svm_drain(64u);\n"
+    << "      }\n"
+    << "    }\n"
+    << "  }\n"
+    << "  while (svm_qn > 0u) svm_drain(svm_qn < 64u ? svm_qn : 64u);\n";
 }
 
 }  // namespace

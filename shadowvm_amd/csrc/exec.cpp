@@ -637,7 +637,8 @@ const TuneChoice kTuneCandidates[] = {
     {256, 4, 0, true, false, true, 2},
     // the same with the rows that pass tier 0 handed out one per lane
     {512, 2, 0, true, false, true, 3},  {512, 4, 0, true, false, true, 3},  {1024, 4, 0, true, false, true, 3},
-    {256, 8, 0, true, false, true, 3},  {512, 8, 0, true, false, true, 3},  {1024, 2, 0, true, false, true, 3}};
+    {256, 8, 0, true, false, true, 3},  {512, 8, 0, true, false, true, 3},  {1024, 2, 0, true, false, true, 3},
+    {1024, 5, 0, true, false, true, 3}, {512, 6, 0, true, false, true, 3},  {512, 7, 0, true, false, true, 3}};
 constexpr TuneChoice kCompactDefault = {512, 8, 0, true, false, true, 3};
 // Below this many source rows the launch configuration hardly matters and tuning
 // would only add compiles to a short query's latency.
