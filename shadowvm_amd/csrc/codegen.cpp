@@ -132,6 +132,18 @@ __device__ __forceinline__ svm_u64 svm_wave_reserve(svm_u64* counter, bool want)
   base = __shfl(base, leader);
   return base + (svm_u64)__popcll(m & ((1ull << lane) - 1ull));
 }
+/* StrCopy with the widest accesses both addresses allow: four bytes, two, or one at a
+   time (the same bytes; a projection's rows are n-byte aligned in both columns). */
+__device__ __forceinline__ void svm_str_copy(char* d, const char* s, int n) {
+  const unsigned long long al = ((unsigned long long)d | (unsigned long long)s | (unsigned long long)n);
+  if ((al & 3ull) == 0ull) {
+    for (int k = 0; k < n; k += 4) *(unsigned*)(d + k) = *(const unsigned*)(s + k);
+  } else if ((al & 1ull) == 0ull) {
+    for (int k = 0; k < n; k += 2) *(unsigned short*)(d + k) = *(const unsigned short*)(s + k);
+  } else {
+    for (int k = 0; k < n; ++k) d[k] = s[k];
+  }
+}
 
 )SVMRT";
 
@@ -964,14 +976,20 @@ std::string gen_select_source(const KernelShape& s) {
       << "  const long long t0 = a.row0 + (long long)tile * (SVM_BLOCK * SVM_RPT);\n"
       << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
       << "  unsigned passbits = 0u;\n";
-    // first sweep: conditions and per-wave counts
-    o << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
-      << "    const long long i = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
-      << "    bool cond = i < r_hi;\n";
+    // first sweep: conditions and per-wave counts.  A row past the end reads the last
+    // row instead (t0 < r_hi for every tile), so every row's loads are unconditional and
+    // the SVM_RPT rows' loads are in flight together; its condition is then cleared.
+    o << "  #pragma unroll\n"
+      << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
+      << "    const long long iu = t0 + (long long)k * SVM_BLOCK + threadIdx.x;\n"
+      << "    const bool inr = iu < r_hi;\n"
+      << "    const long long i = inr ? iu : r_hi - 1;\n"
+      << "    bool cond = true;\n";
     for (int c : s.fact_i32)
-      o << "    const int f" << c << " = cond ? ((const int*)a.ptr[" << c << "])[i] : 0;\n";
+      o << "    const int f" << c << " = ((const int*)a.ptr[" << c << "])[i];\n";
     emit_row_body_joins(o, s, "    ");
-    o << "    const unsigned long long mask = __ballot(cond);\n"
+    o << "    cond = cond && inr;\n"
+      << "    const unsigned long long mask = __ballot(cond);\n"
       << "    if (lane == 0) wcnt[k][wave] = (unsigned)__popcll(mask);\n"
       << "    passbits |= (cond ? 1u : 0u) << k;\n"
       << "  }\n"
@@ -1039,7 +1057,8 @@ std::string gen_select_source(const KernelShape& s) {
         << "  if (s_err) return;  // no rows are written from an inconsistent prefix\n"
         << "  unsigned long long pos = s_pos;\n";
     }
-    o << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
+    o << "  #pragma unroll\n"
+      << "  for (int k = 0; k < SVM_RPT; ++k) {\n"
       << "    unsigned before = 0;\n"
       << "    for (int w = 0; w < wave; ++w) before += wcnt[k][w];\n"
       << "    unsigned round = 0;\n"
@@ -1068,7 +1087,7 @@ std::string gen_select_source(const KernelShape& s) {
       long long size = s.proj_type[q].second;
       std::string dst = "((char*)a.ptr[" + I(SVM_SLOT_OUT + (int)q) + "])";
       if (t == 7) {
-        o << "      StrCopy(" << dst << " + at * " << size << "ull, (const char*)(" << s.proj[q] << "), "
+        o << "      svm_str_copy(" << dst << " + at * " << size << "ull, (const char*)(" << s.proj[q] << "), "
           << size << ");\n";
       } else {
         const char* ct = t == 0 ? "char" : t == 1 ? "short" : t == 3 ? "long long" : "int";
