@@ -56,6 +56,11 @@ int svm_debug_hash_drops(const char* id, int col, uint64_t slots, uint64_t* drop
  * SVM_AUTOTUNE=0 turns tuning off. */
 int svm_debug_tune_report(char* buf, size_t cap, int reset);
 
+/* How many launch configurations each autotuner chooses from: the fact scan's
+ * candidates (SVM_SCAN_VARIANT=0..n-1) and the shared scan's kernel variants (the
+ * first field of SVM_BATCH_VARIANT).  Needs no GPU. */
+int svm_debug_variant_counts(int* scan_candidates, int* batch_variants);
+
 /* The launch configuration of the last fact scan of pipeline `pipeline_id`, as text:
  * "<threads>x<groups per thread> lds-maps|l2-maps [prefetch] [flat]" and, for a scan
  * with staged column loads, " staged(...)", " chained-tiers(...)" or

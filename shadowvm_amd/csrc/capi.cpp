@@ -17,6 +17,10 @@ int server_stop();
 std::string scan_tune_report(bool reset);
 std::string last_scan_config(const std::string& pipeline_id);
 std::string batch_tune_report(bool reset);
+int scan_tune_candidates();
+namespace bk {
+int scan_variants();
+}
 }  // namespace svm
 
 using namespace svm;
@@ -441,6 +445,14 @@ int svm_debug_tune_report(char* buf, size_t cap, int reset) {
     const std::string j = "{\"scan\":" + scan_tune_report(reset != 0) + ",\"batch\":" + batch_tune_report(reset != 0) + "}";
     if (j.size() + 1 > cap) fail(SVM_ERR_INVALID, "output buffer too small (" + std::to_string(j.size() + 1) + " bytes needed)");
     memcpy(buf, j.c_str(), j.size() + 1);
+  });
+}
+
+int svm_debug_variant_counts(int* scan_candidates, int* batch_variants) {
+  return guard([&] {
+    if (!scan_candidates || !batch_variants) fail(SVM_ERR_INVALID, "null output");
+    *scan_candidates = scan_tune_candidates();
+    *batch_variants = bk::scan_variants();
   });
 }
 

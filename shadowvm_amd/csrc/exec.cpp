@@ -1678,6 +1678,8 @@ std::string last_scan_config(const std::string& pipeline_id) {
   return it->second;
 }
 
+int scan_tune_candidates() { return kNumTuneCandidates; }
+
 std::string scan_tune_report(bool reset) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   std::string j = "[";

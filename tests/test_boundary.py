@@ -166,6 +166,16 @@ def test_generated_kernels_compile_for_gfx950(tmp_path, monkeypatch):
     assert len([f for f in os.listdir(tmp_path) if f.startswith("host_") and f.endswith(".so")]) == 5
 
 
+def test_variant_counts_match_the_variant_tests():
+    """The GPU variant tests force every autotuner candidate by index; their counts
+    must be the library's, or a candidate would go unchecked."""
+    from . import test_gpu_variants as V
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    scan, batch = ctypes.c_int(0), ctypes.c_int(0)
+    assert L.svm_debug_variant_counts(ctypes.byref(scan), ctypes.byref(batch)) == 0
+    assert (scan.value, batch.value) == (V.N_SCAN_CANDIDATES, V.N_BATCH_VARIANTS)
+
+
 def test_ssb_query_plans_decode_and_lower():
     """Every plan of the thirteen SSB query tests (dimension and fact pipelines) is
     decoded and its expressions lowered without a GPU."""

@@ -23,8 +23,10 @@ from .helpers import (assert_same_columns, edge_query, host_tables, oracle_query
 
 pytestmark = pytest.mark.gpu
 
-N_SCAN_CANDIDATES = 37   # kTuneCandidates in shadowvm_amd/csrc/exec.cpp
-N_BATCH_VARIANTS = 8     # kVariants in shadowvm_amd/csrc/batch_kernels.hip
+# kTuneCandidates in shadowvm_amd/csrc/exec.cpp, kVariants in shadowvm_amd/csrc/batch_kernels.hip;
+# tests/test_boundary.py checks both against the library (svm_debug_variant_counts)
+N_SCAN_CANDIDATES = 40
+N_BATCH_VARIANTS = 8
 
 
 def tune_report(reset=False):
