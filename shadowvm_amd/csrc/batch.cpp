@@ -840,13 +840,46 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     nprof += sl.prof->P + 1;
   }
   // Slots are looked up in order, each gated by the ones before it (a row no query
-  // still wants reads the shared no-row entry): the smallest key-to-profile arrays
-  // go first, so the rows that reach a large one (which does not stay in L2 beside
-  // the column stream) are those the small ones kept.  The masks are ANDed, so the
-  // order does not change any result.
-  std::stable_sort(P.slots.begin(), P.slots.end(), [](const SlotB& a, const SlotB& b) {
-    return (uint64_t)a.prof->range * (a.prof->narrow ? 1u : 2u) < (uint64_t)b.prof->range * (b.prof->narrow ? 1u : 2u);
-  });
+  // still wants reads the shared no-row entry), so the order sets how many rows
+  // reach each lookup.  As for any chain of filters, the slots go in ascending order
+  // of cost / (1 - pass fraction): the cost of a lookup by the size of its
+  // key-to-profile array (up to 64 KB it stays in LDS or L2 at little cost, up to
+  // 2 MB in L2; larger ones miss L2 for most keys), the pass fraction estimated as
+  // that of keys passing some joining query's filter (an equality atom passes 1/n of
+  // a column's n distinct values, an inequality 1 - 1/n, a range about half; queries
+  // taken as independent).  The masks are ANDed, so the order does not change any
+  // result.
+  {
+    std::vector<std::pair<double, uint64_t>> key(P.slots.size());
+    for (size_t k = 0; k < P.slots.size(); ++k) {
+      const SlotB& sl = P.slots[k];
+      const uint64_t entries = (uint64_t)sl.prof->range + 1;
+      const unsigned ebits = sl.prof->P + 1 <= 16 ? 4 : sl.prof->P + 1 <= 256 ? 8 : 16;
+      double miss = 1.0;  // no joining query passes
+      for (int i : P.qi)
+        for (const JoinL& j : qs[i]->joins) {
+          if (j.dim != sl.dim || j.kcol != sl.kcol || j.fcol != sl.fcol) continue;
+          double f = 1.0;
+          for (const Atom& a : j.atoms) {
+            const double n = std::max<uint32_t>(1u, table_dict(sl.dim, a.col, s)->n);
+            f *= a.kind == bk::DP_STREQ ? 1.0 / n : a.kind == bk::DP_STRNE ? 1.0 - 1.0 / n
+                 : a.lo > a.hi          ? 0.0
+                 : a.lo == a.hi         ? 1.0 / n
+                                        : 0.5;
+          }
+          miss *= 1.0 - f;
+        }
+      const uint64_t bytes = entries * ebits / 8;
+      const double cost = bytes <= (64u << 10) ? 0.25 : bytes <= (2u << 20) ? 1.0 : 3.0;
+      key[k] = {cost / std::max(1e-3, miss), bytes};
+    }
+    std::vector<size_t> ord(P.slots.size());
+    for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+    std::vector<SlotB> sorted;
+    for (size_t k : ord) sorted.push_back(std::move(P.slots[k]));
+    P.slots = std::move(sorted);
+  }
   if (nprof > (unsigned)bk::kMaxProfiles) fail(SVM_ERR_INVALID, "pass exceeds the LDS profile budget");
   auto slot_index = [&](const JoinL& j) {
     for (size_t k = 0; k < P.slots.size(); ++k)
@@ -940,6 +973,8 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     d.metric = (int)(std::find(P.metrics.begin(), P.metrics.end(), m) - P.metrics.begin());
     d.sum_off = (unsigned)so;
     d.flag_off = (unsigned)fo;
+    d.G = (unsigned)f.G;
+    d.dense = bk::kNoDense;
     for (auto& j : f.joins) {
       const int sk = slot_index(j);
       A->slot[sk].joined |= 1ull << q;
@@ -1087,7 +1122,10 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   const bool mask32 = nq <= 32;  // query masks of 32 bits suffice
   const auto tkey = std::make_tuple(A->nslot, ilog2((uint64_t)nq), ilog2(union_rows + 1), mask32);
   BatchTune ch;
-  ch.variant = mask32 ? 1 : 0;  // 512 threads, gated lookups
+  // untuned: 512 threads, LDS hash, gated lookups; at most 32 queries: 1,024 threads,
+  // dense tables, LDS maps and the row queue (selective passes such as whole-table
+  // star joins spend least time there)
+  ch.variant = mask32 ? bk::kDefaultVariantMask32 : 0;
   bool tuned = false;
   {
     std::lock_guard<std::mutex> g(g_btune_mu);
@@ -1117,6 +1155,8 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     auto trial = [&](const BatchTune& c) {
       set_tile(c.tile);
       A->use_bits = c.bits;
+      if (bk::scan_variant_dense(c.variant)) bk::assign_dense(qd, nq, bk::scan_variant_hash_slots(c.variant));
+      bk::assign_lds_maps(A, c.variant, cap);
       SVM_HIP(hipMemcpyAsync(dd, host, PassLayout::total, hipMemcpyHostToDevice, s));
       bk::launch_pass_clear(d_args, s);
       bk::launch_profile_masks(d_margs, s);
@@ -1138,8 +1178,9 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       }
       trials.push_back({(double)c.variant, (double)c.tile, (double)c.bits, (double)ms});
       if (E.cfg.verbose)
-        fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots, %s lookups%s, key bits %d, tile %lld rows: %.3f ms\n",
+        fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots (%s), %s lookups%s, key bits %d, tile %lld rows: %.3f ms\n",
                 bk::scan_variant_block(c.variant), bk::scan_variant_hash_slots(c.variant),
+                bk::scan_variant_dense(c.variant) ? "dense" : "hash",
                 bk::scan_variant_parallel(c.variant) ? "parallel" : "gated",
                 bk::scan_variant_prefetch(c.variant) ? ", prefetch" : "", c.bits, c.tile, ms);
       return ms < 0 ? 1e30f : ms;
@@ -1168,6 +1209,8 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   }
   A->use_bits = ch.bits;
   set_tile(ch.tile);
+  if (bk::scan_variant_dense(ch.variant)) bk::assign_dense(qd, nq, bk::scan_variant_hash_slots(ch.variant));
+  bk::assign_lds_maps(A, ch.variant, cap);
   const int grid = scan_grid(E, ch.variant, cap);
   // the pass: upload descriptors, then clear -> masks -> scan -> emission count
   hipEvent_t e0, e1, eb, es;
@@ -1232,6 +1275,12 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   if (E.cfg.verbose) {
     uint64_t slice_rows = 0;
     for (int q = 0; q < nq; ++q) slice_rows += (uint64_t)(qrow[2 * q + 1] - qrow[2 * q]);
+    std::string lds_maps;
+    for (int k = 0; k < A->nslot; ++k) {
+      const bk::Slot& S = A->slot[k];
+      lds_maps += k ? "," : "";
+      lds_maps += S.lmap >= 0 ? "map" + std::to_string(1 << S.lmap_log2) : S.lbits >= 0 ? "bits" : "-";
+    }
     char phases[128] = "";
     if (!graph) {
       float mb = 0, msc = 0;
@@ -1240,12 +1289,13 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       snprintf(phases, sizeof(phases), ", masks %.3f, scan %.3f", mb, msc);
     }
     fprintf(stderr, "[svm] shared scan of %d queries: gpu %.3f ms%s (%s; %u profiles in %zu slots; %llu rows / %llu "
-                    "slice rows; %d threads, %d LDS slots, %s lookups, key bits %d, tile %lld, %u tiles, grid %d; "
-                    "%d slots checked for dropped inserts, %d queries drop)\n",
+                    "slice rows; %d threads, %d LDS slots (%s), %s lookups, key bits %d, tile %lld, %u tiles, grid %d; "
+                    "LDS maps [%s]; %d slots checked for dropped inserts, %d queries drop)\n",
             nq, ms_all, phases, graph ? "graph" : "eager", nprof, P.slots.size(), (unsigned long long)union_rows,
             (unsigned long long)slice_rows, bk::scan_variant_block(ch.variant), bk::scan_variant_hash_slots(ch.variant),
+            bk::scan_variant_dense(ch.variant) ? "dense" : "hash",
             bk::scan_variant_parallel(ch.variant) ? "parallel" : "gated", ch.bits, (long long)ch.tile, A->ntiles, grid,
-            nchk, __builtin_popcountll(dq));
+            lds_maps.c_str(), nchk, __builtin_popcountll(dq));
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -90,6 +90,10 @@ struct Slot {
   unsigned long long joined;    // queries that join this slot
   unsigned* bits;               // per key offset: some query of the batch passes it (built per batch)
   int lds;                      // first profile of this slot in the LDS staging arrays (P + 1 entries)
+  // Variants with LDS maps (assigned per pass by assign_lds_maps): the key-to-profile
+  // map packed at 1 << lmap_log2 bits per entry starts at word lmap of the scan's LDS
+  // map area, or lmap is -1; the key bits start at word lbits there, or lbits is -1.
+  int lmap, lmap_log2, lbits;
 };
 
 // Per-query descriptor read by every lane that aggregates a row for the query.
@@ -99,7 +103,11 @@ struct QDesc {
   int K;
   int metric;                   // index into the scan's distinct metrics
   unsigned sum_off, flag_off;   // the query's group table in the batch arenas
+  unsigned G;                   // groups of the query
+  unsigned dense;               // dense-accumulator variants: first LDS slot of the query's
+                                //   G groups, or kNoDense (its pairs go to the HBM tables)
 };
+constexpr unsigned kNoDense = 0xFFFFFFFFu;
 
 struct FactPred {
   int q;
@@ -197,11 +205,21 @@ void launch_slot_bits(const ScanArgs* d_args, hipStream_t s);
 void launch_drop_check(const ScanArgs* d_args, hipStream_t s);
 // Shared-scan variants (workgroup size x LDS accumulator size) the autotuner picks from.
 int scan_variants();
+constexpr int kDefaultVariantMask32 = 11;  // kVariants: 1,024 threads, dense, LDS maps, queue, 32-bit masks
 int scan_variant_block(int v);
 int scan_variant_hash_slots(int v);
 bool scan_variant_parallel(int v);  // slot lookups issued together rather than gated in turn
 bool scan_variant_mask32(int v);    // 32-bit query masks (scans of at most 32 queries)
 bool scan_variant_prefetch(int v);  // next row group's loads in flight during this one's lookups
+// Dense-accumulator variants: a query with few groups owns scan_variant_hash_slots(v)
+// LDS slots' worth of a per-workgroup dense table (QDesc::dense, assigned per pass by
+// assign_dense); the others add straight into their HBM tables.
+bool scan_variant_dense(int v);
+void assign_dense(QDesc* qd, int nq, unsigned slots);
+// Variants that stage small key-to-profile maps and key bits in LDS: the maps go to
+// the slots with the smallest packed maps while they fit, the bits to the rest.
+bool scan_variant_lds_maps(int v);
+void assign_lds_maps(ScanArgs* A, int variant, unsigned nprof_cap);
 size_t shared_scan_lds(int variant, unsigned nprof_cap);
 void prepare_shared_scan();  // LDS limits of every variant; call before any capture
 void launch_shared_scan(const ScanArgs* d_args, int variant, unsigned nprof_cap, int grid, hipStream_t s);

@@ -3,6 +3,7 @@
 // row is read once for all queries of the batch, and batched ordered group emission.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "batch.h"
@@ -163,9 +164,9 @@ __device__ __forceinline__ M lds_mask(const unsigned long long* pm, unsigned i) 
 // Profiles of a slot for the four rows of a lane.  A key without a dimension row
 // maps to profile P, whose mask is 0; rows that no joining query wants read entry
 // `range`, which holds P as well, so the four loads issue together unconditionally.
-template <typename M>
+template <typename M, bool LMAP>
 __device__ __forceinline__ void slot_lookup(const Slot& S, const i32x4 (&x)[kMaxCols], const M (&gate)[4],
-                                            bool use_bits, unsigned (&c)[4]) {
+                                            bool use_bits, const unsigned* lmaps, unsigned (&c)[4]) {
   const unsigned range = S.range;
   const M joined = (M)S.joined;
   unsigned idx[4];
@@ -174,14 +175,23 @@ __device__ __forceinline__ void slot_lookup(const Slot& S, const i32x4 (&x)[kMax
     const unsigned off = (unsigned)(pick(x, S.fcol, e) - S.kmin);
     idx[e] = (gate[e] & joined) != (M)0 && off < range ? off : range;
   }
-  if (use_bits) {
+  if (LMAP && S.lbits >= 0) {
+    const unsigned* b = lmaps + S.lbits;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) idx[e] = (b[idx[e] >> 5] >> (idx[e] & 31u)) & 1u ? idx[e] : range;
+  } else if (use_bits) {
     unsigned w[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) w[e] = S.bits[idx[e] >> 5];
 #pragma unroll
     for (int e = 0; e < 4; ++e) idx[e] = (w[e] >> (idx[e] & 31u)) & 1u ? idx[e] : range;
   }
-  if (S.narrow) {
+  if (LMAP && S.lmap >= 0) {
+    const unsigned* m = lmaps + S.lmap;
+    const unsigned lg = (unsigned)S.lmap_log2, per = 32u >> lg, emask = (1u << (1u << lg)) - 1u;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[e] = (m[idx[e] >> (5u - lg)] >> ((idx[e] & (per - 1u)) << lg)) & emask;
+  } else if (S.narrow) {
     const uint8_t* pk = (const uint8_t*)S.pkey;
 #pragma unroll
     for (int e = 0; e < 4; ++e) c[e] = pk[idx[e]];
@@ -211,15 +221,43 @@ __device__ __forceinline__ void load_groups(const ScanArgs* __restrict__ A, unsi
   }
 }
 
+// A wave's queue of rows to aggregate: 64 entries of query mask (2 words), profile ids
+// (3 words, two slots each) and metric values (kMaxMetric words), stored field-major.
+constexpr unsigned kQWords = 5 + kMaxMetric;
+// Rows of a wave's group that passing rows must reach to be aggregated in place.
+constexpr unsigned kQueueInline = 32;
+
+// Group v of the columns, or zeros when v is past the tile's last group.
+__device__ __forceinline__ void load_or_zero(const ScanArgs* __restrict__ A, unsigned colmask, long long v,
+                                             long long vend, i32x4 (&x)[kMaxCols]) {
+  if (v < vend) {
+    load_groups(A, colmask, v, x);
+  } else {
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) x[c] = i32x4{0, 0, 0, 0};
+  }
+}
+
 // Work item: a tile of the base table.  Each 128-bit group of the columns any
 // query reads is loaded once; per row, the slice, fact-predicate and slot lookups
-// produce the mask of queries the row passes, and each lane then walks its own
-// (row, query) pairs, accumulating into one LDS hash table keyed by (query, group)
-// that overflows to the global group tables.
-template <int BLOCK, int HBITS, bool PAR, bool PF, typename M>
+// produce the mask of queries the row passes.  Aggregation:
+//   AGG 0: each lane walks its own (row, query) pairs into one LDS hash table keyed
+//          by (query, group) that overflows to the global group tables;
+//   AGG 1: each lane walks the queries any of its four rows passes; the rows of one
+//          group are added up in registers, then one LDS add into the query's dense
+//          per-workgroup table (queries given one by the host) or one add straight
+//          into its HBM table (the others): no probing, no update waits for another.
+//   Q (with AGG 1): a wave whose rows pass few (query, row) tests in a group appends
+//          those rows (query mask, profile ids, metric values) to its LDS queue and
+//          aggregates the queue 64 rows at a time, one row per lane, so selective
+//          batches do not stall the column stream on a wave's one passing row.
+template <int BLOCK, int HBITS, bool PAR, bool PF, typename M, int AGG, bool LMAP, bool Q>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shared_scan(
     const ScanArgs* __restrict__ A) {
   constexpr unsigned kSlots = 1u << HBITS;  // LDS (query, group) accumulator
+  // hash: one key word per slot; dense: one existence bit per slot
+  constexpr bool DENSE = AGG != 0;
+  constexpr unsigned kKeyWords = DENSE ? kSlots / 32u : kSlots;
   extern __shared__ unsigned long long lds64[];
   __shared__ unsigned long long tmask[2];
   __shared__ unsigned cur, used, tcols;
@@ -229,10 +267,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   unsigned long long* hsum = lds64;
   unsigned long long* pmask = hsum + kSlots;
   unsigned* hkey = (unsigned*)(pmask + nprof);
-  unsigned* praw = hkey + kSlots;
+  unsigned* praw = hkey + kKeyWords;
   QDesc* qd = (QDesc*)(praw + ((nprof + 3u) & ~3u));
   unsigned long long* lsegm = (unsigned long long*)(qd + kMaxBatch);
   int* lsegb = (int*)(lsegm + kMaxSegBounds + kMaxCols);
+  unsigned* const wq = (unsigned*)(lsegb + kMaxSegBounds) + (Q ? (threadIdx.x >> 6) * (kQWords * 64u) : 0u);
+  unsigned* lmaps = (unsigned*)(lsegb + kMaxSegBounds) + (Q ? (BLOCK / 64) * (kQWords * 64u) : 0u);
+  unsigned qn = 0u;  // rows in the wave's queue (the same on every lane)
   unsigned long long* const gsum = A->sums;
   unsigned* const gflag = A->flags;
 
@@ -253,10 +294,33 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     for (unsigned i = threadIdx.x; i < nb; i += BLOCK) lsegb[i] = A->segb[i];
     for (unsigned i = threadIdx.x; i < nb + (unsigned)kMaxCols; i += BLOCK) lsegm[i] = A->segm[i];
   }
-  for (unsigned i = threadIdx.x; i < (unsigned)kSlots; i += BLOCK) {
-    hkey[i] = 0u;
-    hsum[i] = 0ull;
+  if (LMAP) {
+    for (int s = 0; s < nslot; ++s) {
+      const Slot& S = A->slot[s];
+      if (S.lmap >= 0) {
+        // entries 0..range (entry range holds P), packed little-end first
+        const unsigned lg = (unsigned)S.lmap_log2, per = 32u >> lg;
+        const unsigned words = (S.range + per) / per;
+        for (unsigned w = threadIdx.x; w < words; w += BLOCK) {
+          unsigned v = 0u;
+          for (unsigned j = 0; j < per; ++j) {
+            const unsigned k = w * per + j;
+            const unsigned e = k > S.range ? 0u
+                               : S.narrow  ? (unsigned)((const uint8_t*)S.pkey)[k]
+                                           : (unsigned)((const uint16_t*)S.pkey)[k];
+            v |= e << (j << lg);
+          }
+          lmaps[S.lmap + w] = v;
+        }
+      }
+      if (S.lbits >= 0) {
+        const unsigned words = S.range / 32u + 1u, have = 2u * ((S.range + 63u) / 64u);
+        for (unsigned w = threadIdx.x; w < words; w += BLOCK) lmaps[S.lbits + w] = w < have ? S.bits[w] : 0u;
+      }
+    }
   }
+  for (unsigned i = threadIdx.x; i < (unsigned)kSlots; i += BLOCK) hsum[i] = 0ull;
+  for (unsigned i = threadIdx.x; i < kKeyWords; i += BLOCK) hkey[i] = 0u;
   if (threadIdx.x == 0) used = 0u;
   // a scalar query yields its one row even when no row passes
   if (blockIdx.x == 0 && (int)threadIdx.x < nq && A->qd[threadIdx.x].K == 0)
@@ -274,6 +338,47 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         hsum[i] = 0ull;
       }
     }
+  };
+
+  // the first cnt rows of the wave's queue, one per lane
+  auto queue_flush = [&](unsigned cnt) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const unsigned l = threadIdx.x & 63u;
+    if (l < cnt) {
+      unsigned long long mask = wq[l];
+      if (sizeof(M) == 8) mask |= (unsigned long long)wq[64u + l] << 32;
+      const unsigned cw[3] = {wq[128u + l], wq[192u + l], wq[256u + l]};
+      int mv[kMaxMetric];
+#pragma unroll
+      for (int k = 0; k < kMaxMetric; ++k) mv[k] = (int)wq[(5u + (unsigned)k) * 64u + l];
+      for (M u = (M)mask; u; u &= u - (M)1) {
+        const unsigned q = (unsigned)first_bit(u);
+        const QDesc& d = qd[q];
+        unsigned gid = 0u;
+        for (int k = 0; k < d.K; ++k) {
+          const unsigned kw = d.key[k], sl = (kw >> 13) & 7u;
+          const unsigned w = sl < 2u ? cw[0] : sl < 4u ? cw[1] : cw[2];
+          const unsigned c = (w >> ((sl & 1u) << 4)) & 0xFFFFu;
+          gid += ((praw[(kw & 8191u) + c] >> ((kw >> 16) & 31u)) & ((1u << ((kw >> 21) & 31u)) - 1u)) * d.stride[k];
+        }
+        const unsigned long long m = (unsigned long long)(long long)sel_i(mv, d.metric, nmetric);
+        const unsigned lo = d.dense;
+        if (lo != kNoDense) {
+          const unsigned i = lo + gid;
+          atomicAdd(&hsum[i], m);
+          atomicOr(&hkey[i >> 5], 1u << (i & 31u));
+        } else {
+          atomicAdd(gsum + d.sum_off + gid, m);
+          atomicOr(gflag + d.flag_off + (gid >> 5), 1u << (gid & 31u));
+        }
+      }
+    }
+    // every lane has read its entry before the queue is written again
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
   for (;;) {
@@ -309,18 +414,23 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     const long long hi = min(lo + A->tile_rows, A->row_hi);
     if (active) {
       const long long vend = (hi + 3) >> 2;
-      long long v = (lo >> 2) + threadIdx.x;
+      // The loop runs while any lane of the wave has a group (lane 0 has the
+      // wave's first), so every lane stays active for the wave-wide steps; a lane
+      // past the tile loads nothing and passes no row (left <= 0 below).
+      const long long ln = (long long)(threadIdx.x & 63u);
+      const long long vw0 = (lo >> 2) + (long long)(threadIdx.x & ~63u);
       i32x4 x[kMaxCols];
-      if (v < vend) load_groups(A, colmask, v, x);
-      for (; v < vend; v += BLOCK) {
+      load_or_zero(A, colmask, vw0 + ln, vend, x);
+      for (long long vw = vw0; vw < vend; vw += BLOCK) {
+        const long long v = vw + ln;
         // PF: the next group's loads are issued before this group's lookups
         i32x4 xn[kMaxCols];
-        if (PF) load_groups(A, colmask, v + BLOCK < vend ? v + BLOCK : v, xn);
+        if (PF) load_or_zero(A, colmask, v + BLOCK, vend, xn);
         else {
 #pragma unroll
           for (int c = 0; c < kMaxCols; ++c) xn[c] = x[c];
         }
-        const long long left = hi - v * 4;  // rows of this group inside the tile (>= 1)
+        const long long left = hi - v * 4;  // rows of this group inside the tile (<= 0 past it)
         M pass[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -378,7 +488,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
           for (int s = 0; s < kMaxSlot; ++s) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) cs[s][e] = 0u;
-            if (s < nslot && ((M)A->slot[s].joined & active)) slot_lookup<M>(A->slot[s], x, pass, use_bits, cs[s]);
+            if (s < nslot && ((M)A->slot[s].joined & active)) slot_lookup<M, LMAP>(A->slot[s], x, pass, use_bits, lmaps, cs[s]);
           }
 #pragma unroll
           for (int s = 0; s < kMaxSlot; ++s) {
@@ -391,14 +501,55 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
           for (int s = 0; s < kMaxSlot; ++s) {
             unsigned cs[4] = {0u, 0u, 0u, 0u};
             if (s < nslot && ((M)A->slot[s].joined & active)) {
-              slot_lookup<M>(A->slot[s], x, pass, use_bits, cs);
+              slot_lookup<M, LMAP>(A->slot[s], x, pass, use_bits, lmaps, cs);
               slot_apply<M>(A->slot[s], pmask, cs, pass);
             }
             cid[s][0] = cs[0] | cs[1] << 16;
             cid[s][1] = cs[2] | cs[3] << 16;
           }
         }
-        if (pass[0] | pass[1] | pass[2] | pass[3]) {
+        const M lane_any = pass[0] | pass[1] | pass[2] | pass[3];
+        bool agg_here = lane_any != (M)0;
+        if (Q) {
+          unsigned long long qb[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) qb[e] = __ballot(pass[e] != (M)0);
+          const unsigned n = (unsigned)(__popcll(qb[0]) + __popcll(qb[1]) + __popcll(qb[2]) + __popcll(qb[3]));
+          agg_here = n >= kQueueInline;  // a dense group: aggregated in place
+          if (n != 0u && !agg_here) {
+            if (qn + n > 64u) {
+              queue_flush(qn);
+              qn = 0u;
+            }
+            unsigned base = qn;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              if (pass[e] != (M)0) {
+                const unsigned pos =
+                    base + __builtin_amdgcn_mbcnt_hi((unsigned)(qb[e] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)qb[e], 0u));
+                wq[pos] = (unsigned)pass[e];
+                if (sizeof(M) == 8) wq[64u + pos] = (unsigned)((unsigned long long)pass[e] >> 32);
+                unsigned c[kMaxSlot];
+#pragma unroll
+                for (int sl = 0; sl < kMaxSlot; ++sl) c[sl] = (cid[sl][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                wq[128u + pos] = c[0] | c[1] << 16;
+                wq[192u + pos] = c[2] | c[3] << 16;
+                wq[256u + pos] = c[4] | c[5] << 16;
+#pragma unroll
+                for (int k = 0; k < kMaxMetric; ++k) {
+                  if (k < nmetric) {
+                    const int op = A->mop[k];
+                    const unsigned a = (unsigned)pick(x, A->ma[k], e), b = (unsigned)pick(x, A->mb[k], e);
+                    wq[(5u + (unsigned)k) * 64u + pos] = op == MOP_COL ? a : op == MOP_SUB ? a - b : op == MOP_ADD ? a + b : a * b;
+                  }
+                }
+              }
+              base += (unsigned)__popcll(qb[e]);
+            }
+            qn += n;
+          }
+        }
+        if (agg_here) {
         // distinct metric expressions
         int mval[kMaxMetric][4];
 #pragma unroll
@@ -415,6 +566,61 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         }
+        if (DENSE) {
+          unsigned c01[kMaxSlot], c23[kMaxSlot];
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) {
+            c01[s] = cid[s][0];
+            c23[s] = cid[s][1];
+          }
+          for (M u = pass[0] | pass[1] | pass[2] | pass[3]; u; u &= u - (M)1) {
+            const unsigned q = (unsigned)first_bit(u);
+            const QDesc& d = qd[q];
+            unsigned gid[4] = {0u, 0u, 0u, 0u};
+            for (int k = 0; k < d.K; ++k) {
+              const unsigned kw = d.key[k], st = d.stride[k];
+              const unsigned off = kw & 8191u, sh = (kw >> 16) & 31u, msk = (1u << ((kw >> 21) & 31u)) - 1u;
+              const unsigned w01 = sel<kMaxSlot>(c01, (kw >> 13) & 7u, nslot);
+              const unsigned w23 = sel<kMaxSlot>(c23, (kw >> 13) & 7u, nslot);
+              gid[0] += ((praw[off + (w01 & 0xFFFFu)] >> sh) & msk) * st;
+              gid[1] += ((praw[off + (w01 >> 16)] >> sh) & msk) * st;
+              gid[2] += ((praw[off + (w23 & 0xFFFFu)] >> sh) & msk) * st;
+              gid[3] += ((praw[off + (w23 >> 16)] >> sh) & msk) * st;
+            }
+            bool has[4];
+            long long mv[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              int ms[kMaxMetric];
+#pragma unroll
+              for (int k = 0; k < kMaxMetric; ++k) ms[k] = mval[k][e];
+              has[e] = ((pass[e] >> q) & (M)1) != (M)0;
+              mv[e] = (long long)sel_i(ms, d.metric, nmetric);
+            }
+            // the lane's rows in one group of the query are added up first
+#pragma unroll
+            for (int e = 1; e < 4; ++e)
+#pragma unroll
+              for (int f = 0; f < e; ++f)
+                if (has[e] && has[f] && gid[e] == gid[f]) {
+                  mv[f] += mv[e];
+                  has[e] = false;
+                }
+            const unsigned lo = d.dense;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              if (!has[e]) continue;
+              if (lo != kNoDense) {
+                const unsigned i = lo + gid[e];
+                atomicAdd(&hsum[i], (unsigned long long)mv[e]);
+                atomicOr(&hkey[i >> 5], 1u << (i & 31u));
+              } else {
+                atomicAdd(gsum + d.sum_off + gid[e], (unsigned long long)mv[e]);
+                atomicOr(gflag + d.flag_off + (gid[e] >> 5), 1u << (gid[e] & 31u));
+              }
+            }
+          }
+        } else {
         // this lane's (row, query) pairs
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -459,22 +665,40 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         }
+        }  // hash aggregation
         }  // some row of the lane passes
         // the next group: already in flight (PF) or requested now
 #pragma unroll
         for (int c = 0; c < kMaxCols; ++c) x[c] = xn[c];
-        if (!PF && v + BLOCK < vend) load_groups(A, colmask, v + BLOCK, x);
+        if (!PF) load_or_zero(A, colmask, v + BLOCK, vend, x);
       }
     }
     __syncthreads();
-    if (used > (unsigned)kSlots / 2u) {
+    if (!DENSE && used > (unsigned)kSlots / 2u) {
       flush();
       __syncthreads();
       if (threadIdx.x == 0) used = 0u;
     }
   }
+  if (Q && qn) queue_flush(qn);
   __syncthreads();
-  flush();
+  if (!DENSE) {
+    flush();
+    return;
+  }
+  // dense tables: every group with its existence bit set goes to the HBM tables
+  for (int q = 0; q < nq; ++q) {
+    const unsigned lo = qd[q].dense;
+    if (lo == kNoDense) continue;
+    const unsigned G = qd[q].G;
+    for (unsigned g = threadIdx.x; g < G; g += BLOCK) {
+      const unsigned i = lo + g;
+      if ((hkey[i >> 5] >> (i & 31u)) & 1u) {
+        atomicAdd(gsum + qd[q].sum_off + g, hsum[i]);
+        atomicOr(gflag + qd[q].flag_off + (g >> 5), 1u << (g & 31u));
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------------ emission
@@ -701,17 +925,35 @@ namespace {
 struct Variant {
   int block, hbits;
   bool par, pf, mask32;
+  int agg;
+  bool lmap, queue;
   const void* fn;
 };
-#define SVM_SCAN_VARIANT(B, H, PAR, PF, M32) \
-  { B, H, PAR, PF, M32, (const void*)k_shared_scan<B, H, PAR, PF, std::conditional_t<M32, unsigned, unsigned long long>> }
+#define SVM_SCAN_VARIANT(B, H, PAR, PF, M32, D, L, Q)                                                              \
+  {                                                                                                              \
+    B, H, PAR, PF, M32, D, L, Q,                                                                                 \
+        (const void*)k_shared_scan<B, H, PAR, PF, std::conditional_t<M32, unsigned, unsigned long long>, D, L, Q> \
+  }
+// (threads, log2 LDS accumulator slots, parallel lookups, prefetch, 32-bit masks,
+//  aggregation, LDS maps, queue)
 const Variant kVariants[] = {
-    SVM_SCAN_VARIANT(512, 11, false, false, false), SVM_SCAN_VARIANT(512, 11, false, false, true),
-    SVM_SCAN_VARIANT(512, 11, true, false, false),  SVM_SCAN_VARIANT(512, 11, true, false, true),
-    SVM_SCAN_VARIANT(512, 11, false, true, false),  SVM_SCAN_VARIANT(512, 11, false, true, true),
-    SVM_SCAN_VARIANT(256, 10, false, false, false), SVM_SCAN_VARIANT(256, 10, false, false, true),
+    SVM_SCAN_VARIANT(512, 11, false, false, false, 0, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, false, true, 0, false, false),
+    SVM_SCAN_VARIANT(512, 11, true, false, false, 0, false, false),
+    SVM_SCAN_VARIANT(512, 11, true, false, true, 0, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, true, false, 0, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, true, true, 0, false, false),
+    SVM_SCAN_VARIANT(256, 10, false, false, false, 0, false, false),
+    SVM_SCAN_VARIANT(256, 10, false, false, true, 0, false, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, true),
+    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, true),
 };
 #undef SVM_SCAN_VARIANT
+// dynamic LDS of the variants with LDS maps: the CU's 160 KB less a margin for the
+// kernel's static variables
+constexpr size_t kLdsMapsTotal = 160 * 1024 - 512;
 }  // namespace
 
 int scan_variants() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
@@ -720,14 +962,83 @@ int scan_variant_hash_slots(int v) { return 1 << kVariants[v].hbits; }
 bool scan_variant_parallel(int v) { return kVariants[v].par; }
 bool scan_variant_prefetch(int v) { return kVariants[v].pf; }
 bool scan_variant_mask32(int v) { return kVariants[v].mask32; }
+bool scan_variant_dense(int v) { return kVariants[v].agg != 0; }
+
+void assign_dense(QDesc* qd, int nq, unsigned slots) {
+  // queries with the fewest groups first, while their tables fit
+  int order[kMaxBatch];
+  for (int q = 0; q < nq; ++q) order[q] = q;
+  std::stable_sort(order, order + nq, [&](int a, int b) { return qd[a].G < qd[b].G; });
+  unsigned used = 0;
+  for (int k = 0; k < nq; ++k) {
+    QDesc& d = qd[order[k]];
+    if (d.G <= slots - used) {
+      d.dense = used;
+      used += d.G;
+    } else {
+      d.dense = kNoDense;
+    }
+  }
+}
+
+// LDS of a variant before its map area
+size_t fixed_lds(int variant, unsigned nprof_cap) {
+  const size_t slots = (size_t)1 << kVariants[variant].hbits;
+  const size_t key_words = kVariants[variant].agg ? slots / 32 : slots;
+  const size_t queue = kVariants[variant].queue ? (size_t)kVariants[variant].block * kQWords * 4 : 0;
+  return slots * 8 + key_words * 4 + (size_t)nprof_cap * 12 + (size_t)kMaxBatch * sizeof(QDesc) +
+         (size_t)(kMaxSegBounds + kMaxCols) * 8 + (size_t)kMaxSegBounds * 4 + queue;
+}
 
 size_t shared_scan_lds(int variant, unsigned nprof_cap) {
-  return ((size_t)1 << kVariants[variant].hbits) * 12 + (size_t)nprof_cap * 12 + (size_t)kMaxBatch * sizeof(QDesc) +
-         (size_t)(kMaxSegBounds + kMaxCols) * 8 + (size_t)kMaxSegBounds * 4;
+  return kVariants[variant].lmap ? kLdsMapsTotal : fixed_lds(variant, nprof_cap);
+}
+
+bool scan_variant_lds_maps(int v) { return kVariants[v].lmap; }
+
+void assign_lds_maps(ScanArgs* A, int variant, unsigned nprof_cap) {
+  for (int s = 0; s < A->nslot; ++s) {
+    A->slot[s].lmap = -1;
+    A->slot[s].lmap_log2 = 0;
+    A->slot[s].lbits = -1;
+  }
+  if (!kVariants[variant].lmap) return;
+  const size_t fixed = fixed_lds(variant, nprof_cap);
+  size_t left = kLdsMapsTotal > fixed ? (kLdsMapsTotal - fixed) / 4 : 0;  // words
+  size_t used = 0;
+  int order[kMaxSlot];
+  size_t map_words[kMaxSlot];
+  int lg[kMaxSlot];
+  for (int s = 0; s < A->nslot; ++s) {
+    order[s] = s;
+    const uint64_t entries = (uint64_t)A->slot[s].range + 1;  // entry range holds P
+    lg[s] = A->slot[s].P + 1 <= 16 ? 2 : A->slot[s].P + 1 <= 256 ? 3 : 4;
+    const uint64_t per = 32u >> lg[s];
+    map_words[s] = (size_t)((entries + per - 1) / per);
+  }
+  std::stable_sort(order, order + A->nslot, [&](int a, int b) { return map_words[a] < map_words[b]; });
+  for (int k = 0; k < A->nslot; ++k) {
+    Slot& S = A->slot[order[k]];
+    if (map_words[order[k]] <= left - used) {
+      S.lmap = (int)used;
+      S.lmap_log2 = lg[order[k]];
+      used += map_words[order[k]];
+    }
+  }
+  for (int k = 0; k < A->nslot; ++k) {
+    Slot& S = A->slot[order[k]];
+    const size_t words = (size_t)S.range / 32 + 1;
+    if (S.lmap < 0 && words <= left - used) {
+      S.lbits = (int)used;
+      used += words;
+    }
+  }
 }
 
 void prepare_shared_scan() {
   static const bool done = [] {
+    if (kDefaultVariantMask32 >= scan_variants() || !kVariants[kDefaultVariantMask32].mask32)
+      fail(SVM_ERR_STATE, "the default shared-scan variant for 32-bit masks has 64-bit masks");
     for (int v = 0; v < scan_variants(); ++v)
       hip_check(hipFuncSetAttribute(kVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)shared_scan_lds(v, kMaxProfiles)),

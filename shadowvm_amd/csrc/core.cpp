@@ -24,10 +24,20 @@ void hip_check(hipError_t e, const char* what) {
 // returned to a free list on release instead of to the driver (hipFree
 // synchronises the device, which would stall every stream in flight); larger ones
 // (resident fact columns) go straight to hipMalloc / hipFree.
+// Its state is created on first use and never destroyed: caches in other files
+// (profiles, dictionaries, ...) are static objects holding device buffers, and
+// their destructors release those buffers here at process exit, in whatever order
+// the linker gave the files' static objects.
 namespace {
-std::mutex g_alloc_mu;
-std::map<uint64_t, std::vector<void*>> g_free_lists;  // bucket size -> free blocks
-std::map<void*, uint64_t> g_bucket_of;                // live cached block -> bucket size
+struct AllocState {
+  std::mutex mu;
+  std::map<uint64_t, std::vector<void*>> free_lists;  // bucket size -> free blocks
+  std::map<void*, uint64_t> bucket_of;                // live cached block -> bucket size
+};
+AllocState& alloc_state() {
+  static AllocState* const s = new AllocState();
+  return *s;
+}
 constexpr uint64_t kCacheLimit = 1ull << 30;
 
 uint64_t bucket_for(uint64_t b) {
@@ -76,30 +86,32 @@ void* dev_alloc(uint64_t bytes) {
   noalloc_refuse("a device buffer of " + std::to_string(bytes) + " bytes");
   if (bytes >= kCacheLimit) return raw_alloc(bytes);
   uint64_t b = bucket_for(bytes);
+  AllocState& st = alloc_state();
   {
-    std::lock_guard<std::mutex> g(g_alloc_mu);
-    auto& fl = g_free_lists[b];
+    std::lock_guard<std::mutex> g(st.mu);
+    auto& fl = st.free_lists[b];
     if (!fl.empty()) {
       void* p = fl.back();
       fl.pop_back();
-      g_bucket_of[p] = b;
+      st.bucket_of[p] = b;
       return p;
     }
   }
   void* p = raw_alloc(b);
-  std::lock_guard<std::mutex> g(g_alloc_mu);
-  g_bucket_of[p] = b;
+  std::lock_guard<std::mutex> g(st.mu);
+  st.bucket_of[p] = b;
   return p;
 }
 
 void dev_free(void* p) {
   if (!p) return;
+  AllocState& st = alloc_state();
   {
-    std::lock_guard<std::mutex> g(g_alloc_mu);
-    auto it = g_bucket_of.find(p);
-    if (it != g_bucket_of.end()) {
-      g_free_lists[it->second].push_back(p);
-      g_bucket_of.erase(it);
+    std::lock_guard<std::mutex> g(st.mu);
+    auto it = st.bucket_of.find(p);
+    if (it != st.bucket_of.end()) {
+      st.free_lists[it->second].push_back(p);
+      st.bucket_of.erase(it);
       return;
     }
   }
