@@ -47,11 +47,16 @@ SVM_FN unsigned StrHash(const char* p, int n) {
   for (int k = 0; k < n && p[k] != '\0'; ++k) h = h * 31u + (unsigned)(int)(signed char)p[k];
   return h & 0x7FFFFFFFu;
 }
-/* Prefix equality: 0 once either side reaches NUL, else the first byte difference
-   (com/cpp/cuda-runtime.cu:34-54).  Up to 64 bytes (the widths of fixed string columns;
-   every operand is a row of a padded column or a zero-tailed literal) all bytes are
-   read first and the answer is selected without branches, so the loads issue together;
-   the result is the same as the early-exit loop's. */
+)SVMRT";
+
+// StrCmp, prefix equality (com/cpp/cuda-runtime.cu:34-54): 0 once either side
+// reaches NUL, else the first byte difference.  Every operand is a row of a padded
+// column or a zero-tailed literal, so up to 64 bytes (the widths of fixed string
+// columns) both sides are read in full before the answer is selected without
+// branches, and the result is the same as the early-exit loop's.  The host engine
+// reads them byte by byte; the kernels read whole aligned 32-bit words and take each
+// operand's bytes out of word pairs (v_alignbyte), a quarter of the loads.
+static const char* kHostStrCmp = R"SVMRT(
 SVM_FN int StrCmp(const char* a, const char* b, int n) {
   if (n <= 64) {
     int r = 0;
@@ -74,6 +79,47 @@ SVM_FN int StrCmp(const char* a, const char* b, int n) {
   }
   return 0;
 }
+)SVMRT";
+
+static const char* kDeviceStrCmp = R"SVMRT(
+SVM_FN int StrCmp(const char* a, const char* b, int n) {
+  if (n <= 64) {
+    const svm_u64 pa = (svm_u64)a, pb = (svm_u64)b;
+    const unsigned* wa = (const unsigned*)(pa & ~3ull);
+    const unsigned* wb = (const unsigned*)(pb & ~3ull);
+    const unsigned sa = (unsigned)(pa & 3ull), sb = (unsigned)(pb & 3ull);
+    int r = 0;
+    bool done = false;
+    unsigned ca = wa[0], cb = wb[0];
+    #pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      if (4 * m >= n) break;
+      const unsigned na = wa[m + 1], nb = wb[m + 1];
+      const unsigned xa = __builtin_amdgcn_alignbyte(na, ca, sa), xb = __builtin_amdgcn_alignbyte(nb, cb, sb);
+      ca = na;
+      cb = nb;
+      #pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (4 * m + j >= n) break;
+        const int x = (int)(signed char)(xa >> (8 * j)), y = (int)(signed char)(xb >> (8 * j));
+        const int d = x - y;
+        const bool stop = (x == 0) | (y == 0);
+        r = (!done && !stop && d != 0) ? d : r;
+        done = done | stop | (d != 0);
+      }
+    }
+    return r;
+  }
+  for (int k = 0; k < n; ++k) {
+    char x = a[k], y = b[k];
+    if (x == '\0' || y == '\0') return 0;
+    if (x != y) return (int)(signed char)x - (int)(signed char)y;
+  }
+  return 0;
+}
+)SVMRT";
+
+static const char* kCommonRT2 = R"SVMRT(
 SVM_FN void StrCopy(char* d, const char* s, int n) {
   for (int k = 0; k < n; ++k) d[k] = s[k];
 }
@@ -148,9 +194,12 @@ __device__ __forceinline__ void svm_str_copy(char* d, const char* s, int n) {
 )SVMRT";
 
 std::string device_runtime_text() {
-  return std::string("#define SVM_FN __device__ __forceinline__\n") + kCommonRT + kDeviceOnlyRT;
+  return std::string("#define SVM_FN __device__ __forceinline__\n") + kCommonRT + kDeviceStrCmp + kCommonRT2 +
+         kDeviceOnlyRT;
 }
-std::string host_runtime_text() { return std::string("#define SVM_FN static inline\n") + kCommonRT; }
+std::string host_runtime_text() {
+  return std::string("#define SVM_FN static inline\n") + kCommonRT + kHostStrCmp + kCommonRT2;
+}
 
 namespace {
 

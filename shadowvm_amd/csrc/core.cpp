@@ -183,8 +183,12 @@ ScanState scan_state(hipStream_t s, uint64_t tiles) {
   ScanBuf& b = g_scan_bufs[s];
   if (b.cap < tiles) {
     noalloc_refuse("a larger ordered-compaction status buffer");
-    // runs on a stream are sequential and synchronised, so the old buffer is idle
-    if (b.buf) (void)hipFree(b.buf);
+    // an earlier run on the stream may still be compacting (a select whose row count
+    // is pending): the old buffer is freed once the stream is idle
+    if (b.buf) {
+      SVM_HIP(hipStreamSynchronize(s));
+      (void)hipFree(b.buf);
+    }
     uint64_t cap = 4096;
     while (cap < tiles) cap <<= 1;
     b.buf = (unsigned long long*)raw_alloc((cap + 1) * sizeof(unsigned long long));
@@ -239,6 +243,54 @@ void scan_state_reset(hipStream_t s) {
   if (it != g_scan_bufs.end()) it->second.dirty = true;
 }
 
+// The pinned word pool: one mapped allocation of kPinnedEntries x 4 words, made on
+// first use and freed with the engine's other per-stream state.
+// Entries are numbered with the pool's generation, so one taken from a pool since
+// freed is never given to its successor.
+constexpr int kPinnedEntries = 1024;
+static volatile unsigned long long* g_pin_host = nullptr;
+static unsigned long long* g_pin_dev = nullptr;
+static std::vector<int> g_pin_free;
+static int g_pin_gen = 0;
+
+bool pinned_words_take(PinnedWords& w) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  if (noalloc_active()) return false;
+  if (!g_pin_host) {
+    void* h = nullptr;
+    void* d = nullptr;
+    if (hipHostMalloc(&h, kPinnedEntries * 4 * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipHostFree(h);
+      return false;
+    }
+    g_pin_host = (volatile unsigned long long*)h;
+    g_pin_dev = (unsigned long long*)d;
+    g_pin_gen = (g_pin_gen + 1) & 0x3FFFF;
+    g_pin_free.clear();
+    for (int i = kPinnedEntries - 1; i >= 0; --i) g_pin_free.push_back(i);
+  }
+  if (g_pin_free.empty()) return false;
+  const int i = g_pin_free.back();
+  g_pin_free.pop_back();
+  w.slot = g_pin_gen << 12 | i;
+  w.host = g_pin_host + 4 * i;
+  w.dev = g_pin_dev + 4 * i;
+  for (int k = 0; k < 4; ++k) w.host[k] = 0ull;
+  return true;
+}
+
+void pinned_words_give(int slot) {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  if (g_pin_host && (slot >> 12) == g_pin_gen) g_pin_free.push_back(slot & 4095);
+}
+
 HostSlots host_slots(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   auto it = g_host_slots.find(s);
@@ -265,6 +317,10 @@ void arena_release_all() {
   g_host_slots.clear();
   for (auto& kv : g_scan_bufs) (void)hipFree(kv.second.buf);
   g_scan_bufs.clear();
+  if (g_pin_host) (void)hipHostFree((void*)g_pin_host);
+  g_pin_host = nullptr;
+  g_pin_dev = nullptr;
+  g_pin_free.clear();
   for (auto& kv : g_staging) (void)hipHostFree(kv.second.p);
   g_staging.clear();
 }
@@ -285,8 +341,37 @@ void dev_free_async(void* p, hipStream_t s) {
 }
 
 Table::~Table() {
+  // a launch may still write the columns
+  if (auto p = std::atomic_load(&pending)) p->wait();
   for (auto& c : cols)
     if (c.owns) dev_free(c.dev);
+}
+
+void PendingRows::wait() {
+  if (done) (void)hipEventSynchronize(done);
+}
+
+PendingRows::~PendingRows() {
+  wait();
+  if (done) (void)hipEventDestroy(done);
+  pinned_words_give(slot);
+}
+
+void resolve_rows(Table& t) {
+  std::shared_ptr<PendingRows> p = std::atomic_load(&t.pending);
+  if (!p) return;
+  std::lock_guard<std::mutex> g(p->mu);
+  if (std::atomic_load(&t.pending) != p) return;  // resolved by another thread meanwhile
+  if (p->done) SVM_HIP(hipEventSynchronize(p->done));
+  const bool lost = p->host[3] != 0ull;
+  const uint64_t n = lost ? 0 : (uint64_t)p->host[0];
+  t.rows = n;
+  t.bulk0 = n;
+  std::atomic_store(&t.pending, std::shared_ptr<PendingRows>());
+  if (lost) {
+    scan_state_reset(p->stream);
+    fail(SVM_ERR_DEVICE, "select '" + t.id + "': the ordered compaction lost a tile status");
+  }
 }
 
 TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t rows) {
@@ -348,16 +433,27 @@ void Store::put(const std::string& id, TablePtr t) {
 }
 
 TablePtr Store::get(const std::string& id) {
-  std::lock_guard<std::mutex> g(mu_);
-  auto it = tables_.find(id);
-  if (it == tables_.end()) fail(SVM_ERR_NOT_FOUND, "no table or pipeline output with id '" + id + "'");
-  return it->second;
+  TablePtr t;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = tables_.find(id);
+    if (it == tables_.end()) fail(SVM_ERR_NOT_FOUND, "no table or pipeline output with id '" + id + "'");
+    t = it->second;
+  }
+  resolve_rows(*t);
+  return t;
 }
 
 TablePtr Store::find(const std::string& id) {
-  std::lock_guard<std::mutex> g(mu_);
-  auto it = tables_.find(id);
-  return it == tables_.end() ? nullptr : it->second;
+  TablePtr t;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = tables_.find(id);
+    if (it == tables_.end()) return nullptr;
+    t = it->second;
+  }
+  resolve_rows(*t);
+  return t;
 }
 
 bool Store::drop(const std::string& id) {

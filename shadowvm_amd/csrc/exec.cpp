@@ -1570,44 +1570,65 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
       }
     }
   };
+  // An output sized for every source row whose count no one reads yet (the result is
+  // not returned) is left pending: the count lands in pinned words of its own, and
+  // the store waits for it when the table is next looked up, so the host prepares
+  // the following pipelines while this one runs.
+  PinnedWords pw;
+  const bool defer = capacity && !opt.want_host && !opt.noalloc && pinned_words_take(pw);
+  std::shared_ptr<PendingRows> pend;
+  if (defer) {
+    pend = std::make_shared<PendingRows>();
+    pend->host = pw.host;
+    pend->slot = pw.slot;
+    pend->stream = c.s;
+  }
+  volatile unsigned long long* const count_host = defer ? pw.host : slots.host;
+  unsigned long long* const count_dev = defer ? pw.dev : slots.dev;
+  auto finish_count = [&]() {
+    if (defer) {
+      SVM_HIP(hipEventCreateWithFlags(&pend->done, hipEventDisableTiming));
+      SVM_HIP(hipEventRecord(pend->done, c.s));
+      std::atomic_store(&out->pending, pend);
+      return;
+    }
+    SVM_HIP(hipStreamSynchronize(c.s));
+    if (count_host[3]) {
+      scan_state_reset(c.s);
+      fail(SVM_ERR_DEVICE, "select '" + c.info.id + "': the ordered compaction lost a tile status");
+    }
+    total = count_host[0];
+    // the passing rows (and bulk 0: the source is one bulk)
+    out->rows = total;
+    out->bulk0 = total;
+  };
   if (single) {
     src = select_source(ks);
     hipFunction_t fsingle = jit_function(src->text, src->hash, "svm_sel_single");
     out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
     bind_outputs();
     const ScanState st = scan_state(c.s, tiles);
-    a.out[5] = slots.dev + 0;
+    a.out[5] = count_dev;
     a.out[7] = st.buf;
     a.iv[46] = (int)st.epoch;
-    slots.host[3] = 0;
+    count_host[3] = 0;
     launch(fsingle, (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
-    SVM_HIP(hipStreamSynchronize(c.s));
-    if (slots.host[3]) {
-      scan_state_reset(c.s);
-      fail(SVM_ERR_DEVICE, "select '" + c.info.id + "': the ordered compaction lost a tile status");
-    }
-    total = slots.host[0];
-    // the passing rows (and bulk 0: the source is one bulk)
-    out->rows = total;
-    out->bulk0 = total;
+    finish_count();
   } else if (capacity) {
     // count, scan and write back to back into an output sized for every source row,
-    // then one synchronisation for the row count (the write's last tile ends the
-    // output with its padding)
+    // then the row count (the write's last tile ends the output with its padding)
     src = select_source(ks);
     DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
     a.out[6] = offs.p;
     a.out[7] = counts.p;
     out = c.E.store.create(c.info.id, oschema, rows, &mat, c.s, /*zero_pad=*/false);
     bind_outputs();
+    count_host[3] = 0;
     launch(jit_function(src->text, src->hash, "svm_sel_count"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
     k::exclusive_scan_u32(counts.as<uint32_t>(), tiles, offs.as<unsigned long long>(),
-                          offs.as<unsigned long long>() + tiles, c.s, slots.dev + 0);
+                          offs.as<unsigned long long>() + tiles, c.s, count_dev + 0);
     launch(jit_function(src->text, src->hash, "svm_sel_write"), (unsigned)tiles, (unsigned)ks.block, (unsigned)(lds_words * 4), c.s, a);
-    SVM_HIP(hipStreamSynchronize(c.s));
-    total = slots.host[0];
-    out->rows = total;
-    out->bulk0 = total;
+    finish_count();
   } else {
     DevBuf counts(tiles * 4 + 4, c.s), offs(tiles * 8 + 16, c.s);
     a.out[6] = offs.p;
@@ -1655,6 +1676,14 @@ std::unique_ptr<svm_result> run_select(Ctx& c, const RunOptions& opt) {
   res->kind = 0;
   res->rows = (int64_t)total;
   for (int q = 0; q < P; ++q) res->types.push_back(oschema[q]);
+  if (defer) {
+    // the result is not returned; its timings would need the launch to finish
+    c.E.store.put(c.info.id, out);
+    if (c.E.cfg.verbose)
+      fprintf(stderr, "[svm] %s: select host %.3f ms; rows pending (capacity %llu)\n", c.info.id.c_str(),
+              host_ms(t_host0), (unsigned long long)rows);
+    return res;
+  }
   if (opt.want_host)
     copy_out(*out, oschema, P, total, c.s, res->cols);
   else
@@ -1724,8 +1753,9 @@ std::unique_ptr<svm_result> run_pipeline(const PipelineInfo& info, const RunOpti
     return run_load(info, plan, opt);
   }
   Ctx c{E, plan, info, opt.stream ? opt.stream : E.stream, nullptr, {}, {}};
-  // runs on one stream are sequential and each ends synchronised, so the previous
-  // run's temporaries on this stream are free
+  // runs on one stream are sequential: the previous run's temporaries on this stream
+  // are reused only by this run's work on the same stream (a select that left its row
+  // count pending may still be running there)
   arena_reset(c.s);
   resolve(c);
   if (plan.is_reduce && !plan.select_exp.empty())

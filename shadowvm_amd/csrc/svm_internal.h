@@ -81,6 +81,19 @@ struct Column {
   int origin_col = -1;
   uint64_t origin_version = 0;  // origin's data_version() when the projection was made
 };
+// A select output whose row count a launch is still computing.  The kernels write the
+// count (and the one-launch compaction's lost-status flag) into pinned host words;
+// until resolve_rows() has waited for the launch, the table's rows are its capacity.
+struct PendingRows {
+  hipEvent_t done = nullptr;
+  volatile unsigned long long* host = nullptr;  // [0] rows; [3] nonzero: a tile status was lost
+  int slot = -1;                                 // its words in the pinned pool
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  void wait();     // returns once the launch has finished
+  ~PendingRows();  // waits, then gives back the event and the words
+};
+
 struct Table {
   std::string id;
   uint64_t rows = 0;
@@ -109,8 +122,24 @@ struct Table {
   const char* col_ptr(int c) const {
     return cols[c].dev ? (const char*)cols[c].dev + row0 * (uint64_t)cols[c].size : nullptr;
   }
+  // Set while a launch still counts the rows (accessed with std::atomic_load/store);
+  // the store resolves it before it hands the table out.
+  std::shared_ptr<PendingRows> pending;
   ~Table();
 };
+
+// Waits for a pending table's row count and sets rows and bulk 0 (a no-op otherwise).
+void resolve_rows(Table& t);
+
+// Pinned host words the device writes and the host reads without a copy: four per
+// entry, zeroed when taken.  take() returns false when every entry is in use.
+struct PinnedWords {
+  volatile unsigned long long* host = nullptr;
+  unsigned long long* dev = nullptr;
+  int slot = -1;
+};
+bool pinned_words_take(PinnedWords& w);
+void pinned_words_give(int slot);
 
 // Zero-copy view of rows [row0, row0+rows) of `base` (row0 relative to base).
 TablePtr make_view(const std::string& id, TablePtr base, uint64_t row0, uint64_t rows);
