@@ -633,8 +633,8 @@ struct Scratch {
   std::vector<DevBuf> bufs;
   void* host = nullptr;  // pinned staging for descriptors
   size_t host_bytes = 0;
-  // (variant, LDS profiles, grid, drop check)
-  std::map<std::tuple<int, unsigned, int, bool>, hipGraphExec_t> graphs;
+  // (variant, LDS profiles, grid, drop check, key bits)
+  std::map<std::tuple<int, unsigned, int, bool, bool>, hipGraphExec_t> graphs;
   void* get(size_t slot, uint64_t bytes) {
     if (bufs.size() <= slot) bufs.resize(slot + 1);
     if (bufs[slot].bytes < bytes) bufs[slot] = DevBuf(bytes + bytes / 4 + 256);
@@ -1212,6 +1212,9 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   if (bk::scan_variant_dense(ch.variant)) bk::assign_dense(qd, nq, bk::scan_variant_hash_slots(ch.variant));
   bk::assign_lds_maps(A, ch.variant, cap);
   const int grid = scan_grid(E, ch.variant, cap);
+  // the key bits are built only when the scan reads them
+  bool need_bits = A->use_bits != 0;
+  for (int k = 0; k < A->nslot; ++k) need_bits = need_bits || A->slot[k].lbits >= 0;
   // the pass: upload descriptors, then clear -> masks -> scan -> emission count
   hipEvent_t e0, e1, eb, es;
   SVM_HIP(hipEventCreate(&e0));
@@ -1222,14 +1225,14 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   SVM_HIP(hipEventRecord(e0, s));
   const bool graph = !env_off("SVM_BATCH_GRAPH");
   if (graph) {
-    hipGraphExec_t& ge = sc.graphs[std::make_tuple(ch.variant, cap, grid, nchk > 0)];
+    hipGraphExec_t& ge = sc.graphs[std::make_tuple(ch.variant, cap, grid, nchk > 0, need_bits)];
     if (!ge) {
       hipGraph_t g = nullptr;
       SVM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       bk::launch_pass_clear(d_args, s);
       bk::launch_profile_masks(d_margs, s);
       if (nchk) bk::launch_drop_check(d_args, s);
-      bk::launch_slot_bits(d_args, s);
+      if (need_bits) bk::launch_slot_bits(d_args, s);
       bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
       bk::launch_emit_count(d_emit, d_args, counts, offsets, totals, s);
       SVM_HIP(hipStreamEndCapture(s, &g));
@@ -1241,7 +1244,7 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     bk::launch_pass_clear(d_args, s);
     bk::launch_profile_masks(d_margs, s);
     if (nchk) bk::launch_drop_check(d_args, s);
-    bk::launch_slot_bits(d_args, s);
+    if (need_bits) bk::launch_slot_bits(d_args, s);
     SVM_HIP(hipEventRecord(eb, s));
     bk::launch_shared_scan(d_args, ch.variant, cap, grid, s);
     SVM_HIP(hipEventRecord(es, s));
@@ -1699,7 +1702,6 @@ void worker_main(Server* S, int index) {
   hipStream_t s = nullptr;
   SVM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   Scratch sc;
-  (void)index;
   for (;;) {
     std::vector<std::shared_ptr<Ticket>> batch;
     uint64_t need = 0;
@@ -1752,6 +1754,7 @@ void worker_main(Server* S, int index) {
       const auto now = std::chrono::steady_clock::now();
       for (auto& t : batch) t->admitted = now;
     }
+    const auto t_run0 = std::chrono::steady_clock::now();
     try {
       if (alone) {
         run_plain_query(*batch[0], s);
@@ -1772,6 +1775,17 @@ void worker_main(Server* S, int index) {
       for (auto& t : batch) { t->status = e.code; t->error = e.what(); }
     } catch (const std::exception& e) {
       for (auto& t : batch) { t->status = SVM_ERR_DEVICE; t->error = e.what(); }
+    }
+    if (E.cfg.verbose) {
+      int light = 0;
+      uint64_t rows = 0;
+      for (auto& t : batch) {
+        light += t->light ? 1 : 0;
+        rows += t->rows;
+      }
+      fprintf(stderr, "[svm] server worker %d: %s of %zu queries (%d light), %llu slice rows, %.3f ms\n", index,
+              alone ? "plain run" : "shared scan", batch.size(), light, (unsigned long long)rows,
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_run0).count());
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);

@@ -87,16 +87,20 @@ __global__ void k_string_exact(const char* __restrict__ col, int size, uint64_t 
   }
 }
 
-// One thread per (slot, profile); the grid covers kMaxSlot x kMaxProfiles so the
-// launch does not depend on the batch.
+// One thread per (slot, profile, run of kTermsPerThread terms), ORing its queries
+// into the profile's mask (cleared by k_pass_clear); the grid covers kMaxSlot x
+// kMaxProfiles x kMaxTerms so the launch does not depend on the batch.
+constexpr int kTermsPerThread = 8;
 __global__ void k_profile_masks(const MaskArgs* __restrict__ M) {
   if ((int)blockIdx.y >= M->nslot) return;
   const MaskSlot& S = M->slots[blockIdx.y];
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= S.P) return;
+  const int t0 = S.t0 + (int)blockIdx.z * kTermsPerThread;
+  const int t1 = min(S.t1, t0 + kTermsPerThread);
+  if (p >= S.P || t0 >= t1) return;
   const unsigned raw = (unsigned)S.praw[p];
   unsigned long long m = 0ull;
-  for (int t = S.t0; t < S.t1; ++t) {
+  for (int t = t0; t < t1; ++t) {
     const MaskTerm& T = M->terms[t];
     bool ok = true;
     for (int k = 0; k < T.natom && ok; ++k) {
@@ -113,7 +117,7 @@ __global__ void k_profile_masks(const MaskArgs* __restrict__ M) {
     }
     if (ok) m |= 1ull << T.q;
   }
-  S.mask[p] = m;
+  if (m) atomicOr(S.mask + p, m);
 }
 
 // --------------------------------------------------------------- shared scan
@@ -720,6 +724,11 @@ __global__ void k_pass_clear(const ScanArgs* __restrict__ A) {
   unsigned* flags = A->flags;
   for (uint64_t i = t0; i < A->sum_words; i += stride) sums[i] = 0ull;
   for (uint64_t i = t0; i < A->flag_words; i += stride) flags[i] = 0u;
+  // the profile masks k_profile_masks ORs into
+  for (int s = 0; s < A->nslot; ++s) {
+    unsigned long long* m = (unsigned long long*)A->slot[s].mask;
+    for (uint64_t i = t0; i < A->slot[s].P; i += stride) m[i] = 0ull;
+  }
   if (t0 == 0) {
     *A->counter = 0u;
     if (A->dropq) *A->dropq = 0ull;
@@ -907,7 +916,8 @@ void launch_pass_clear(const ScanArgs* d_args, hipStream_t s) {
 }
 
 void launch_profile_masks(const MaskArgs* d_margs, hipStream_t s) {
-  hipLaunchKernelGGL(k_profile_masks, dim3(kMaxProfiles / kBlock, kMaxSlot), dim3(kBlock), 0, s, d_margs);
+  hipLaunchKernelGGL(k_profile_masks, dim3(kMaxProfiles / kBlock, kMaxSlot, kMaxTerms / kTermsPerThread), dim3(kBlock),
+                     0, s, d_margs);
   hip_check(hipGetLastError(), "k_profile_masks");
 }
 
