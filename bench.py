@@ -194,6 +194,50 @@ def bench_batched(wl, nq, seed, reps=3, batch_runs=30):
                         "completion, over batch_runs runs; plain latency = each query's own time)"}
 
 
+def bench_batched_heavy(wl, nq, seed, reps=3, batch_runs=10):
+    """nq heavy queries of the serving mix (the four-way star join over the whole
+    fact table, seeded constants) as one svm_batch_run against the same queries run
+    one after another through the plain per-query kernels."""
+    from shadowvm_amd import pipeline as P
+    from shadowvm_amd import ssb
+    from shadowvm_amd.proto import serialize
+    qs = ssb.serve_queries(wl, nq, seed=seed, light_frac=0.0, prefix="hv")
+    ssb.make_views(qs)
+    per_query = [[serialize(p) for p in q.plans()] for q in qs]
+    plans, pos = [], []
+    for qp in per_query:
+        plans += qp
+        pos.append(len(plans) - 1)
+
+    def run_plain():
+        res = []
+        for qp in per_query:
+            for d in qp[:-1]:
+                P.run_plan(d, want_result=False)
+            res.append(P.run_plan(qp[-1]))
+        return res
+
+    run_plain()
+    P.batch_run(plans)  # warm: caches, code objects and the autotuned shape
+    plain_t, batch_t = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        pres = run_plain()
+        plain_t.append(time.perf_counter() - t0)
+    for _ in range(batch_runs):
+        t0 = time.perf_counter()
+        bres, fused = P.batch_run(plans)
+        batch_t.append(time.perf_counter() - t0)
+    exact = all(_same(bres[p], r) for p, r in zip(pos, pres))
+    tp, tb = sorted(plain_t)[len(plain_t) // 2], sorted(batch_t)[len(batch_t) // 2]
+    log(f"[batch] {nq} heavy queries: plain {tp*1e3:.2f} ms, batched {tb*1e3:.2f} ms ({fused} fused), "
+        f"bit-exact={exact}")
+    return {"queries": nq, "fused": fused, "bit_exact_vs_plain": exact, "batched_ms": tb * 1e3,
+            "plain_ms": tp * 1e3, "speedup": tp / tb, "batch_runs": batch_runs, "plain_runs": reps,
+            "workload": "the serving mix's heavy queries: the four-way star join over all fact rows with "
+                        "seeded year / customer nation / supplier nation constants"}
+
+
 def _token(i, width):
     """A unique ASCII id of exactly `width` characters for submission i."""
     digits = "0123456789abcdefghijklmnopqrstuvwxyz"
@@ -363,6 +407,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter-calibration copy kernel")
     ap.add_argument("--batch-queries", type=int, default=64)
+    ap.add_argument("--batch-heavy", type=int, default=8,
+                    help="also batch this many whole-table four-way queries (0: skip)")
     ap.add_argument("--serve-rates", default="50,200,500,1000,2000,3000,4000,6000,8000,12000,16000",
                     help="offered query rates (q/s), swept upward per mode")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -481,6 +527,8 @@ def main():
             f"({rows / (hms / 1e3):.3e} fact rows/s), bit-exact vs GPU: {hybrid['bit_exact_vs_gpu']}")
 
     batched = None if args.no_batch else bench_batched(wl, args.batch_queries, args.seed)
+    if batched is not None and args.batch_heavy:
+        batched["heavy"] = bench_batched_heavy(wl, args.batch_heavy, args.seed + 4)
     served = None
     if not args.no_serve:
         rates = [float(r) for r in args.serve_rates.split(",")]

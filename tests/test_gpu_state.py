@@ -149,3 +149,40 @@ def test_stop_while_a_wait_is_blocked(served):
     finally:
         if not stopped:
             P.server_stop()
+
+
+def test_select_counts_pending_until_read(engine):
+    """A dimension select whose result the caller does not take returns with its row
+    count still being computed on the device; looking the table up (its row count, its
+    columns, a view, the fact pipeline that joins it) waits for the count, and dropping
+    the table right away waits for its kernels before its columns are freed."""
+    wl = ssb.four_way(seed=41, fact_rows=2_000_000, n_cust=300_000, n_supp=20_000, n_part=140_000,
+                      prefix="pend")
+    ssb.load_device(wl)
+    tabs = host_tables(wl)
+    want, dims = oracle_query(wl, tabs)
+    # dropped while pending, several times over: each drop waits for its launch
+    for _ in range(3):
+        for pt, src, refs in wl.dims:
+            plan = pt.plan(src, refs)
+            P.run_plan(plan, want_result=False)
+            P.table_drop(plan["id"])
+    # each select's table, looked up while its count may still be pending
+    for pt, src, refs in wl.dims:
+        plan = pt.plan(src, refs)
+        P.run_plan(plan, want_result=False)
+        w = dims[plan["id"]]
+        rows, _ = P.table_info(plan["id"])
+        assert rows == w.rows, (plan["id"], rows, w.rows)
+        assert_same_columns([P.table_column(plan["id"], c) for c in range(len(w.columns))], w.columns)
+    # a view of a freshly selected table, then the fact pipeline over pending objects
+    pt, src, refs = wl.dims[1]
+    plan = pt.plan(src, refs)
+    P.run_plan(plan, want_result=False)
+    P.table_view("pend_view", plan["id"], 0, 1)
+    assert P.table_info("pend_view")[0] == 1
+    P.table_drop("pend_view")
+    ssb.run_dims(wl)
+    _same(ssb.run_fact(wl), want)
+    for t in wl.tables:
+        P.table_drop(t.id)
