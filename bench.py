@@ -75,8 +75,10 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     column of tier 0 is streamed whole, and a column of a later tier (the key of a
     later join, or, after the last tier, the metric and remaining key columns) is read
     in the `line`-byte L2 lines that hold a row passing every join of the earlier
-    tiers: HBM returns whole lines, so one passing row costs its line.  The pass
-    masks come from the host copy of the tables and the dimension results."""
+    tiers: HBM returns whole lines, so one passing row costs its line.  A tier-0
+    column read from its 24-bit copy (`packed24(c)` in the configuration) costs
+    1,536 bytes per 512 rows.  The pass masks come from the host copy of the tables
+    and the dimension results."""
     import re
     import numpy as np
     rows = len(fact_cols[0])
@@ -87,6 +89,8 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     if not m:
         return nlines * line * ncols, {"model": "every column streamed", "line_bytes": line}
     tiers = [int(x) for x in (m.group(1) or m.group(2)).split(",")]
+    pm = re.search(r"packed24\(([0-9,]+)\)", config or "")
+    packed = {int(x) for x in pm.group(1).split(",")} if pm else set()
     fq, src, refs = wl.fact
     last = max(tiers) + 1
     # join j probes fact column j (orderdate, custkey, suppkey, partkey) against the
@@ -118,13 +122,18 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     cache, total, detail = {}, 0, []
     for c in range(ncols):
         t = col_tier[c]
+        if t == 0 and c in packed:
+            b = (rows + 511) // 512 * 1536
+            total += b
+            detail.append({"column": c, "tier": t, "bytes": b, "copy": "24-bit"})
+            continue
         if t not in cache:
             cache[t] = lines_passing(t)
         total += cache[t] * line
         detail.append({"column": c, "tier": t, "lines": cache[t]})
-    return total, {"model": "tier-0 columns streamed; later columns by the 128-byte lines holding a row "
-                            "that passed the earlier tiers", "line_bytes": line, "tiers": tiers,
-                   "columns": detail}
+    return total, {"model": "tier-0 columns streamed (a 24-bit copy at 3 bytes a row); later columns by "
+                            "the 128-byte lines holding a row that passed the earlier tiers",
+                   "line_bytes": line, "tiers": tiers, "columns": detail}
 
 
 def _pct(xs, p):
@@ -563,10 +572,18 @@ def main():
 
     # HBM bytes per svm_scan launch from the committed FETCH_SIZE / WRITE_SIZE counter
     # runs of this same command (scaled by a copy kernel of known size)
+    # (the launch shape aside, the profiled configuration must be the one this run chose:
+    # a scan of another form reads other bytes)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "svm_scan_four_way_600M_traffic.json")
     if rows == 600_000_000 and os.path.exists(tpath):
-        traffic = json.load(open(tpath))["traffic_bytes"]
+        import re
+        prof = json.load(open(tpath))
+        form = lambda c: re.sub(r"^\d+x\d+ ", "", c or "")
+        if form(prof["configuration"]) == form(scan_config):
+            traffic = prof["traffic_bytes"]
+        else:
+            log(f"[gpu] no traffic figure: profiled form '{prof['configuration']}', this run '{scan_config}'")
 
     out = {
         "metric": "fact rows/s + % HBM roofline per star-join query; batched p50/p99 latency",

@@ -166,6 +166,7 @@ SVM_FN int svm_probe_str(const char* key, int key_size, const char* tab, const i
 // Device-only helpers of the generated kernels.
 static const char* kDeviceOnlyRT = R"SVMRT(
 typedef int svm_i32x4 __attribute__((ext_vector_type(4)));
+typedef int svm_i32x2 __attribute__((ext_vector_type(2)));
 /* Reserves one entry of an append buffer per active lane with `want` set: one atomic
    per wave on the counter, each lane's position its rank among the wanting lanes. */
 __device__ __forceinline__ svm_u64 svm_wave_reserve(svm_u64* counter, bool want) {
@@ -571,6 +572,20 @@ void emit_tiered(std::ostringstream& o, const KernelShape& s, const Tiers& tz, i
 // rows (filter and every join again, for the payloads) and aggregates them, one per
 // lane.
 void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int U) {
+  // The one tier-0 column read from its 24-bit copy: a lane streams groups of 8 rows
+  // (a 128-bit load of their low 16 bits and a 64-bit load of their high bytes, both
+  // coalesced over the wave's 512 rows) instead of int4 groups of 4 rows.
+  int c0 = -1;
+  {
+    std::vector<int> t0;
+    for (int c : s.fact_i32)
+      if (tz.col.at(c) == 0) t0.push_back(c);
+    if (t0.size() == 1 && !s.packed.empty() && s.packed[0] == t0[0]) c0 = t0[0];
+  }
+  const int R = c0 >= 0 ? 8 : 4, RS = c0 >= 0 ? 3 : 2;
+  if (R * U > 32) U = 32 / R;  // a lane's rows are the bits of one 32-bit mask
+  // a later tier's gather of fact column c at row i
+  auto gather = [&](int c) { return "((const int*)a.ptr[" + I(c) + "])[i]"; };
   o << "  const unsigned svm_lane = threadIdx.x & 63u;\n"
     // rows that passed every tier wait in a 128-entry ring per wave until 64 have
     // gathered; the wave then evaluates and aggregates them together, one per lane
@@ -590,29 +605,47 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     << "    svm_qh = (svm_qh + cnt) & 127u;\n"
     << "    svm_qn -= cnt;\n"
     << "  };\n";
-  o << "  const long long CH = (long long)SVM_BLOCK * " << U << ";\n"
-    << "  for (long long vb = v_lo + (long long)blockIdx.x * CH; vb < v_hi; vb += (long long)gridDim.x * CH) {\n";
+  o << "  const long long g_lo = r_lo >> " << RS << ", g_hi = (r_hi + " << R - 1 << ") >> " << RS << ";\n"
+    << "  const long long CH = (long long)SVM_BLOCK * " << U << ";\n"
+    << "  for (long long vb = g_lo + (long long)blockIdx.x * CH; vb < g_hi; vb += (long long)gridDim.x * CH) {\n";
   for (int u = 0; u < U; ++u) {
     o << "    const long long v" << u << "_ = vb + " << u << "ll * SVM_BLOCK + threadIdx.x;\n"
-      << "    const long long vc" << u << " = v" << u << "_ < v_hi ? v" << u << "_ : v_lo;\n";
-    for (int c : s.fact_i32)
-      if (tz.col.at(c) == 0)
-        o << "    const svm_i32x4 x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
-          << "] + vc" << u << ");\n";
-    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[4];\n";
+      << "    const long long vc" << u << " = v" << u << "_ < g_hi ? v" << u << "_ : g_lo;\n";
+    if (c0 >= 0) {
+      // group g of 8 rows: block g / 64 of 1,536 bytes, lane slot g % 64 of each plane
+      o << "    const char* svm_pb" << u << " = (const char*)a.ptr[" << SVM_SLOT_PACK << "] + 1536ll * (vc" << u
+        << " >> 6);\n"
+        << "    const svm_i32x4 lo" << u << " = __builtin_nontemporal_load((const svm_i32x4*)svm_pb" << u << " + (vc" << u
+        << " & 63));\n"
+        << "    const svm_i32x2 hi" << u << " = __builtin_nontemporal_load((const svm_i32x2*)(svm_pb" << u
+        << " + 1024) + (vc" << u << " & 63));\n";
+    } else {
+      for (int c : s.fact_i32)
+        if (tz.col.at(c) == 0)
+          o << "    const svm_i32x4 x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
+            << "] + vc" << u << ");\n";
+    }
+    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[" << R << "];\n";
   }
   o << "    unsigned svm_m = 0u;\n";
   const bool filt0 = s.filter != "true" && tz.filter == 0;
   for (int u = 0; u < U; ++u) {
-    o << "    #pragma unroll\n    for (int e = 0; e < 4; ++e) {\n"
-      << "      const long long i = v" << u << "_ * 4 + e;\n      (void)i;\n";
-    for (int c : s.fact_i32)
-      if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
+    o << "    #pragma unroll\n    for (int e = 0; e < " << R << "; ++e) {\n"
+      << "      const long long i = v" << u << "_ * " << R << " + e;\n      (void)i;\n";
+    if (c0 >= 0) {
+      // row e of the group: halfword e of the low plane, byte e of the high plane
+      o << "      const unsigned svm_l = ((unsigned)lo" << u << "[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;\n"
+        << "      const unsigned svm_h = ((unsigned)hi" << u << "[e >> 2] >> (8 * (e & 3))) & 0xFFu;\n"
+        << "      const int f" << c0 << " = (int)(svm_l | (svm_h << 16)) + a.iv[48];\n      (void)f" << c0 << ";\n";
+    } else {
+      for (int c : s.fact_i32)
+        if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
+    }
     o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
     if (filt0) o << "      cond = cond && (" << s.filter << ");\n";
     for (size_t j = 0; j < s.joins.size(); ++j)
       if (s.join_tier[j] == 0) emit_flat_join_test(o, s, j, u);
-    o << "      svm_m |= (cond ? 1u : 0u) << (" << 4 * u << " + e);\n    }\n";
+    o << "      svm_m |= (cond ? 1u : 0u) << (" << R * u << " + e);\n    }\n";
   }
   o << "    const unsigned svm_c = (unsigned)__popc(svm_m);\n"
     << "    unsigned svm_in = svm_c;\n"
@@ -637,15 +670,15 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     << "        unsigned mm = mo;\n"
     << "        for (unsigned t = 0; t < nth; ++t) mm &= mm - 1u;\n"
     << "        const unsigned bit = (unsigned)__builtin_ctz(mm);\n"
-    << "        const long long i = (vb + (long long)(bit >> 2) * SVM_BLOCK + (long long)((threadIdx.x & ~63u) + (unsigned)L)) * 4"
-       " + (long long)(bit & 3u);\n"
+    << "        const long long i = (vb + (long long)(bit >> " << RS << ") * SVM_BLOCK + (long long)((threadIdx.x & ~63u) + "
+       "(unsigned)L)) * " << R << " + (long long)(bit & " << R - 1 << "u);\n"
     << "        bool cond = true;\n";
   for (int t = 1; t < tz.last; ++t) {
     o << "        if (cond) {\n";
     for (size_t j = 0; j < s.joins.size(); ++j) {
       if (s.join_tier[j] != t) continue;
       const JoinShape& J = s.joins[j];
-      o << "          { const unsigned off = (unsigned)(((const int*)a.ptr[" << J.lcol << "])[i] - a.iv[" << 4 * j << "]);\n";
+      o << "          { const unsigned off = (unsigned)(" << gather(J.lcol) << " - a.iv[" << 4 * j << "]);\n";
       if (J.defer)
         o << "            cond = cond && off < (unsigned)a.iv[" << 4 * j + 1 << "] && svm_map_get<1>(mb" << j
           << ", off) != 0u; }\n";
@@ -672,6 +705,20 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
 }
 
 }  // namespace
+
+std::vector<int> compact_packable(const KernelShape& s) {
+  std::vector<int> out;
+  bool flat = s.flat;
+  for (auto& J : s.joins) flat = flat && J.kind == JOIN_MAP;
+  const Tiers tz = column_tiers(s, flat);
+  if (!tz.on || !s.compact) return out;
+  std::vector<int> t0;
+  for (int c : s.fact_i32)
+    if (tz.col.at(c) == 0) t0.push_back(c);
+  if (t0.size() != 1 || (s.filter != "true" && tz.filter == 0)) return out;
+  out.push_back(t0[0]);
+  return out;
+}
 
 std::string gen_scan_source(const KernelShape& s, const char* entry) {
   std::ostringstream o;
@@ -858,7 +905,7 @@ namespace {
 void put(std::ostringstream& k, const std::string& v) { k << v.size() << ':' << v; }
 
 std::string shape_key(const KernelShape& s, char kind, const char* entry) {
-  static_assert(sizeof(KernelShape) == 336 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
+  static_assert(sizeof(KernelShape) == 360 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
                 "a field was added to the kernel shape: add it to shape_key");
   std::ostringstream k;
   k << kind << entry << '|' << s.block << ',' << s.unroll << ',' << s.prefetch << s.flat << ',' << s.agg << ','
@@ -885,7 +932,8 @@ std::string shape_key(const KernelShape& s, char kind, const char* entry) {
   for (auto& t : s.proj_type) k << t.first << ',' << t.second << ';';
   k << '|';
   for (int t : s.join_tier) k << t << ',';
-  k << '|' << s.compact;
+  k << '|' << s.compact << '|';
+  for (int c : s.packed) k << c << ',';
   return k.str();
 }
 

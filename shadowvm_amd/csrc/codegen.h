@@ -81,6 +81,11 @@ struct KernelShape {
   // the rows that pass it out one per lane, and those rows' later tiers, filter and
   // aggregation run row by row from 4-byte gathers
   bool compact = false;
+  // compacting form: the one tier-0 column, when it is packed[0], is streamed from
+  // its 24-bit copy (a.ptr[SVM_SLOT_PACK], minimum a.iv[48]; blocks of 512 rows as
+  // 16-bit and 8-bit planes): a lane reads 8 rows per 128-bit plus 64-bit load,
+  // `unroll` groups of 8 rows per iteration.  Later tiers gather the Int32 columns.
+  std::vector<int> packed;
   // select
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs
@@ -100,6 +105,10 @@ std::shared_ptr<const GeneratedSource> select_source(const KernelShape& s);
 // so that profiles tell them from the query's own scans)
 std::string gen_scan_source(const KernelShape& s, const char* entry = "svm_scan");
 std::string gen_select_source(const KernelShape& s);
+
+// The fact column the compacting form of `s` can stream from a 24-bit copy: its one
+// tier-0 column, when it has one and the filter is applied later; else empty.
+std::vector<int> compact_packable(const KernelShape& s);
 // C++ source of the host engine's task function for the same pipeline shape:
 //   extern "C" void svm_host_task(const SvmArgs*, long long r_lo, long long r_hi, SvmHostOut*)
 // (a select when s.proj is set, else the reduce of s.agg).

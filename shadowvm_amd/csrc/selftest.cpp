@@ -99,6 +99,12 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.prefetch = false;
     s.unroll = 4;
     out.push_back({"scan_compacted", scan_checked(s)});
+    // its tier-0 key column streamed from a 24-bit copy, four groups of 8 rows a lane
+    s.packed = compact_packable(s);
+    if (s.packed.size() != 1 || s.packed[0] != 1) fail(SVM_ERR_JIT, "compact_packable: unexpected columns");
+    s.unroll = 4;
+    out.push_back({"scan_compacted_packed", scan_checked(s)});
+    s.packed.clear();
     s.compact = false;
     s.join_tier.clear();
     s.joins[1].bits_lds = false;

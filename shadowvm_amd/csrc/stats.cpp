@@ -53,8 +53,44 @@ bool current(const E& e, const TablePtr& t) {
   return e.owner.lock() == t && e.version == t->data_version();
 }
 std::map<std::tuple<const Table*, int, uint64_t>, ClusterEntry> g_clusters;
+std::map<std::pair<const Table*, int>, std::shared_ptr<const Packed24>> g_packed;
 
 }  // namespace
+
+std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s) {
+  if (t->base || t->row0 != 0 || col < 0 || col >= (int)t->cols.size() || t->cols[col].type != Int32 ||
+      !t->cols[col].dev || !t->rows)
+    return nullptr;
+  const auto key = std::make_pair(t.get(), col);
+  {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto it = g_packed.find(key);
+    if (it != g_packed.end() && current(*it->second, t)) return it->second;
+  }
+  // the key range is cached: a column too wide is told apart without a kernel
+  const KeyEntry ke = table_key(t, col, s);
+  if ((int64_t)ke.kmax - (int64_t)ke.kmin >= (int64_t)(1 << 24)) return nullptr;
+  noalloc_refuse("a 24-bit copy of a column");
+  std::shared_ptr<Packed24> p;
+  {
+    p = std::make_shared<Packed24>();
+    p->owner = t;
+    p->version = t->data_version();
+    p->kmin = ke.kmin;
+    const uint64_t blocks = (t->rows + 511) / 512;
+    p->bytes = DevBuf(blocks * 1536 + 256);
+    // the last block and the padding are zeroed first (a partial block's unused rows)
+    k::fill_u32(p->bytes.as<char>() + (blocks - 1) * 1536, (1536 + 256) / 4, 0u, s);
+    k::pack24((const int32_t*)t->col_ptr(col), t->rows, ke.kmin, p->bytes.as<uint8_t>(), s);
+    SVM_HIP(hipStreamSynchronize(s));
+  }
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  if (g_packed.size() >= 64)
+    for (auto it = g_packed.begin(); it != g_packed.end();)
+      it = it->second->owner.expired() ? g_packed.erase(it) : std::next(it);
+  g_packed[key] = p;
+  return p;
+}
 
 std::pair<TablePtr, int> column_root(const TablePtr& t, int col) {
   TablePtr cur = t;

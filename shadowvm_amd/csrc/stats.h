@@ -30,6 +30,20 @@ std::pair<TablePtr, int> column_root(const TablePtr& t, int col);
 std::shared_ptr<DictEntry> table_dict(const TablePtr& t, int col, hipStream_t s);
 KeyEntry table_key(const TablePtr& t, int col, hipStream_t s);
 
+// A 24-bit copy of Int32 column `col` of `t` (a table that owns its columns, row 0
+// at the start of the allocation), in blocks of 512 rows: row i's value - kmin has its
+// low 16 bits at uint16 i % 512 and its high 8 bits at byte 1024 + i % 512 of block
+// i / 512 (1,536 bytes per block).  The compacting scans stream their tier-0 key
+// column from it.  Null when the column's range needs more than 24 bits.  Built on
+// first use, cached with the table.
+struct Packed24 {
+  std::weak_ptr<Table> owner;
+  uint64_t version = 0;
+  int32_t kmin = 0;
+  DevBuf bytes;
+};
+std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s);
+
 constexpr uint64_t kHashRate = 5;  // com/cpp/define.hpp:29
 
 // Rows of bulk 0 of `t` that the reference's join hash table on Int32 column `col`

@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 # kTuneCandidates in shadowvm_amd/csrc/exec.cpp, kVariants in shadowvm_amd/csrc/batch_kernels.hip;
 # tests/test_boundary.py checks both against the library (svm_debug_variant_counts)
-N_SCAN_CANDIDATES = 40
+N_SCAN_CANDIDATES = 50
 N_BATCH_VARIANTS = 12
 
 
