@@ -121,12 +121,20 @@ __global__ void k_first_rows(const int32_t* __restrict__ keys, uint64_t n, int32
   }
 }
 
-__global__ void k_dup_flag(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
-                           const int32_t* __restrict__ first, uint32_t* flag) {
+// *flag = 1 when two rows share a key.  One bit per key offset, set with atomicOr; a
+// plain read first skips the atomic for keys already seen (a set bit can only come from
+// another row, and a stale 0 falls through to the exact atomic), so a column with few
+// distinct keys does not serialise on them.  Threads stop once the flag is up.
+__global__ void k_dup_bits(const int32_t* __restrict__ keys, uint64_t n, int32_t kmin, uint32_t range,
+                           uint32_t* bits, uint32_t* flag) {
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t off = (uint32_t)(keys[r] - kmin);
-    if (off < range && first[off] != (int32_t)r) *flag = 1u;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const uint32_t off = (uint32_t)(keys[r] - kmin);
+    if (off >= range) continue;
+    const uint32_t b = 1u << (off & 31u);
+    uint32_t* w = bits + (off >> 5);
+    if ((*w & b) || (atomicOr(w, b) & b)) atomicExch(flag, 1u);
   }
 }
 
@@ -437,13 +445,14 @@ void pack_map(const int32_t* first, uint32_t range, int payload, const int32_t* 
 
 bool has_duplicates(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, hipStream_t s) {
   if (!n || !range) return false;
-  DevBuf first((uint64_t)range * 4), flag(4);
-  first_rows(keys, n, kmin, range, first.as<int32_t>(), s);
-  fill_u32(flag.p, 1, 0, s);
-  k_dup_flag<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, first.as<int32_t>(), flag.as<uint32_t>());
+  const uint64_t nbw = ((uint64_t)range + 31) / 32;
+  DevBuf bits(nbw * 4 + 4);
+  uint32_t* flag = bits.as<uint32_t>() + nbw;
+  fill_u32(bits.p, nbw + 1, 0, s);
+  k_dup_bits<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, bits.as<uint32_t>(), flag);
   LAUNCH_CHECK("has_duplicates");
   uint32_t h = 0;
-  SVM_HIP(hipMemcpyAsync(&h, flag.p, 4, hipMemcpyDeviceToHost, s));
+  SVM_HIP(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s));
   SVM_HIP(hipStreamSynchronize(s));
   return h != 0;
 }

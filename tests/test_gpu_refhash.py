@@ -165,3 +165,36 @@ def test_batch_and_server_leave_dropping_queries_to_the_plain_path(engine):
             assert_same_columns(r.columns, w.columns)
     finally:
         P.server_stop()
+
+
+@pytest.mark.parametrize("dups", [0, 1, 3000])
+def test_dimension_with_repeated_keys(engine, dups):
+    """A dimension root whose key column repeats keys (`dups` extra rows drawn from its
+    keys, shuffled in): its filtered object is not built as a one-row-per-key map, and
+    every probe finds the first row of its key, as in the oracle.  With dups == 0 the
+    keys are unique and the object takes the direct build."""
+    rng = np.random.default_rng(40 + dups)
+    keys = np.arange(1, 20_001, dtype=np.int32)
+    keys = rng.permutation(np.concatenate([keys, rng.choice(keys, dups)])).astype(np.int32)
+    attr = (np.arange(len(keys)) % 97).astype(np.int32)  # differs between a key's rows
+    base = [keys, attr]
+    fk = rng.integers(1, 20_100, 1_000_003).astype(np.int32)
+    val = rng.integers(-1000, 1000, len(fk)).astype(np.int32)
+    fact = [fk, val]
+    tag = f"rk{dups}"
+    print(f"[refhash] repeated keys: {dups} extra rows", flush=True)
+    P.table_upload(f"{tag}_base", [I, I], [4, 4], base)
+    P.table_upload(f"{tag}_fact", [I, I], [4, 4], fact)
+    try:
+        dim_plan(f"{tag}_dim", "$(0.1)!=5").Run(f"{tag}_base", want_result=False)
+        q = fact_plan(f"{tag}_q", "$(1.1)", "$(0.1)")
+        got = q.Run(f"{tag}_fact", [f"{tag}_dim"])
+        d = O.run_pipeline(dim_plan(f"{tag}_dim", "$(0.1)!=5").plan("b"), O.OTable([I, I], [4, 4], base, 3_000_000),
+                           tuple_per_bulk=3_000_000)
+        want = O.run_pipeline(q.plan("f", ["d"]), O.OTable([I, I], [4, 4], fact, 3_000_000), [d.table],
+                              tuple_per_bulk=3_000_000)
+        assert got.rows == want.rows == 96
+        assert_same_columns(got.columns, want.columns)
+    finally:
+        for t in (f"{tag}_base", f"{tag}_fact", f"{tag}_dim"):
+            P.table_drop(t)
