@@ -406,11 +406,16 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
 // Staged column loads: the tier of each join's key column (KernelShape::join_tier), or
 // none.  A column is read in 32-byte sectors (8 rows); a sector of a column loaded
 // after tiers that pass an estimated fraction r of rows is read with probability
-// 1 - (1 - r)^8.  Dense-map joins that filter (estimated pass fraction below 1) are
-// taken most selective first: mode 1 puts the first k in one tier, mode 2 gives each
-// its own tier; k is chosen to minimise the estimated column bytes (plus a small
-// charge per tier for the dependent load), and staging is used only when that is at
-// most 85% of reading every column.
+// 1 - (1 - r)^8.  A join's test costs little when its map or existence bits sit in LDS
+// (assign_lds) and about kL2LookupCost columns' worth per row that reaches it when they
+// are read from L2 (one scattered load per row: the texture path, not HBM, bounds it).
+// Every ordered choice of the dense-map joins that filter (estimated pass fraction
+// below 1) is costed: mode 1 puts the chosen joins in one tier, mode 2 gives each its
+// own tier in that order; the cheapest plan (plus a small charge per tier for the
+// dependent load) is taken, and staging is used only when it costs at most 85% of
+// reading every column.  Among equal costs, the most selective joins first.
+constexpr double kL2LookupCost = 2.5;
+
 std::vector<int> plan_tiers(const std::vector<JoinBuild>& joins, const KernelShape& ks, int mode) {
   std::vector<size_t> order;
   for (size_t j = 0; j < joins.size(); ++j)
@@ -418,34 +423,43 @@ std::vector<int> plan_tiers(const std::vector<JoinBuild>& joins, const KernelSha
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return joins[x].sel < joins[y].sel; });
   const double ncols = (double)ks.fact_i32.size();
   auto sector = [](double r) { return 1.0 - std::pow(1.0 - r, 8.0); };
+  auto in_lds = [&](size_t j) { return joins[j].staged_lds || joins[j].staged_bits; };
   double best = ncols * 0.85;
   std::vector<int> plan;
-  for (size_t k = 1; k <= order.size(); ++k) {
-    std::vector<int> tier(joins.size(), -1);
-    for (size_t q = 0; q < k; ++q) tier[order[q]] = mode == 2 ? (int)q : 0;
-    // each column at its earliest tier; the rest after the last
-    std::map<int, int> ct;
-    for (size_t q = 0; q < k; ++q) {
-      const int c = joins[order[q]].shape.lcol, t = tier[order[q]];
-      ct[c] = ct.count(c) ? std::min(ct[c], t) : t;
-    }
-    const int ntiers = mode == 2 ? (int)k : 1;
-    std::vector<double> pass(ntiers + 1, 1.0);  // rows passing the tiers before t
-    for (int t = 1; t <= ntiers; ++t) {
-      pass[t] = pass[t - 1];
+  // permutations of positions into `order` (the first is `order` itself); each prefix
+  // of each permutation is a candidate plan
+  std::vector<size_t> perm(order.size());
+  for (size_t q = 0; q < perm.size(); ++q) perm[q] = q;
+  do {
+    for (size_t k = 1; k <= perm.size(); ++k) {
+      std::vector<int> tier(joins.size(), -1);
+      for (size_t q = 0; q < k; ++q) tier[order[perm[q]]] = mode == 2 ? (int)q : 0;
+      // each column at its earliest tier; the rest after the last
+      std::map<int, int> ct;
+      for (size_t q = 0; q < k; ++q) {
+        const int c = joins[order[perm[q]]].shape.lcol, t = tier[order[perm[q]]];
+        ct[c] = ct.count(c) ? std::min(ct[c], t) : t;
+      }
+      const int ntiers = mode == 2 ? (int)k : 1;
+      std::vector<double> pass(ntiers + 1, 1.0);  // rows passing the tiers before t
+      for (int t = 1; t <= ntiers; ++t) {
+        pass[t] = pass[t - 1];
+        for (size_t q = 0; q < k; ++q)
+          if (tier[order[perm[q]]] == t - 1) pass[t] *= joins[order[perm[q]]].sel;
+      }
+      double cost = 0.05 * ntiers;
+      for (int c : ks.fact_i32) {
+        const int t = ct.count(c) ? ct[c] : ntiers;
+        cost += t == 0 ? 1.0 : sector(pass[t]);
+      }
       for (size_t q = 0; q < k; ++q)
-        if (tier[order[q]] == t - 1) pass[t] *= joins[order[q]].sel;
+        if (!in_lds(order[perm[q]])) cost += kL2LookupCost * pass[tier[order[perm[q]]]];
+      if (cost < best - 1e-9) {
+        best = cost;
+        plan = tier;
+      }
     }
-    double cost = 0.05 * ntiers;
-    for (int c : ks.fact_i32) {
-      const int t = ct.count(c) ? ct[c] : ntiers;
-      cost += t == 0 ? 1.0 : sector(pass[t]);
-    }
-    if (cost < best) {
-      best = cost;
-      plan = tier;
-    }
-  }
+  } while (std::next_permutation(perm.begin(), perm.end()));
   return plan;
 }
 
