@@ -849,6 +849,18 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
   // a column's n distinct values, an inequality 1 - 1/n, a range about half; queries
   // taken as independent).  The masks are ANDed, so the order does not change any
   // result.
+  // estimated fraction of a slot's keys that pass join j's dimension filter
+  auto join_pass = [&](const JoinL& j) {
+    double f = 1.0;
+    for (const Atom& a : j.atoms) {
+      const double n = std::max<uint32_t>(1u, table_dict(j.dim, a.col, s)->n);
+      f *= a.kind == bk::DP_STREQ ? 1.0 / n : a.kind == bk::DP_STRNE ? 1.0 - 1.0 / n
+           : a.lo > a.hi          ? 0.0
+           : a.lo == a.hi         ? 1.0 / n
+                                  : 0.5;
+    }
+    return f;
+  };
   {
     std::vector<std::pair<double, uint64_t>> key(P.slots.size());
     for (size_t k = 0; k < P.slots.size(); ++k) {
@@ -859,15 +871,7 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       for (int i : P.qi)
         for (const JoinL& j : qs[i]->joins) {
           if (j.dim != sl.dim || j.kcol != sl.kcol || j.fcol != sl.fcol) continue;
-          double f = 1.0;
-          for (const Atom& a : j.atoms) {
-            const double n = std::max<uint32_t>(1u, table_dict(sl.dim, a.col, s)->n);
-            f *= a.kind == bk::DP_STREQ ? 1.0 / n : a.kind == bk::DP_STRNE ? 1.0 - 1.0 / n
-                 : a.lo > a.hi          ? 0.0
-                 : a.lo == a.hi         ? 1.0 / n
-                                        : 0.5;
-          }
-          miss *= 1.0 - f;
+          miss *= 1.0 - join_pass(j);
         }
       const uint64_t bytes = entries * ebits / 8;
       const double cost = bytes <= (64u << 10) ? 0.25 : bytes <= (2u << 20) ? 1.0 : 3.0;
@@ -1061,6 +1065,53 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
     }
     A->seg0[bk::kMaxCols] = nb;
   }
+  // Staged variants: the key columns of the first slots form tier 0 when that is
+  // estimated to read fewer bytes.  After slots 0..k-1 a row passes query q with the
+  // product of q's estimated join pass fractions on them (times the share of the
+  // union of slices q covers), and passes some query with 1 - prod_q (1 - that);
+  // every other column is then read in the 128-byte lines (32 rows) holding such a
+  // row.  k = 0 (every column for every row) unless some k costs at most 85 % of it.
+  {
+    unsigned cols = 0;
+    for (int q = 0; q < nq; ++q) cols |= A->qcols[q];
+    const double span = row_hi > row_lo ? (double)(row_hi - row_lo) : 1.0;
+    const double all = (double)__builtin_popcount(cols);
+    double best = 0.85 * all;
+    A->nslot0 = 0;
+    A->tier0 = 0x3Fu;
+    for (size_t k = 1; k <= P.slots.size(); ++k) {
+      unsigned t0 = 0;
+      for (size_t sk = 0; sk < k; ++sk) t0 |= 1u << P.slots[sk].fcol;
+      double none = 1.0;  // a row passes no query
+      for (int q = 0; q < nq; ++q) {
+        const FactLowered& f = *qs[P.qi[q]];
+        double pq = (double)(f.row1 - f.row0) / span;
+        for (auto& j : f.joins)
+          if (slot_index(j) < (int)k) pq *= join_pass(j);
+        none *= 1.0 - std::min(1.0, std::max(0.0, pq));
+      }
+      const double line = 1.0 - std::pow(none, 32.0);
+      const double cost = (double)__builtin_popcount(cols & t0) + (double)__builtin_popcount(cols & ~t0) * line;
+      if (cost < best) {
+        best = cost;
+        A->nslot0 = (int)k;
+        A->tier0 = t0;
+      }
+    }
+    // SVM_BATCH_TIER0_SLOTS=<k>: tier 0 holds the first k slots' key columns whatever
+    // the estimate (0: no staging)
+    if (const char* e = getenv("SVM_BATCH_TIER0_SLOTS")) {
+      char* end = nullptr;
+      const long k = strtol(e, &end, 10);
+      if (!*e || !end || *end || k < 0) fail(SVM_ERR_INVALID, "SVM_BATCH_TIER0_SLOTS must be a slot count");
+      A->nslot0 = (int)std::min<long>(k, (long)P.slots.size());
+      A->tier0 = 0x3Fu;
+      if (A->nslot0 > 0) {
+        A->tier0 = 0;
+        for (int sk = 0; sk < A->nslot0; ++sk) A->tier0 |= 1u << P.slots[sk].fcol;
+      }
+    }
+  }
   int nt = 0;
   for (size_t k = 0; k < P.slots.size(); ++k) {
     bk::MaskSlot& ms = M->slots[k];
@@ -1178,11 +1229,13 @@ static void run_pass(const std::vector<FactLowered*>& qs, PassB& P,
       }
       trials.push_back({(double)c.variant, (double)c.tile, (double)c.bits, (double)ms});
       if (E.cfg.verbose)
-        fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots (%s), %s lookups%s, key bits %d, tile %lld rows: %.3f ms\n",
+        fprintf(stderr, "[svm] batch tune: %d threads, %d LDS slots (%s), %s lookups%s%s, key bits %d, tile %lld rows: %.3f ms\n",
                 bk::scan_variant_block(c.variant), bk::scan_variant_hash_slots(c.variant),
                 bk::scan_variant_dense(c.variant) ? "dense" : "hash",
                 bk::scan_variant_parallel(c.variant) ? "parallel" : "gated",
-                bk::scan_variant_prefetch(c.variant) ? ", prefetch" : "", c.bits, c.tile, ms);
+                bk::scan_variant_prefetch(c.variant) ? ", prefetch" : "",
+                bk::scan_variant_staged(c.variant) ? (", staged over " + std::to_string(A->nslot0) + " slots").c_str() : "",
+                c.bits, c.tile, ms);
       return ms < 0 ? 1e30f : ms;
     };
     float best = 1e30f;

@@ -163,6 +163,11 @@ struct ScanArgs {
   unsigned long long segm[kMaxSegBounds + kMaxCols];
   unsigned long long colq[kMaxCols];
   int segsteps[kMaxCols];
+  // Staged variants: the columns in tier0 are loaded for every row group, slots
+  // 0..nslot0-1 (whose key columns they hold) tested first, and the other columns
+  // loaded only for lanes with a row that still passes some query.
+  unsigned tier0;
+  int nslot0;
   const DropCheck* chk;         // slots to check for dropped inserts
   int nchk;
   unsigned long long* dropq;    // bit q: query q's dimension object drops an insert
@@ -211,6 +216,7 @@ int scan_variant_hash_slots(int v);
 bool scan_variant_parallel(int v);  // slot lookups issued together rather than gated in turn
 bool scan_variant_mask32(int v);    // 32-bit query masks (scans of at most 32 queries)
 bool scan_variant_prefetch(int v);  // next row group's loads in flight during this one's lookups
+bool scan_variant_staged(int v);    // ScanArgs::tier0 columns first, the rest for passing lanes only
 // Dense-accumulator variants: a query with few groups owns scan_variant_hash_slots(v)
 // LDS slots' worth of a per-workgroup dense table (QDesc::dense, assigned per pass by
 // assign_dense); the others add straight into their HBM tables.

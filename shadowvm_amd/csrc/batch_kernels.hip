@@ -255,9 +255,15 @@ __device__ __forceinline__ void load_or_zero(const ScanArgs* __restrict__ A, uns
 //          those rows (query mask, profile ids, metric values) to its LDS queue and
 //          aggregates the queue 64 rows at a time, one row per lane, so selective
 //          batches do not stall the column stream on a wave's one passing row.
-template <int BLOCK, int HBITS, bool PAR, bool PF, typename M, int AGG, bool LMAP, bool Q>
+//   ST: staged loads.  Only the columns in A->tier0 are loaded for every group; the
+//          slots 0..nslot0-1 are tested on them, and a lane loads the other columns
+//          only when one of its rows still passes some query (then the fact
+//          predicates and the remaining slots).  The masks are ANDed, so the order
+//          of the tests does not change any result.
+template <int BLOCK, int HBITS, bool PAR, bool PF, typename M, int AGG, bool LMAP, bool Q, bool ST>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_shared_scan(
     const ScanArgs* __restrict__ A) {
+  static_assert(!(ST && (PF || PAR)), "staged loads take gated lookups without prefetch");
   constexpr unsigned kSlots = 1u << HBITS;  // LDS (query, group) accumulator
   // hash: one key word per slot; dense: one existence bit per slot
   constexpr bool DENSE = AGG != 0;
@@ -414,6 +420,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     if (t >= A->ntiles) break;
     const M full = (M)tmask[0], part = (M)tmask[1], active = full | part;
     const unsigned colmask = tcols;
+    // columns loaded for every group (all of them without staged loads)
+    const unsigned lmask = ST ? colmask & A->tier0 : colmask;
     const long long lo = A->row_lo + (long long)t * A->tile_rows;  // a multiple of 4
     const long long hi = min(lo + A->tile_rows, A->row_hi);
     if (active) {
@@ -424,7 +432,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
       const long long ln = (long long)(threadIdx.x & 63u);
       const long long vw0 = (lo >> 2) + (long long)(threadIdx.x & ~63u);
       i32x4 x[kMaxCols];
-      load_or_zero(A, colmask, vw0 + ln, vend, x);
+      load_or_zero(A, lmask, vw0 + ln, vend, x);
       for (long long vw = vw0; vw < vend; vw += BLOCK) {
         const long long v = vw + ln;
         // PF: the next group's loads are issued before this group's lookups
@@ -451,6 +459,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         // fact predicates of the tile's queries, grouped by column: tested one by one
         // when the tile has few on the column, else through the column's value
         // segments (a search over its sorted bounds, then that segment's query mask)
+        auto apply_preds = [&]() {
 #pragma unroll
         for (int c = 0; c < kMaxCols; ++c) {
           const M cq = (M)A->colq[c] & active;
@@ -481,11 +490,40 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         }
+        };
+        // gated lookups of slots [s0, s1): each slot only for the rows some joining
+        // query still wants
+        unsigned cid[kMaxSlot][2];  // two rows' 16-bit profile ids per register
+#pragma unroll
+        for (int s = 0; s < kMaxSlot; ++s) cid[s][0] = cid[s][1] = 0u;
+        auto gated_slots = [&](int s0, int s1) {
+#pragma unroll
+          for (int s = 0; s < kMaxSlot; ++s) {
+            if (s < s0 || s >= s1 || s >= nslot || !((M)A->slot[s].joined & active)) continue;
+            unsigned cs[4] = {0u, 0u, 0u, 0u};
+            slot_lookup<M, LMAP>(A->slot[s], x, pass, use_bits, lmaps, cs);
+            slot_apply<M>(A->slot[s], pmask, cs, pass);
+            cid[s][0] = cs[0] | cs[1] << 16;
+            cid[s][1] = cs[2] | cs[3] << 16;
+          }
+        };
+        if (ST) {
+          const int n0 = A->nslot0;
+          gated_slots(0, n0);
+          // the other columns, for lanes with a row some query still wants
+          const bool any0 = (pass[0] | pass[1] | pass[2] | pass[3]) != (M)0;
+          const unsigned rest = colmask & ~lmask;
+#pragma unroll
+          for (int c = 0; c < kMaxCols; ++c)
+            if (rest & (1u << c)) x[c] = any0 ? __builtin_nontemporal_load((const i32x4*)A->cols[c] + v) : i32x4{0, 0, 0, 0};
+          apply_preds();
+          gated_slots(n0, kMaxSlot);
+        } else {
+        apply_preds();
         // joins: one profile lookup per slot for the rows some joining query still
         // wants.  PAR issues every slot's lookups before combining any (one memory
         // latency per row group); otherwise each slot is gated by the slots before
         // it (fewer lookups when the first joins are selective).
-        unsigned cid[kMaxSlot][2];  // two rows' 16-bit profile ids per register
         if (PAR) {
           unsigned cs[kMaxSlot][4];
 #pragma unroll
@@ -501,17 +539,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             cid[s][1] = cs[s][2] | cs[s][3] << 16;
           }
         } else {
-#pragma unroll
-          for (int s = 0; s < kMaxSlot; ++s) {
-            unsigned cs[4] = {0u, 0u, 0u, 0u};
-            if (s < nslot && ((M)A->slot[s].joined & active)) {
-              slot_lookup<M, LMAP>(A->slot[s], x, pass, use_bits, lmaps, cs);
-              slot_apply<M>(A->slot[s], pmask, cs, pass);
-            }
-            cid[s][0] = cs[0] | cs[1] << 16;
-            cid[s][1] = cs[2] | cs[3] << 16;
-          }
+          gated_slots(0, kMaxSlot);
         }
+        }  // not staged
         const M lane_any = pass[0] | pass[1] | pass[2] | pass[3];
         bool agg_here = lane_any != (M)0;
         if (Q) {
@@ -674,7 +704,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         // the next group: already in flight (PF) or requested now
 #pragma unroll
         for (int c = 0; c < kMaxCols; ++c) x[c] = xn[c];
-        if (!PF) load_or_zero(A, colmask, v + BLOCK, vend, x);
+        if (!PF) load_or_zero(A, lmask, v + BLOCK, vend, x);
       }
     }
     __syncthreads();
@@ -936,29 +966,33 @@ struct Variant {
   int block, hbits;
   bool par, pf, mask32;
   int agg;
-  bool lmap, queue;
+  bool lmap, queue, staged;
   const void* fn;
 };
-#define SVM_SCAN_VARIANT(B, H, PAR, PF, M32, D, L, Q)                                                              \
-  {                                                                                                              \
-    B, H, PAR, PF, M32, D, L, Q,                                                                                 \
-        (const void*)k_shared_scan<B, H, PAR, PF, std::conditional_t<M32, unsigned, unsigned long long>, D, L, Q> \
+#define SVM_SCAN_VARIANT(B, H, PAR, PF, M32, D, L, Q, ST)                                                            \
+  {                                                                                                                \
+    B, H, PAR, PF, M32, D, L, Q, ST,                                                                               \
+        (const void*)k_shared_scan<B, H, PAR, PF, std::conditional_t<M32, unsigned, unsigned long long>, D, L, Q, ST> \
   }
 // (threads, log2 LDS accumulator slots, parallel lookups, prefetch, 32-bit masks,
-//  aggregation, LDS maps, queue)
+//  aggregation, LDS maps, queue, staged loads)
 const Variant kVariants[] = {
-    SVM_SCAN_VARIANT(512, 11, false, false, false, 0, false, false),
-    SVM_SCAN_VARIANT(512, 11, false, false, true, 0, false, false),
-    SVM_SCAN_VARIANT(512, 11, true, false, false, 0, false, false),
-    SVM_SCAN_VARIANT(512, 11, true, false, true, 0, false, false),
-    SVM_SCAN_VARIANT(512, 11, false, true, false, 0, false, false),
-    SVM_SCAN_VARIANT(512, 11, false, true, true, 0, false, false),
-    SVM_SCAN_VARIANT(256, 10, false, false, false, 0, false, false),
-    SVM_SCAN_VARIANT(256, 10, false, false, true, 0, false, false),
-    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, false),
-    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, false),
-    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, true),
-    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, true),
+    SVM_SCAN_VARIANT(512, 11, false, false, false, 0, false, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, false, true, 0, false, false, false),
+    SVM_SCAN_VARIANT(512, 11, true, false, false, 0, false, false, false),
+    SVM_SCAN_VARIANT(512, 11, true, false, true, 0, false, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, true, false, 0, false, false, false),
+    SVM_SCAN_VARIANT(512, 11, false, true, true, 0, false, false, false),
+    SVM_SCAN_VARIANT(256, 10, false, false, false, 0, false, false, false),
+    SVM_SCAN_VARIANT(256, 10, false, false, true, 0, false, false, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, false, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, false, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, true, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, true, false),
+    SVM_SCAN_VARIANT(1024, 12, false, false, false, 1, true, true, true),
+    SVM_SCAN_VARIANT(1024, 12, false, false, true, 1, true, true, true),
+    SVM_SCAN_VARIANT(512, 11, false, false, false, 0, false, false, true),
+    SVM_SCAN_VARIANT(512, 11, false, false, true, 0, false, false, true),
 };
 #undef SVM_SCAN_VARIANT
 // dynamic LDS of the variants with LDS maps: the CU's 160 KB less a margin for the
@@ -973,6 +1007,7 @@ bool scan_variant_parallel(int v) { return kVariants[v].par; }
 bool scan_variant_prefetch(int v) { return kVariants[v].pf; }
 bool scan_variant_mask32(int v) { return kVariants[v].mask32; }
 bool scan_variant_dense(int v) { return kVariants[v].agg != 0; }
+bool scan_variant_staged(int v) { return kVariants[v].staged; }
 
 void assign_dense(QDesc* qd, int nq, unsigned slots) {
   // queries with the fewest groups first, while their tables fit

@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 # kTuneCandidates in shadowvm_amd/csrc/exec.cpp, kVariants in shadowvm_amd/csrc/batch_kernels.hip;
 # tests/test_boundary.py checks both against the library (svm_debug_variant_counts)
 N_SCAN_CANDIDATES = 50
-N_BATCH_VARIANTS = 12
+N_BATCH_VARIANTS = 16
 
 
 def tune_report(reset=False):
@@ -135,6 +135,20 @@ def test_every_shared_scan_variant_matches_oracle(batch_case, monkeypatch, varia
             monkeypatch.setenv("SVM_BATCH_VARIANT", f"{variant},{tile},{bits}")
             print(f"[variants] batch: variant {variant} tile {tile} key bits {bits}", flush=True)
             _check_batch(batch_case, (variant, tile, bits))
+
+
+STAGED_BATCH_VARIANTS = (12, 13, 14, 15)
+
+
+@pytest.mark.parametrize("variant", STAGED_BATCH_VARIANTS)
+def test_staged_shared_scan_over_any_tier0(batch_case, monkeypatch, variant):
+    """Staged loads with tier 0 forced to the first k slots' key columns (k = 0: no
+    staging; k past the slot count: every slot in tier 0)."""
+    for k in (0, 1, 2, 3, 6):
+        monkeypatch.setenv("SVM_BATCH_VARIANT", f"{variant},16384,{k % 2}")
+        monkeypatch.setenv("SVM_BATCH_TIER0_SLOTS", str(k))
+        print(f"[variants] batch: staged variant {variant} tier-0 slots {k}", flush=True)
+        _check_batch(batch_case, (variant, "tier0", k))
 
 
 def test_batch_tuner_trials_agree(batch_case, monkeypatch):
