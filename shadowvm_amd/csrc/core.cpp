@@ -237,6 +237,31 @@ void* host_staging(hipStream_t s, uint64_t bytes) {
   return st.p;
 }
 
+namespace {
+struct Mapped {
+  void* host = nullptr;
+  void* dev = nullptr;
+  uint64_t cap = 0;
+};
+std::map<hipStream_t, Mapped> g_mapped;  // guarded by g_arena_mu
+}  // namespace
+
+MappedStaging mapped_staging(hipStream_t s, uint64_t bytes) {
+  std::lock_guard<std::mutex> g(g_arena_mu);
+  Mapped& st = g_mapped[s];
+  if (st.cap < bytes) {
+    noalloc_refuse("a larger host-mapped staging buffer");
+    if (st.host) (void)hipHostFree(st.host);
+    uint64_t cap = 1 << 20;
+    while (cap < bytes) cap <<= 1;
+    st = Mapped{};
+    SVM_HIP(hipHostMalloc(&st.host, cap, hipHostMallocMapped | hipHostMallocCoherent));
+    SVM_HIP(hipHostGetDevicePointer(&st.dev, st.host, 0));
+    st.cap = cap;
+  }
+  return MappedStaging{(char*)st.host, (char*)st.dev};
+}
+
 void scan_state_reset(hipStream_t s) {
   std::lock_guard<std::mutex> g(g_arena_mu);
   auto it = g_scan_bufs.find(s);
@@ -323,6 +348,8 @@ void arena_release_all() {
   g_pin_free.clear();
   for (auto& kv : g_staging) (void)hipHostFree(kv.second.p);
   g_staging.clear();
+  for (auto& kv : g_mapped) (void)hipHostFree(kv.second.host);
+  g_mapped.clear();
 }
 
 void* dev_alloc_async(uint64_t bytes, hipStream_t s) {

@@ -94,6 +94,14 @@ struct Event {
 // buffer with one synchronisation when they fit in kStagedCopyBytes, else one
 // synchronous copy per column.
 constexpr uint64_t kStagedCopyBytes = 64ull << 20;
+// Group spaces whose every-group-id output fits this many bytes per column set are
+// emitted with one synchronisation (the output and a host-mapped copy sized by G).
+constexpr uint64_t kOneSyncEmitBytes = 16ull << 20;
+uint64_t row_bytes_of(const std::vector<std::pair<int, int64_t>>& schema) {
+  uint64_t b = 0;
+  for (auto& c : schema) b += (uint64_t)c.second;
+  return b;
+}
 void copy_out(const Table& out, const std::vector<std::pair<int, int64_t>>& schema, int ncols, uint64_t rows,
               hipStream_t s, std::vector<std::vector<uint8_t>>& cols) {
   uint64_t total = 0;
@@ -1363,6 +1371,69 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     out = c.E.store.create(c.info.id, oschema, 1, nullptr, c.s);
     for (int m = 0; m < M; ++m)
       SVM_HIP(hipMemcpyAsync(out->cols[m].dev, (char*)ssum.p + 8 * m, 8, hipMemcpyDeviceToDevice, c.s));
+  } else if (opt.want_host && G && G * row_bytes_of(oschema) <= kOneSyncEmitBytes) {
+    // One synchronisation: the group count, the emission into an output sized for
+    // every group id and the host copy (the emission also writes each row into
+    // host-mapped memory) run back to back; the rows are known when the stream is done.
+    uint64_t nb = (nflag + 255) / 256;
+    DevBuf blocks(nb * 8 + 8, c.s), offs(nb * 8 + 16, c.s);
+    const HostSlots hs = host_slots(c.s);
+    hs.host[2] = 0;
+    const uint64_t tot = k::group_count_async(gflag.as<uint32_t>(), G, blocks.as<unsigned long long>(),
+                                              offs.as<unsigned long long>(), hs.dev + 2, c.s);
+    out = c.E.store.create(c.info.id, oschema, G, nullptr, c.s, /*zero_pad=*/false);
+    const MappedStaging ms = mapped_staging(c.s, G * row_bytes_of(oschema));
+    k::EmitArgs ea;
+    memset(&ea, 0, sizeof(ea));
+    ea.flags = gflag.as<uint32_t>();
+    ea.sums = gsum.as<uint64_t>();
+    ea.G = G;
+    ea.K = K;
+    ea.M = M;
+    uint64_t hoff[SVM_MAX_SELECT + 8] = {};
+    uint64_t off = 0;
+    for (int q = 0; q < K + M; ++q) {
+      hoff[q] = off;
+      off += G * (uint64_t)oschema[q].second;
+    }
+    for (int q = 0; q < K; ++q) {
+      ea.n[q] = keys[q].n;
+      ea.stride[q] = keys[q].stride;
+      ea.fmin[q] = keys[q].fmin;
+      ea.dict[q] = keys[q].shape.src == KEY_FACT ? nullptr : keys[q].dict->dict.as<int32_t>();
+      ea.keys_out[q] = (int32_t*)out->cols[q].dev;
+      ea.keys_host[q] = (int32_t*)(ms.dev + hoff[q]);
+    }
+    for (int m = 0; m < M; ++m) {
+      ea.sums_out[m] = (int64_t*)out->cols[K + m].dev;
+      ea.sums_host[m] = (int64_t*)(ms.dev + hoff[K + m]);
+    }
+    ea.total = offs.as<unsigned long long>() + tot;
+    k::group_emit(ea, offs.as<unsigned long long>(), c.s);
+    out->max_tuple_count = c.E.cfg.tuple_per_bulk;
+    ev3.record(c.s);
+    SVM_HIP(hipStreamSynchronize(c.s));
+    out_rows = hs.host[2];
+    if (out_rows > G) fail(SVM_ERR_DEVICE, "group emission: more rows than group ids");
+    out->rows = out_rows;
+    out->bulk0 = out_rows;
+    res->kind = 1;
+    res->rows = (int64_t)out_rows;
+    res->types = oschema;
+    res->cols.clear();
+    for (int q = 0; q < K + M; ++q) {
+      const char* h = ms.host + hoff[q];
+      res->cols.emplace_back(h, h + out_rows * (uint64_t)oschema[q].second);
+    }
+    res->build_ms = elapsed(ev0, ev1);
+    res->scan_ms = elapsed(ev1, ev2);
+    res->finish_ms = elapsed(ev2, ev3);
+    c.E.store.put(c.info.id, out);
+    if (c.E.cfg.verbose)
+      fprintf(stderr, "[svm] %s: host %.3f ms (joins built %.3f, launch %.3f); gpu build %.3f scan %.3f finish %.3f; G=%llu agg=%d %s\n",
+              c.info.id.c_str(), host_ms(t_host0), t_build_host, t_launch_host, res->build_ms, res->scan_ms,
+              res->finish_ms, (unsigned long long)G, ks.agg, res->tune.c_str());
+    return res;
   } else {
     uint64_t nb = (nflag + 255) / 256;
     DevBuf blocks(nb * 8 + 8, c.s), offs(nb * 8 + 16, c.s);

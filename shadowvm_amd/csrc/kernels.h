@@ -68,11 +68,21 @@ struct EmitArgs {
   const int32_t* dict[3];     // sorted dictionary of the component, or null
   int32_t* keys_out[3];       // K int32 columns
   int64_t* sums_out[8];       // M int64 columns
+  // optional: the same rows also written here (host-mapped memory), and the row count
+  // on the device, after which block 0 zeroes kColumnPad bytes of every output column
+  int32_t* keys_host[3];
+  int64_t* sums_host[8];
+  const unsigned long long* total;
 };
 // Writes the groups whose flag is set, in group-id order, and returns their count
 // (synchronises once to size the output).
 uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* scratch_blocks,
                      unsigned long long* scratch_offsets, hipStream_t s);
+// The same without synchronising: the count goes to *count_out (a device address, e.g.
+// of a host-mapped word) and to scratch_offsets[blocks], blocks = (ceil(G/32)+255)/256;
+// returns that offset.
+uint64_t group_count_async(const uint32_t* flags, uint64_t G, unsigned long long* scratch_blocks,
+                           unsigned long long* scratch_offsets, unsigned long long* count_out, hipStream_t s);
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s);
 
 // dst[i] += src[i] for n 64-bit words; dst[i] |= src[i] for n 32-bit words (merging

@@ -273,6 +273,12 @@ __global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ 
   const bool set = g < a.G && ((a.flags[g >> 5] >> (g & 31)) & 1u);
   const unsigned long long m = __ballot(set);
   if (lane == 0) cnt[wave] = (unsigned)__popcll(m);
+  // the padding behind the last row of every column (rows are written below it only)
+  if (a.total && blockIdx.x == 0 && threadIdx.x < kPadBytes) {
+    const unsigned long long n = *a.total;
+    for (int q = 0; q < a.K; ++q) ((char*)(a.keys_out[q] + n))[threadIdx.x] = 0;
+    for (int m2 = 0; m2 < a.M; ++m2) ((char*)(a.sums_out[m2] + n))[threadIdx.x] = 0;
+  }
   __syncthreads();
   if (!set) return;
   unsigned long long row = offsets[w0 / kBlock] + red[0] + red[1] + red[2] + red[3];
@@ -280,9 +286,15 @@ __global__ void k_group_emit(EmitArgs a, const unsigned long long* __restrict__ 
   row += (unsigned)__popcll(m & ((1ull << lane) - 1ull));
   for (int q = 0; q < a.K; ++q) {
     uint32_t code = (uint32_t)((g / a.stride[q]) % a.n[q]);
-    a.keys_out[q][row] = a.dict[q] ? a.dict[q][code] : a.fmin[q] + (int32_t)code;
+    const int32_t v = a.dict[q] ? a.dict[q][code] : a.fmin[q] + (int32_t)code;
+    a.keys_out[q][row] = v;
+    if (a.keys_host[q]) a.keys_host[q][row] = v;
   }
-  for (int m2 = 0; m2 < a.M; ++m2) a.sums_out[m2][row] = (int64_t)a.sums[g * (uint64_t)a.M + m2];
+  for (int m2 = 0; m2 < a.M; ++m2) {
+    const int64_t v = (int64_t)a.sums[g * (uint64_t)a.M + m2];
+    a.sums_out[m2][row] = v;
+    if (a.sums_host[m2]) a.sums_host[m2][row] = v;
+  }
 }
 
 struct ZeroRanges {
@@ -474,6 +486,17 @@ uint64_t group_count(const uint32_t* flags, uint64_t G, unsigned long long* bloc
   LAUNCH_CHECK("group_count");
   SVM_HIP(hipStreamSynchronize(s));
   return hs.host[2];
+}
+
+uint64_t group_count_async(const uint32_t* flags, uint64_t G, unsigned long long* blocks,
+                           unsigned long long* offsets, unsigned long long* count_out, hipStream_t s) {
+  const uint64_t nwords = (G + 31) / 32;
+  const uint64_t nb = (nwords + kBlock - 1) / kBlock;
+  if (!nb) fail(SVM_ERR_INVALID, "group_count_async: no groups");
+  k_group_popc<<<(unsigned)nb, kBlock, 0, s>>>(flags, nwords, blocks);
+  k_exscan<unsigned long long><<<1, 1024, 0, s>>>(blocks, nb, offsets, offsets + nb, count_out);
+  LAUNCH_CHECK("group_count_async");
+  return nb;
 }
 
 void group_emit(const EmitArgs& a, const unsigned long long* offsets, hipStream_t s) {

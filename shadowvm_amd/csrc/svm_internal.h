@@ -198,6 +198,14 @@ HostSlots host_slots(hipStream_t s);
 // a run owns it until it has synchronised).  Used to copy results out with one
 // synchronisation for all columns.
 void* host_staging(hipStream_t s, uint64_t bytes);
+// A stream's host-mapped, coherent pinned buffer of at least `bytes` bytes that
+// kernels write directly (host and device addresses of the same memory; grown on
+// demand; a run owns it until it has synchronised).
+struct MappedStaging {
+  char* host = nullptr;
+  char* dev = nullptr;
+};
+MappedStaging mapped_staging(hipStream_t s, uint64_t bytes);
 // Per-stream state of the one-launch ordered compaction (a tile ticket, which each
 // launch leaves at 0, and one status word per tile, tagged with a launch epoch so
 // the statuses are never cleared between launches): buf[0] is the ticket,
