@@ -179,6 +179,7 @@ int svm_table_drop(const char* id) {
   return guard([&] {
     need_id(id);
     if (!engine().store.drop(id)) fail(SVM_ERR_NOT_FOUND, std::string("no table '") + id + "'");
+    release_dead_derived();
   });
 }
 

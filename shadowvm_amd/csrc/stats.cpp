@@ -57,6 +57,27 @@ std::map<std::pair<const Table*, int>, std::shared_ptr<const Packed24>> g_packed
 
 }  // namespace
 
+namespace {
+template <typename E>
+bool dead(const E& e) { return e.owner.expired(); }
+template <typename E>
+bool dead(const std::shared_ptr<E>& e) { return !e || e->owner.expired(); }
+template <typename M>
+void sweep(M& m) {
+  for (auto it = m.begin(); it != m.end();) it = dead(it->second) ? m.erase(it) : std::next(it);
+}
+}  // namespace
+
+void release_dead_derived() {
+  std::lock_guard<std::mutex> g(g_cache_mu);
+  sweep(g_dicts);
+  sweep(g_keys);
+  sweep(g_kept);
+  sweep(g_tables);
+  sweep(g_clusters);
+  sweep(g_packed);
+}
+
 std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s) {
   if (t->base || t->row0 != 0 || col < 0 || col >= (int)t->cols.size() || t->cols[col].type != Int32 ||
       !t->cols[col].dev || !t->rows)

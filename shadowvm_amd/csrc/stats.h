@@ -44,6 +44,10 @@ struct Packed24 {
 };
 std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s);
 
+// Frees the cached facts (and their device memory: 24-bit copies, kept-row bytes, hash
+// tables) of tables that no longer exist.  Called when a client drops a table.
+void release_dead_derived();
+
 constexpr uint64_t kHashRate = 5;  // com/cpp/define.hpp:29
 
 // Rows of bulk 0 of `t` that the reference's join hash table on Int32 column `col`
