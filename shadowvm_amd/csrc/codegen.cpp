@@ -266,10 +266,10 @@ void emit_stage_maps(std::ostringstream& o, const KernelShape& s) {
   bool any = false;
   for (size_t j = 0; j < s.joins.size(); ++j) {
     if (s.joins[j].bits_lds) {
-      // the existence bits: ceil(range / 32) words after the packed map
+      // the existence bits after the packed map: range / 32 + 1 words (bit `range` is 0)
       o << "  { const unsigned* g = (const unsigned*)a.ptr[" << SVM_SLOT_MAP + j << "] + a.iv[" << 4 * j + 2 << "];\n"
         << "    unsigned* d = svm_lds + a.iv[" << 4 * j + 3 << "];\n"
-        << "    const int nw = (int)(((unsigned)a.iv[" << 4 * j + 1 << "] + 31u) >> 5);\n"
+        << "    const int nw = (int)(((unsigned)a.iv[" << 4 * j + 1 << "] >> 5) + 1u);\n"
         << "    for (int w = threadIdx.x; w < nw; w += SVM_BLOCK) d[w] = g[w]; }\n";
       continue;
     }
@@ -625,13 +625,20 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
           o << "    const svm_i32x4 x" << c << "_" << u << " = __builtin_nontemporal_load((const svm_i32x4*)a.ptr[" << c
             << "] + vc" << u << ");\n";
     }
-    for (size_t j = 0; j < s.joins.size(); ++j) o << "    unsigned pp" << j << "_" << u << "[" << R << "];\n";
   }
   o << "    unsigned svm_m = 0u;\n";
   const bool filt0 = s.filter != "true" && tz.filter == 0;
   for (int u = 0; u < U; ++u) {
-    o << "    #pragma unroll\n    for (int e = 0; e < " << R << "; ++e) {\n"
-      << "      const long long i = v" << u << "_ * " << R << " + e;\n      (void)i;\n";
+    // the group's rows inside [r_lo, r_hi): all of them except in the first and last
+    // groups of the range, so the rows are tested without a bound check each and the
+    // group's mask is trimmed once
+    o << "    unsigned svm_v" << u << ";\n"
+      << "    { const long long r0 = v" << u << "_ * " << R << ";\n"
+      << "      svm_v" << u << " = r0 >= r_lo && r0 + " << R << " <= r_hi ? " << (1u << R) - 1u << "u : 0u;\n"
+      << "      if (svm_v" << u << " == 0u)\n"
+      << "        for (int e = 0; e < " << R << "; ++e) svm_v" << u << " |= (unsigned)((r0 + e >= r_lo) & (r0 + e < r_hi)) << e; }\n";
+    o << "    unsigned svm_g" << u << " = 0u;\n"
+      << "    #pragma unroll\n    for (int e = 0; e < " << R << "; ++e) {\n";
     if (c0 >= 0) {
       // row e of the group: halfword e of the low plane, byte e of the high plane
       o << "      const unsigned svm_l = ((unsigned)lo" << u << "[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;\n"
@@ -641,11 +648,26 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
       for (int c : s.fact_i32)
         if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
     }
-    o << "      bool cond = (i >= r_lo) & (i < r_hi);\n";
-    if (filt0) o << "      cond = cond && (" << s.filter << ");\n";
-    for (size_t j = 0; j < s.joins.size(); ++j)
-      if (s.join_tier[j] == 0) emit_flat_join_test(o, s, j, u);
-    o << "      svm_m |= (cond ? 1u : 0u) << (" << R * u << " + e);\n    }\n";
+    // every row's tests run without branches: a key outside the map's range is clamped
+    // to slot `range`, whose existence bit is 0; a map lookup reads slot 0 instead
+    o << "      unsigned svm_b = 1u;\n";
+    if (filt0) o << "      svm_b &= (" << s.filter << ") ? 1u : 0u;\n";
+    for (size_t j = 0; j < s.joins.size(); ++j) {
+      if (s.join_tier[j] != 0) continue;
+      const JoinShape& J = s.joins[j];
+      if (J.defer) {
+        o << "      { const unsigned off = min((unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]), (unsigned)a.iv["
+          << 4 * j + 1 << "]);\n"
+          << "        svm_b &= svm_map_get<1>(mb" << j << ", off); }\n";
+      } else {
+        o << "      { const unsigned off = (unsigned)(f" << J.lcol << " - a.iv[" << 4 * j << "]);\n"
+          << "        const bool in = off < (unsigned)a.iv[" << 4 * j + 1 << "];\n"
+          << "        const unsigned p = svm_map_get<" << J.wbits << ">(m" << j << ", in ? off : 0u);\n"
+          << "        svm_b &= (in && p != 0u) ? 1u : 0u; }\n";
+      }
+    }
+    o << "      svm_g" << u << " |= svm_b << e;\n    }\n"
+      << "    svm_m |= (svm_g" << u << " & svm_v" << u << ") << " << R * u << ";\n";
   }
   o << "    const unsigned svm_c = (unsigned)__popc(svm_m);\n"
     << "    unsigned svm_in = svm_c;\n"

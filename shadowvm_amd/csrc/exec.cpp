@@ -368,7 +368,7 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
     jb.words = ((uint64_t)jb.range + per - 1) / per;
     // a large code map is probed through its existence bits (stored after it)
     jb.shape.defer = allow_defer && payload == PAY_CODE && jb.words * 4 > kDeferMapBytes;
-    const uint64_t bitwords = jb.shape.defer ? ((uint64_t)jb.range + 31) / 32 : 0;
+    const uint64_t bitwords = jb.shape.defer ? k::exist_words(jb.range) : 0;
     jb.packed = DevBuf((jb.words + bitwords) * 4 + 16, c.s);
     auto finish_map = [&](JoinBuild& b) {
       if (b.shape.defer)
@@ -494,7 +494,7 @@ void assign_lds(std::vector<JoinBuild>& joins, uint64_t agg_bytes, uint64_t lds_
   for (size_t j : order) {
     JoinBuild& jb = joins[j];
     if (!jb.words || jb.table) continue;
-    const uint64_t need = jb.shape.defer ? ((uint64_t)jb.range + 31) / 32 : jb.words;
+    const uint64_t need = jb.shape.defer ? k::exist_words(jb.range) : jb.words;
     if ((used + need) * 4 > kLdsBudget) continue;
     (jb.shape.defer ? jb.staged_bits : jb.staged_lds) = true;
     jb.staged_off = (uint32_t)used;

@@ -40,6 +40,9 @@ uint64_t ht_kept(const int32_t* keys, uint64_t n, uint64_t slots, uint8_t* kept,
 // bits[0 .. ceil(range/32)): bit s set when slot s of the packed map (wbits per slot)
 // holds a payload.
 void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bits, hipStream_t s);
+// Words of a dense map's existence bits over `range` slots: one bit per slot plus bit
+// `range`, always 0, so a lookup may clamp its slot to `range` instead of testing it.
+inline uint64_t exist_words(uint32_t range) { return (uint64_t)range / 32 + 1; }
 
 // The packed map of a dimension object with unique keys, built in one pass over its
 // rows (zeroes the map first).  payload 0: existence, 1: code + 1 of the row's value

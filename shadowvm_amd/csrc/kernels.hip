@@ -428,7 +428,7 @@ void first_rows(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, i
 }
 
 void exist_bits(const uint32_t* packed, int wbits, uint32_t range, uint32_t* bits, hipStream_t s) {
-  const uint64_t nbw = ((uint64_t)range + 31) / 32;
+  const uint64_t nbw = exist_words(range);
   if (!nbw) return;
   k_exist_bits<<<grid_for(nbw), kBlock, 0, s>>>(packed, wbits, range, bits, nbw);
   LAUNCH_CHECK("exist_bits");
@@ -440,7 +440,7 @@ void pack_direct(const int32_t* keys, uint64_t n, int32_t kmin, uint32_t range, 
   const uint64_t per = 32 / (uint64_t)wbits;
   const uint64_t words = ((uint64_t)range + per - 1) / per;
   // the existence bits, when kept, follow the packed words: both cleared at once
-  fill_u32(packed, words + (bits ? ((uint64_t)range + 31) / 32 : 0), 0u, s);
+  fill_u32(packed, words + (bits ? exist_words(range) : 0), 0u, s);
   if (n) k_pack_direct<<<grid_for(n), kBlock, 0, s>>>(keys, n, kmin, range, payload, (const char*)attr, atype,
                                                      asize, dict, nd, wbits, packed, bits);
   LAUNCH_CHECK("pack_direct");
