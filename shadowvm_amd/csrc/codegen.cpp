@@ -640,10 +640,13 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     o << "    unsigned svm_g" << u << " = 0u;\n"
       << "    #pragma unroll\n    for (int e = 0; e < " << R << "; ++e) {\n";
     if (c0 >= 0) {
-      // row e of the group: halfword e of the low plane, byte e of the high plane
-      o << "      const unsigned svm_l = ((unsigned)lo" << u << "[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;\n"
-        << "      const unsigned svm_h = ((unsigned)hi" << u << "[e >> 2] >> (8 * (e & 3))) & 0xFFu;\n"
-        << "      const int f" << c0 << " = (int)(svm_l | (svm_h << 16)) + a.iv[48];\n      (void)f" << c0 << ";\n";
+      // row e of the group: halfword e of the low plane, byte e of the high plane, put
+      // together by one byte permute ({high word, low word}: bytes 2e', 2e'+1 of the
+      // low word, byte e'' of the high word, then a zero byte)
+      o << "      const unsigned svm_sel = 0x0C000000u | ((4u + (unsigned)(e & 3)) << 16) | "
+           "((2u * (unsigned)(e & 1) + 1u) << 8) | (2u * (unsigned)(e & 1));\n"
+        << "      const int f" << c0 << " = (int)__builtin_amdgcn_perm((unsigned)hi" << u << "[e >> 2], (unsigned)lo" << u
+        << "[e >> 1], svm_sel) + a.iv[48];\n      (void)f" << c0 << ";\n";
     } else {
       for (int c : s.fact_i32)
         if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
