@@ -76,8 +76,8 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     later join, or, after the last tier, the metric and remaining key columns) is read
     in the `line`-byte L2 lines that hold a row passing every join of the earlier
     tiers: HBM returns whole lines, so one passing row costs its line.  A tier-0
-    column read from its 24-bit copy (`packed24(c)` in the configuration) costs
-    1,536 bytes per 512 rows.  The pass masks come from the host copy of the tables
+    column read from its narrow copy (`packed<b>(c)` in the configuration, b = 16, 18
+    or 24 bits a row) costs 1,024 + 64 x (b - 16) bytes per 512 rows.  The pass masks come from the host copy of the tables
     and the dimension results."""
     import re
     import numpy as np
@@ -89,8 +89,9 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     if not m:
         return nlines * line * ncols, {"model": "every column streamed", "line_bytes": line}
     tiers = [int(x) for x in (m.group(1) or m.group(2)).split(",")]
-    pm = re.search(r"packed24\(([0-9,]+)\)", config or "")
-    packed = {int(x) for x in pm.group(1).split(",")} if pm else set()
+    pm = re.search(r"packed(16|18|24)\(([0-9,]+)\)", config or "")
+    packed = {int(x) for x in pm.group(2).split(",")} if pm else set()
+    pbits = int(pm.group(1)) if pm else 24
     fq, src, refs = wl.fact
     last = max(tiers) + 1
     # join j probes fact column j (orderdate, custkey, suppkey, partkey) against the
@@ -123,15 +124,15 @@ def scan_bytes_model(config, fact_cols, dims, wl, line=128):
     for c in range(ncols):
         t = col_tier[c]
         if t == 0 and c in packed:
-            b = (rows + 511) // 512 * 1536
+            b = (rows + 511) // 512 * (1024 + 64 * (pbits - 16))
             total += b
-            detail.append({"column": c, "tier": t, "bytes": b, "copy": "24-bit"})
+            detail.append({"column": c, "tier": t, "bytes": b, "copy": f"{pbits}-bit"})
             continue
         if t not in cache:
             cache[t] = lines_passing(t)
         total += cache[t] * line
         detail.append({"column": c, "tier": t, "lines": cache[t]})
-    return total, {"model": "tier-0 columns streamed (a 24-bit copy at 3 bytes a row); later columns by "
+    return total, {"model": "tier-0 columns streamed (a narrow copy at 16, 18 or 24 bits a row); later columns by "
                             "the 128-byte lines holding a row that passed the earlier tiers",
                    "line_bytes": line, "tiers": tiers, "columns": detail}
 

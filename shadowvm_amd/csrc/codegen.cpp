@@ -572,9 +572,10 @@ void emit_tiered(std::ostringstream& o, const KernelShape& s, const Tiers& tz, i
 // rows (filter and every join again, for the payloads) and aggregates them, one per
 // lane.
 void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, int U) {
-  // The one tier-0 column read from its 24-bit copy: a lane streams groups of 8 rows
-  // (a 128-bit load of their low 16 bits and a 64-bit load of their high bytes, both
-  // coalesced over the wave's 512 rows) instead of int4 groups of 4 rows.
+  // The one tier-0 column read from its narrow copy: a lane streams groups of 8 rows
+  // (a 128-bit load of their low 16 bits and, for a range beyond 16 bits, a 16-bit or
+  // 64-bit load of their high bits, both coalesced over the wave's 512 rows) instead
+  // of int4 groups of 4 rows.
   int c0 = -1;
   {
     std::vector<int> t0;
@@ -582,7 +583,7 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
       if (tz.col.at(c) == 0) t0.push_back(c);
     if (t0.size() == 1 && !s.packed.empty() && s.packed[0] == t0[0]) c0 = t0[0];
   }
-  const int R = c0 >= 0 ? 8 : 4, RS = c0 >= 0 ? 3 : 2;
+  const int R = c0 >= 0 ? 8 : 4, RS = c0 >= 0 ? 3 : 2, HB = s.pack_high;
   if (R * U > 32) U = 32 / R;  // a lane's rows are the bits of one 32-bit mask
   // a later tier's gather of fact column c at row i
   auto gather = [&](int c) { return "((const int*)a.ptr[" + I(c) + "])[i]"; };
@@ -612,13 +613,18 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     o << "    const long long v" << u << "_ = vb + " << u << "ll * SVM_BLOCK + threadIdx.x;\n"
       << "    const long long vc" << u << " = v" << u << "_ < g_hi ? v" << u << "_ : g_lo;\n";
     if (c0 >= 0) {
-      // group g of 8 rows: block g / 64 of 1,536 bytes, lane slot g % 64 of each plane
-      o << "    const char* svm_pb" << u << " = (const char*)a.ptr[" << SVM_SLOT_PACK << "] + 1536ll * (vc" << u
-        << " >> 6);\n"
+      // group g of 8 rows: block g / 64 of 1,024 + 64 x HB bytes, lane slot g % 64 of
+      // each plane
+      o << "    const char* svm_pb" << u << " = (const char*)a.ptr[" << SVM_SLOT_PACK << "] + " << 1024 + 64 * HB
+        << "ll * (vc" << u << " >> 6);\n"
         << "    const svm_i32x4 lo" << u << " = __builtin_nontemporal_load((const svm_i32x4*)svm_pb" << u << " + (vc" << u
-        << " & 63));\n"
-        << "    const svm_i32x2 hi" << u << " = __builtin_nontemporal_load((const svm_i32x2*)(svm_pb" << u
-        << " + 1024) + (vc" << u << " & 63));\n";
+        << " & 63));\n";
+      if (HB == 8)
+        o << "    const svm_i32x2 hi" << u << " = __builtin_nontemporal_load((const svm_i32x2*)(svm_pb" << u
+          << " + 1024) + (vc" << u << " & 63));\n";
+      else if (HB == 2)
+        o << "    const unsigned hi" << u << " = (unsigned)__builtin_nontemporal_load((const unsigned short*)(svm_pb" << u
+          << " + 1024) + (vc" << u << " & 63));\n";
     } else {
       for (int c : s.fact_i32)
         if (tz.col.at(c) == 0)
@@ -640,13 +646,22 @@ void emit_compact(std::ostringstream& o, const KernelShape& s, const Tiers& tz, 
     o << "    unsigned svm_g" << u << " = 0u;\n"
       << "    #pragma unroll\n    for (int e = 0; e < " << R << "; ++e) {\n";
     if (c0 >= 0) {
-      // row e of the group: halfword e of the low plane, byte e of the high plane, put
-      // together by one byte permute ({high word, low word}: bytes 2e', 2e'+1 of the
-      // low word, byte e'' of the high word, then a zero byte)
-      o << "      const unsigned svm_sel = 0x0C000000u | ((4u + (unsigned)(e & 3)) << 16) | "
-           "((2u * (unsigned)(e & 1) + 1u) << 8) | (2u * (unsigned)(e & 1));\n"
-        << "      const int f" << c0 << " = (int)__builtin_amdgcn_perm((unsigned)hi" << u << "[e >> 2], (unsigned)lo" << u
-        << "[e >> 1], svm_sel) + a.iv[48];\n      (void)f" << c0 << ";\n";
+      if (HB == 8) {
+        // row e of the group: halfword e of the low plane, byte e of the high plane, put
+        // together by one byte permute ({high word, low word}: bytes 2e', 2e'+1 of the
+        // low word, byte e'' of the high word, then a zero byte)
+        o << "      const unsigned svm_sel = 0x0C000000u | ((4u + (unsigned)(e & 3)) << 16) | "
+             "((2u * (unsigned)(e & 1) + 1u) << 8) | (2u * (unsigned)(e & 1));\n"
+          << "      const int f" << c0 << " = (int)__builtin_amdgcn_perm((unsigned)hi" << u << "[e >> 2], (unsigned)lo" << u
+          << "[e >> 1], svm_sel) + a.iv[48];\n      (void)f" << c0 << ";\n";
+      } else {
+        // row e of the group: halfword e of the low plane (a byte permute with zero
+        // bytes above it) and, for HB = 2, bits 2e, 2e + 1 of the high word as bits 16, 17
+        o << "      const unsigned svm_sel = 0x0C0C0000u | ((2u * (unsigned)(e & 1) + 1u) << 8) | (2u * (unsigned)(e & 1));\n"
+          << "      const int f" << c0 << " = (int)(__builtin_amdgcn_perm(0u, (unsigned)lo" << u << "[e >> 1], svm_sel)";
+        if (HB == 2) o << " | (((hi" << u << " >> (2 * e)) & 3u) << 16)";
+        o << ") + a.iv[48];\n      (void)f" << c0 << ";\n";
+      }
     } else {
       for (int c : s.fact_i32)
         if (tz.col.at(c) == 0) o << "      const int f" << c << " = x" << c << "_" << u << "[e];\n      (void)f" << c << ";\n";
@@ -930,7 +945,7 @@ namespace {
 void put(std::ostringstream& k, const std::string& v) { k << v.size() << ':' << v; }
 
 std::string shape_key(const KernelShape& s, char kind, const char* entry) {
-  static_assert(sizeof(KernelShape) == 360 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
+  static_assert(sizeof(KernelShape) == 368 && sizeof(JoinShape) == 36 && sizeof(KeyShape) == 16,
                 "a field was added to the kernel shape: add it to shape_key");
   std::ostringstream k;
   k << kind << entry << '|' << s.block << ',' << s.unroll << ',' << s.prefetch << s.flat << ',' << s.agg << ','
@@ -959,6 +974,7 @@ std::string shape_key(const KernelShape& s, char kind, const char* entry) {
   for (int t : s.join_tier) k << t << ',';
   k << '|' << s.compact << '|';
   for (int c : s.packed) k << c << ',';
+  k << '|' << s.pack_high;
   return k.str();
 }
 

@@ -82,10 +82,12 @@ struct KernelShape {
   // aggregation run row by row from 4-byte gathers
   bool compact = false;
   // compacting form: the one tier-0 column, when it is packed[0], is streamed from
-  // its 24-bit copy (a.ptr[SVM_SLOT_PACK], minimum a.iv[48]; blocks of 512 rows as
-  // 16-bit and 8-bit planes): a lane reads 8 rows per 128-bit plus 64-bit load,
-  // `unroll` groups of 8 rows per iteration.  Later tiers gather the Int32 columns.
+  // its narrow copy (a.ptr[SVM_SLOT_PACK], minimum a.iv[48]; blocks of 512 rows as a
+  // 16-bit plane and a plane of pack_high = 0, 2 or 8 high bits a row): a lane reads 8
+  // rows per 128-bit load plus one load of 8 x pack_high bits, `unroll` groups of 8
+  // rows per iteration.  Later tiers gather the Int32 columns.
   std::vector<int> packed;
+  int pack_high = 8;
   // select
   std::vector<std::string> proj;                      // lowered projection expressions
   std::vector<std::pair<int, long long>> proj_type;   // (ValType, val_size) of outputs
@@ -106,7 +108,7 @@ std::shared_ptr<const GeneratedSource> select_source(const KernelShape& s);
 std::string gen_scan_source(const KernelShape& s, const char* entry = "svm_scan");
 std::string gen_select_source(const KernelShape& s);
 
-// The fact column the compacting form of `s` can stream from a 24-bit copy: its one
+// The fact column the compacting form of `s` can stream from a narrow copy: its one
 // tier-0 column, when it has one and the filter is applied later; else empty.
 std::vector<int> compact_packable(const KernelShape& s);
 // C++ source of the host engine's task function for the same pipeline shape:

@@ -625,7 +625,7 @@ struct TuneChoice {
   bool flat = false;                      // branch-free filter + joins for a lane's four rows
   int late = 0;                           // staged column loads (flat form): 1 one tier, 2 chained tiers,
                                           // 3 chained tiers with the passing rows compacted per wave
-  bool packed = false;                    // late 3: the tier-0 key column read from its 24-bit copy
+  bool packed = false;                    // late 3: the tier-0 key column read from its narrow copy
                                           // (unroll counts groups of 8 rows)
 };
 struct TuneState {
@@ -663,8 +663,8 @@ const TuneChoice kTuneCandidates[] = {
     {512, 2, 0, true, false, true, 3},  {512, 4, 0, true, false, true, 3},  {1024, 4, 0, true, false, true, 3},
     {256, 8, 0, true, false, true, 3},  {512, 8, 0, true, false, true, 3},  {1024, 2, 0, true, false, true, 3},
     {1024, 5, 0, true, false, true, 3}, {512, 6, 0, true, false, true, 3},  {512, 7, 0, true, false, true, 3},
-    // the same with the tier-0 key column streamed from its 24-bit copy (3 bytes a row:
-    // 8 rows per 128-bit and 64-bit load)
+    // the same with the tier-0 key column streamed from its narrow copy (2, 2.25 or 3
+    // bytes a row: 8 rows per 128-bit load and one load of their high bits)
     {256, 4, 0, true, false, true, 3, true},  {512, 2, 0, true, false, true, 3, true},
     {512, 4, 0, true, false, true, 3, true},  {1024, 2, 0, true, false, true, 3, true},
     {1024, 4, 0, true, false, true, 3, true}, {512, 1, 0, true, false, true, 3, true},
@@ -955,9 +955,9 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   const uint64_t agg_words = (agg_bytes + 7) / 8 * 2;
   // (a staged scan uses the staged LDS layout; `layout_args` puts its offsets into a
   // copy of the launch arguments)
-  // the 24-bit copy of the tier-0 key column (of the table that holds the column, rows
+  // the narrow copy of the tier-0 key column (of the table that holds the column, rows
   // indexed as the scan indexes them), by column
-  std::map<int, std::shared_ptr<const Packed24>> packs;
+  std::map<int, std::shared_ptr<const PackedKeys>> packs;
   const TablePtr col_owner = c.src->base ? c.src->base : c.src;
   auto shape_for = [&](const TuneChoice& t, KernelShape& k2, uint64_t& words) {
     k2 = ks;
@@ -971,14 +971,15 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     k2.packed.clear();
     if (t.packed && k2.compact) {
       for (int col : compact_packable(k2)) {
-        std::shared_ptr<const Packed24> pk;
+        std::shared_ptr<const PackedKeys> pk;
         if (col_owner->cols[col].dev == c.src->cols[col].dev) {
           auto it = packs.find(col);
-          pk = it != packs.end() ? it->second : column_packed24(col_owner, col, c.s);
+          pk = it != packs.end() ? it->second : column_packed_keys(col_owner, col, c.s);
         }
         if (!pk) break;
         packs[col] = pk;
         k2.packed.push_back(col);
+        k2.pack_high = pk->high_bits;
       }
       if (k2.packed.empty()) k2.unroll = t.unroll * 2;  // the same rows per lane, as int4 groups
     }
@@ -995,7 +996,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
       for (size_t j = 0; j < joins.size(); ++j)
         if (!joins[j].table) a2.iv[4 * j + 3] = (int)joins[j].staged_off;
     if (!k2.packed.empty()) {
-      const Packed24& pk = *packs.at(k2.packed[0]);
+      const PackedKeys& pk = *packs.at(k2.packed[0]);
       a2.ptr[SVM_SLOT_PACK] = pk.bytes.p;
       a2.iv[48] = pk.kmin;
     }
@@ -1039,7 +1040,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
         const TuneChoice& cand = kTuneCandidates[ci];
         KernelShape k2;
         uint64_t w2 = 0;
-        shape_for(cand, k2, w2);  // (a packed candidate without a 24-bit tier-0 column runs as its int4 twin)
+        shape_for(cand, k2, w2);  // (a packed candidate without a narrow tier-0 copy runs as its int4 twin)
         const auto g2src = scan_source(k2, "svm_scan_tune");
         hipFunction_t f2 = jit_function(g2src->text, g2src->hash, "svm_scan_tune");
         unsigned g2 = scan_grid(c.E, f2, vec, k2.block, k2.unroll, w2 * 4, cand.bpc);
@@ -1097,7 +1098,7 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
     for (size_t j = 0; j < ks.join_tier.size(); ++j) res->tune += (j ? "," : "") + std::to_string(ks.join_tier[j]);
     res->tune += ")";
     if (!ks.packed.empty()) {
-      res->tune += " packed24(";
+      res->tune += " packed" + std::to_string(16 + ks.pack_high) + "(";
       for (size_t k = 0; k < ks.packed.size(); ++k) res->tune += (k ? "," : "") + std::to_string(ks.packed[k]);
       res->tune += ")";
     }

@@ -8,14 +8,14 @@
 //   24..31 packed join payload maps, by join position
 //   32..47 dimension columns read by expressions, assigned by the generator
 //   48..50 per-dimension-row key code columns (group keys read through a row id)
-//   51     the 24-bit copy of the tier-0 key column (KernelShape::packed)
+//   51     the narrow copy of the tier-0 key column (KernelShape::packed)
 //   54..63 projected output columns of a select pipeline
 // out[]: 0 group sums (int64), 1 group flags (bit per group), 2 scalar sums,
 //        6 select tile offsets (u64), 7 select tile counts (u32)
 // iv[]: 4j+0 kmin, 4j+1 key range, 4j+2 map words, 4j+3 LDS word offset of join j;
 //       32+4q+0 fact-key minimum, 32+4q+1 stride, 32+4q+2 cardinality of key q;
 //       44 group count, 45 LDS word offset of the aggregation table;
-//       48 the minimum added back to the values of the 24-bit copy
+//       48 the minimum added back to the values of the narrow copy
 #define SVM_KARGS_DEF                                                                     \
   struct SvmArgs {                                                                        \
     const void* ptr[64];                                                                  \

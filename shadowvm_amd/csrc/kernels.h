@@ -15,11 +15,11 @@ namespace k {
 void fill_u32(void* p, uint64_t nwords, uint32_t v, hipStream_t s);
 // out[0] = min, out[1] = max of col[0..n) (out[0..1] are initialised here).
 void minmax_i32(const int32_t* col, uint64_t n, int32_t* out, hipStream_t s);
-// 24-bit copy of col (values in [kmin, kmin + 2^24)) in blocks of 512 rows: row i's
-// value - kmin has its low 16 bits at uint16 i % 512 and its high 8 bits at byte
-// 1024 + i % 512 of block i / 512 (1,536 bytes per block).  Writes every row's bytes;
-// the unused rows of a partial last block are left as they were.
-void pack24(const int32_t* col, uint64_t n, int32_t kmin, uint8_t* out, hipStream_t s);
+// Narrow copy of col (values in [kmin, kmin + 2^(16 + hb)), hb = 0, 2 or 8) in blocks
+// of 512 rows (1,024 + 64 x hb bytes each): row i's value - kmin has its low 16 bits at
+// uint16 i % 512 and its hb high bits at bit (i % 512) x hb of byte 1024 of block i / 512.
+// Writes every row's bits; the unused rows of a partial last block are left as they were.
+void pack_keys(const int32_t* col, uint64_t n, int32_t kmin, int hb, uint8_t* out, hipStream_t s);
 // Group-key value of each dimension row: the int value, or Hash(bytes, size) for strings.
 void values_of(const void* col, int type, int64_t size, uint64_t n, int32_t* vals, hipStream_t s);
 // Sorted distinct values of vals[0..n) into dict; returns the count (synchronises).

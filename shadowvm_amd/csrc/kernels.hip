@@ -53,12 +53,16 @@ __global__ void k_minmax(const int32_t* __restrict__ col, uint64_t n, int32_t* o
   }
 }
 
-// 24-bit copy of an Int32 column, in blocks of 512 rows: the low 16 bits of the
-// block's values v - kmin (< 2^24) as 512 uint16, then their high 8 bits as 512 bytes
-// (1,536 bytes per block).  A wave of a scan then reads 512 rows as one coalesced
-// 1 KB load and one coalesced 512-byte load.  One thread per 8 rows: two 128-bit
-// loads, one 128-bit and one 64-bit store.
-__global__ void k_pack24(const int32_t* __restrict__ col, uint64_t n, int32_t kmin, uint8_t* __restrict__ out) {
+// Narrow copy of an Int32 column, in blocks of 512 rows: the low 16 bits of the
+// block's values v - kmin (< 2^(16 + HB)) as 512 uint16, then their HB high bits
+// (HB = 0, 2 or 8) packed row by row (1,024 + 64 x HB bytes per block).  A wave of a
+// scan then reads 512 rows as one coalesced 1 KB load and one coalesced load of 64 x
+// HB bytes.  One thread per 8 rows: two 128-bit loads, one 128-bit store and one store
+// of its 8 x HB high bits.
+template <int HB>
+__global__ void k_pack_keys(const int32_t* __restrict__ col, uint64_t n, int32_t kmin, uint8_t* __restrict__ out) {
+  constexpr uint64_t kBlockBytes = 1024 + 64 * HB;
+  constexpr unsigned kMask = (1u << (16 + HB)) - 1u;
   const uint64_t groups = (n + 7) / 8;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += (uint64_t)gridDim.x * blockDim.x) {
     const int4* src = (const int4*)col + 2 * g;  // the column padding covers a partial last group
@@ -66,13 +70,20 @@ __global__ void k_pack24(const int32_t* __restrict__ col, uint64_t n, int32_t km
     const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     unsigned x[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = 8 * g + (uint64_t)j < n ? (unsigned)(v[j] - kmin) & 0xFFFFFFu : 0u;
-    uint8_t* blk = out + 1536ull * (g >> 6);
+    for (int j = 0; j < 8; ++j) x[j] = 8 * g + (uint64_t)j < n ? (unsigned)(v[j] - kmin) & kMask : 0u;
+    uint8_t* blk = out + kBlockBytes * (g >> 6);
     *(uint4*)(blk + 16 * (g & 63)) = make_uint4((x[0] & 0xFFFFu) | (x[1] << 16), (x[2] & 0xFFFFu) | (x[3] << 16),
                                                 (x[4] & 0xFFFFu) | (x[5] << 16), (x[6] & 0xFFFFu) | (x[7] << 16));
-    *(uint2*)(blk + 1024 + 8 * (g & 63)) =
-        make_uint2((x[0] >> 16) | (x[1] >> 16 << 8) | (x[2] >> 16 << 16) | (x[3] >> 16 << 24),
-                   (x[4] >> 16) | (x[5] >> 16 << 8) | (x[6] >> 16 << 16) | (x[7] >> 16 << 24));
+    if (HB == 8) {
+      *(uint2*)(blk + 1024 + 8 * (g & 63)) =
+          make_uint2((x[0] >> 16) | (x[1] >> 16 << 8) | (x[2] >> 16 << 16) | (x[3] >> 16 << 24),
+                     (x[4] >> 16) | (x[5] >> 16 << 8) | (x[6] >> 16 << 16) | (x[7] >> 16 << 24));
+    } else if (HB == 2) {
+      unsigned h = 0u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) h |= (x[j] >> 16) << (2 * j);
+      *(uint16_t*)(blk + 1024 + 2 * (g & 63)) = (uint16_t)h;
+    }
   }
 }
 
@@ -377,10 +388,13 @@ void fill_u32(void* p, uint64_t nwords, uint32_t v, hipStream_t s) {
   LAUNCH_CHECK("fill_u32");
 }
 
-void pack24(const int32_t* col, uint64_t n, int32_t kmin, uint8_t* out, hipStream_t s) {
+void pack_keys(const int32_t* col, uint64_t n, int32_t kmin, int hb, uint8_t* out, hipStream_t s) {
   if (!n) return;
-  k_pack24<<<grid_for((n + 7) / 8, kBlock, 4096), kBlock, 0, s>>>(col, n, kmin, out);
-  LAUNCH_CHECK("pack24");
+  const auto grid = grid_for((n + 7) / 8, kBlock, 4096);
+  if (hb == 8) k_pack_keys<8><<<grid, kBlock, 0, s>>>(col, n, kmin, out);
+  else if (hb == 2) k_pack_keys<2><<<grid, kBlock, 0, s>>>(col, n, kmin, out);
+  else k_pack_keys<0><<<grid, kBlock, 0, s>>>(col, n, kmin, out);
+  LAUNCH_CHECK("pack_keys");
 }
 
 void minmax_i32(const int32_t* col, uint64_t n, int32_t* out, hipStream_t s) {

@@ -99,11 +99,17 @@ std::vector<std::pair<std::string, std::string>> selftest_sources() {
     s.prefetch = false;
     s.unroll = 4;
     out.push_back({"scan_compacted", scan_checked(s)});
-    // its tier-0 key column streamed from a 24-bit copy, four groups of 8 rows a lane
+    // its tier-0 key column streamed from a narrow copy (24, 18 and 16 bits a row),
+    // four groups of 8 rows a lane
     s.packed = compact_packable(s);
     if (s.packed.size() != 1 || s.packed[0] != 1) fail(SVM_ERR_JIT, "compact_packable: unexpected columns");
     s.unroll = 4;
     out.push_back({"scan_compacted_packed", scan_checked(s)});
+    s.pack_high = 2;
+    out.push_back({"scan_compacted_packed18", scan_checked(s)});
+    s.pack_high = 0;
+    out.push_back({"scan_compacted_packed16", scan_checked(s)});
+    s.pack_high = 8;
     s.packed.clear();
     s.compact = false;
     s.join_tier.clear();

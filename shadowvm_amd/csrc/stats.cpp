@@ -6,6 +6,9 @@
 // for a new table or when svm_table_put_column has rewritten the table's columns.
 #include "stats.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace svm {
@@ -53,7 +56,7 @@ bool current(const E& e, const TablePtr& t) {
   return e.owner.lock() == t && e.version == t->data_version();
 }
 std::map<std::tuple<const Table*, int, uint64_t>, ClusterEntry> g_clusters;
-std::map<std::pair<const Table*, int>, std::shared_ptr<const Packed24>> g_packed;
+std::map<std::pair<const Table*, int>, std::shared_ptr<const PackedKeys>> g_packed;
 
 }  // namespace
 
@@ -78,31 +81,37 @@ void release_dead_derived() {
   sweep(g_packed);
 }
 
-std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s) {
+std::shared_ptr<const PackedKeys> column_packed_keys(const TablePtr& t, int col, hipStream_t s) {
   if (t->base || t->row0 != 0 || col < 0 || col >= (int)t->cols.size() || t->cols[col].type != Int32 ||
       !t->cols[col].dev || !t->rows)
     return nullptr;
   const auto key = std::make_pair(t.get(), col);
+  // a high plane of at least `forced` bits (a cached copy narrower than that is rebuilt)
+  const char* fe = getenv("SVM_PACK_HIGH_BITS");
+  const int forced = fe ? (atoi(fe) >= 8 ? 8 : atoi(fe) >= 2 ? 2 : 0) : 0;
   {
     std::lock_guard<std::mutex> g(g_cache_mu);
     auto it = g_packed.find(key);
-    if (it != g_packed.end() && current(*it->second, t)) return it->second;
+    if (it != g_packed.end() && current(*it->second, t) && it->second->high_bits >= forced) return it->second;
   }
   // the key range is cached: a column too wide is told apart without a kernel
   const KeyEntry ke = table_key(t, col, s);
-  if ((int64_t)ke.kmax - (int64_t)ke.kmin >= (int64_t)(1 << 24)) return nullptr;
-  noalloc_refuse("a 24-bit copy of a column");
-  std::shared_ptr<Packed24> p;
+  const int64_t span = (int64_t)ke.kmax - (int64_t)ke.kmin;
+  if (span >= (int64_t)(1 << 24)) return nullptr;
+  const int hb = std::max(span < (1 << 16) ? 0 : span < (1 << 18) ? 2 : 8, forced);
+  noalloc_refuse("a narrow copy of a column");
+  std::shared_ptr<PackedKeys> p;
   {
-    p = std::make_shared<Packed24>();
+    p = std::make_shared<PackedKeys>();
     p->owner = t;
     p->version = t->data_version();
     p->kmin = ke.kmin;
-    const uint64_t blocks = (t->rows + 511) / 512;
-    p->bytes = DevBuf(blocks * 1536 + 256);
+    p->high_bits = hb;
+    const uint64_t blocks = (t->rows + 511) / 512, bb = 1024 + 64 * (uint64_t)hb;
+    p->bytes = DevBuf(blocks * bb + 256);
     // the last block and the padding are zeroed first (a partial block's unused rows)
-    k::fill_u32(p->bytes.as<char>() + (blocks - 1) * 1536, (1536 + 256) / 4, 0u, s);
-    k::pack24((const int32_t*)t->col_ptr(col), t->rows, ke.kmin, p->bytes.as<uint8_t>(), s);
+    k::fill_u32(p->bytes.as<char>() + (blocks - 1) * bb, (bb + 256) / 4, 0u, s);
+    k::pack_keys((const int32_t*)t->col_ptr(col), t->rows, ke.kmin, hb, p->bytes.as<uint8_t>(), s);
     SVM_HIP(hipStreamSynchronize(s));
   }
   std::lock_guard<std::mutex> g(g_cache_mu);

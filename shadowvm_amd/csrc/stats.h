@@ -30,21 +30,25 @@ std::pair<TablePtr, int> column_root(const TablePtr& t, int col);
 std::shared_ptr<DictEntry> table_dict(const TablePtr& t, int col, hipStream_t s);
 KeyEntry table_key(const TablePtr& t, int col, hipStream_t s);
 
-// A 24-bit copy of Int32 column `col` of `t` (a table that owns its columns, row 0
+// A narrow copy of Int32 column `col` of `t` (a table that owns its columns, row 0
 // at the start of the allocation), in blocks of 512 rows: row i's value - kmin has its
-// low 16 bits at uint16 i % 512 and its high 8 bits at byte 1024 + i % 512 of block
-// i / 512 (1,536 bytes per block).  The compacting scans stream their tier-0 key
+// low 16 bits at uint16 i % 512 of block i / 512 and its `high_bits` high bits (0, 2
+// or 8: the fewest that hold the column's range) in a plane after them, row i % 512 at
+// bit (i % 512) * high_bits of byte offset 1024 (1,024 + 64 x high_bits bytes per
+// block: 16, 18 or 24 bits a row).  The compacting scans stream their tier-0 key
 // column from it.  Null when the column's range needs more than 24 bits.  Built on
-// first use, cached with the table.
-struct Packed24 {
+// first use, cached with the table.  SVM_PACK_HIGH_BITS=2 or 8 widens the high plane
+// to at least that many bits.
+struct PackedKeys {
   std::weak_ptr<Table> owner;
   uint64_t version = 0;
   int32_t kmin = 0;
+  int high_bits = 8;
   DevBuf bytes;
 };
-std::shared_ptr<const Packed24> column_packed24(const TablePtr& t, int col, hipStream_t s);
+std::shared_ptr<const PackedKeys> column_packed_keys(const TablePtr& t, int col, hipStream_t s);
 
-// Frees the cached facts (and their device memory: 24-bit copies, kept-row bytes, hash
+// Frees the cached facts (and their device memory: narrow key copies, kept-row bytes, hash
 // tables) of tables that no longer exist.  Called when a client drops a table.
 void release_dead_derived();
 

@@ -44,6 +44,16 @@ def test_packed_tier0_column_costs_three_bytes_a_row():
     assert detail["columns"][1] == {"column": 1, "tier": 0, "bytes": 3072, "copy": "24-bit"}
 
 
+def test_narrower_copies_cost_their_bits():
+    cols, dims, wl = _case()
+    # 18 bits a row: 1,024 + 128 bytes per block of 512 rows; 16 bits: 1,024
+    total, detail = bench.scan_bytes_model("flat compacted-tiers(-1,0,1,-1) packed18(1)", cols, dims, wl)
+    assert total == 2 * 1152 + 2 * 128 + 4 * 128
+    assert detail["columns"][1]["copy"] == "18-bit"
+    total, _ = bench.scan_bytes_model("flat compacted-tiers(-1,0,1,-1) packed16(1)", cols, dims, wl)
+    assert total == 2 * 1024 + 2 * 128 + 4 * 128
+
+
 def test_unstaged_streams_every_column():
     cols, dims, wl = _case()
     total, _ = bench.scan_bytes_model("256x1 hash", cols, dims, wl)

@@ -159,8 +159,8 @@ def test_generated_kernels_compile_for_gfx950(tmp_path, monkeypatch):
     rc = L.svm_debug_jit_selftest(buf, len(buf))
     report = buf.value.decode()
     assert rc == 0, report
-    assert report.count(": ok") == 18
-    assert len([f for f in os.listdir(tmp_path) if f.endswith(".co")]) == 18  # code objects cached per source
+    assert report.count(": ok") == 20
+    assert len([f for f in os.listdir(tmp_path) if f.endswith(".co")]) == 20  # code objects cached per source
     # the host engine's code of five shapes, compiled by the host compiler and loaded
     assert report.count(": host ok") == 5
     assert len([f for f in os.listdir(tmp_path) if f.startswith("host_") and f.endswith(".so")]) == 5

@@ -85,6 +85,42 @@ def test_every_scan_variant_matches_oracle(engine, monkeypatch, name):
         P.table_drop(t.id)
 
 
+# kTuneCandidates' last ten stream the tier-0 key column from its narrow copy
+PACKED_CANDIDATES = range(N_SCAN_CANDIDATES - 10, N_SCAN_CANDIDATES)
+
+
+def scan_config(pipeline_id):
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    f = L.svm_debug_scan_config
+    f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    buf = ctypes.create_string_buffer(4096)
+    assert f(pipeline_id.encode(), buf, len(buf)) == 0
+    return buf.value.decode()
+
+
+def test_packed_candidates_at_every_copy_width(engine, monkeypatch):
+    """The narrow copy of the tier-0 key column holds 16, 18 or 24 bits a row (the
+    fewest its range needs, or at least SVM_PACK_HIGH_BITS high bits): every packed
+    candidate is checked against the oracle at each width.  Both filtering keys' ranges
+    fit 16 bits, so the forced widths are the ones used."""
+    wl = ssb.four_way(seed=29, fact_rows=4_000_005, n_cust=60_000, n_supp=20_000, n_part=140_000, prefix="vn")
+    print(f"[variants] narrow copies: oracle on {wl.fact_rows} rows", flush=True)
+    want, _ = oracle_query(wl, host_tables(wl))
+    ssb.load_device(wl)
+    ssb.run_dims(wl)
+    pid = wl.fact[0].pipe_info["id"]
+    for high, bits in (("0", 16), ("2", 18), ("8", 24)):
+        monkeypatch.setenv("SVM_PACK_HIGH_BITS", high)
+        for v in PACKED_CANDIDATES:
+            monkeypatch.setenv("SVM_SCAN_VARIANT", str(v))
+            print(f"[variants] narrow copies: {bits} bits, svm_scan candidate {v}", flush=True)
+            _same(ssb.run_fact(wl), want)
+            cfg = scan_config(pid)
+            assert f"packed{bits}(" in cfg, cfg
+    for t in wl.tables:
+        P.table_drop(t.id)
+
+
 @pytest.fixture(scope="module")
 def batch_case(engine):
     wl = ssb.four_way(seed=23, fact_rows=6_000_000, n_cust=300_000, n_supp=20_000, n_part=140_000, prefix="vb")
