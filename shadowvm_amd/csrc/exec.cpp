@@ -30,6 +30,10 @@ namespace svm {
 namespace {
 
 constexpr uint64_t kMaxDenseRange = 1ull << 28;  // keys per dense join map
+// Beyond this many keys a dense map must hold at least one dimension row per
+// kMaxDenseSpread keys of its range, else the join probes the hash table.
+constexpr uint64_t kSparseDenseFloor = 1ull << 24;
+constexpr uint64_t kMaxDenseSpread = 64;
 constexpr uint64_t kMaxGroups = 1ull << 27;
 constexpr uint64_t kMaxFactKeyRange = 1ull << 24;
 constexpr uint64_t kDenseAggBytes = 16 * 1024;  // LDS for a dense per-workgroup table
@@ -294,7 +298,10 @@ std::vector<JoinBuild> build_joins(Ctx& c, const std::set<std::pair<int, int>>& 
       ke = table_key(root.first, root.second, c.s);
       range = (uint64_t)((int64_t)ke.kmax - (int64_t)ke.kmin + 1);
     }
-    const bool probe = rt.first != Int32 || range > kMaxDenseRange;
+    // (a sparse key range too: a dense map wider than 2^24 slots holding fewer than one
+    // key in 64 would cost far more memory than the hash table's kHashRate slots a row)
+    const bool probe = rt.first != Int32 || range > kMaxDenseRange ||
+                       (range > kSparseDenseFloor && range / kMaxDenseSpread > (n0 ? n0 : 1));
     if ((root.first != Dp || root.second != R.col) && root.first->bulk0_rows() > 0)
       jb.sel = std::min(1.0, (double)n0 / (double)root.first->bulk0_rows());
     // payload selection
@@ -1093,6 +1100,13 @@ std::unique_ptr<svm_result> run_reduce(Ctx& c, const RunOptions& opt) {
   }
   res->tune = std::to_string(ks.block) + "x" + std::to_string(ks.unroll) + (choice.stage ? " lds-maps" : " l2-maps") +
               (choice.prefetch ? " prefetch" : "") + (choice.flat ? " flat" : "");
+  {
+    // joins that probe the reference's hash table instead of a dense map
+    std::string pj;
+    for (size_t j = 0; j < joins.size(); ++j)
+      if (joins[j].table) pj += (pj.empty() ? "" : ",") + std::to_string(j);
+    if (!pj.empty()) res->tune += " probe(" + pj + ")";
+  }
   if (!ks.join_tier.empty()) {
     res->tune += choice.late == 3 ? " compacted-tiers(" : choice.late == 2 ? " chained-tiers(" : " staged(";
     for (size_t j = 0; j < ks.join_tier.size(); ++j) res->tune += (j ? "," : "") + std::to_string(ks.join_tier[j]);

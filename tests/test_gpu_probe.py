@@ -123,6 +123,31 @@ def _keyed_case(prefix, ktype, ksize, keys, seed, n_fact=400_000):
                 ([ktype, I32], [ksize, 4], [fk, val]), tabs, 3_000_000)
 
 
+def scan_config(pipeline_id):
+    import ctypes
+    from shadowvm_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    f = L.svm_debug_scan_config
+    f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    buf = ctypes.create_string_buffer(4096)
+    assert f(pipeline_id.encode(), buf, len(buf)) == 0
+    return buf.value.decode()
+
+
+def test_sparse_int32_keys_probe(engine):
+    """20,000 Int32 keys spread over 2^26 (one key in 3,355 of the range): a dense map
+    would need 2^26 slots, so the join probes the reference's hash table, and the
+    answer is the oracle's."""
+    rng = np.random.default_rng(15)
+    keys = np.unique(rng.integers(0, 2**26, 20_000, dtype=np.int64)).astype(np.int32)
+    got, want = _keyed_case("pks", I32, 4, rng.permutation(keys), 16)
+    cfg = scan_config("pks_q")
+    print(f"[probe] sparse int32: {got.rows} groups, scan {cfg}", flush=True)
+    assert "probe(0)" in cfg, cfg
+    assert got.rows == want.rows == 30
+    assert_same_columns(got.columns, want.columns)
+
+
 @pytest.mark.parametrize("case", ["int64", "int16", "int32_wide"])
 def test_typed_key_probes(engine, case):
     rng = np.random.default_rng(11)
